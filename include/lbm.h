@@ -1,0 +1,152 @@
+/*
+ * lbm.h -- C ABI of the gfx950 D2Q9 lattice-Boltzmann engine (liblbm.so).
+ *
+ * This is the drop-in boundary for the reference's kernel entry points.  The reference launches CUDA
+ * kernels through CuPy (cp.RawModule(...).get_function(name)(grid, block, args), lbm_smemaa_main.py:84-86,
+ * 141-142); here each kernel becomes an extern "C" function taking plain device pointers, sizes and a
+ * HIP stream, and the Python driver (lbm-cuda_amd/lbm) binds them with ctypes.
+ *
+ * Conventions (shared by every entry point):
+ *   - Populations are SoA FP32 planes f[q][y][x], x fastest, plane stride W*H (lbm_core4.cu:26-27,40;
+ *     lbm_smemaa_main.py:35,43).  q follows the reference's D2Q9 order: c_q = (0,0) (1,0) (0,1) (-1,0)
+ *     (0,-1) (1,1) (-1,1) (-1,-1) (1,-1) (lbm_core4.cu:1-3).
+ *   - mask[y][x] is one byte per cell, non-zero = solid (the reference's bool mask, lbm_smemaa_main.py:68).
+ *     In channel mode rows 0 and H-1 must be solid and columns 0 and W-1 open on rows 1..H-2
+ *     (lbm_smemaa_main.py:61-66); the Python driver enforces this.
+ *   - inlet_u[y] is the Zou-He inlet speed per row (the reference computes u_in * ramp(y) inline,
+ *     lbm_core4.cu:46-51; the driver builds the ramp or a parabola on the host in FP64).
+ *   - AA parity: 0 = canonical (slot q at x holds the incoming f_q(x)); 1 = reversed (slot opp(q) at x
+ *     holds the post-collision f*_q(x)).  This is the A/B convention of lbm_core_smemaa.cu:92-106,274-292
+ *     applied to the HBM state.
+ *   - Pointers are device addresses owned by the caller; nothing here allocates.  Calls are asynchronous
+ *     on `stream` (NULL = default stream).  Every function returns 0 on success, otherwise a negative
+ *     library code or a positive hipError_t, and lbm_last_error() describes the last failure of the
+ *     calling thread.
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *lbm_stream_t; /* == hipStream_t */
+
+enum lbm_model {
+    LBM_MRT_LES = 0, /* MRT + Smagorinsky with the reference clamps (lbm_core4.cu:124-210) */
+    LBM_BGK = 1      /* BGK of the temporally blocked family (lbm_core_smemaa.cu:194-220)  */
+};
+
+enum lbm_bc {
+    LBM_BC_CHANNEL = 0, /* wall rows, Zou-He inlet at x=0, zero-gradient outlet ghost at x=W-1 */
+    LBM_BC_PERIODIC = 1 /* both directions periodic; mask cells are bounce-back obstacles      */
+};
+
+enum lbm_status {
+    LBM_OK = 0,
+    LBM_ERR_ARG = -1,      /* invalid argument (see lbm_last_error) */
+    LBM_ERR_SIZE = -2,     /* lattice too large / too small          */
+    LBM_ERR_UNSUPPORTED = -3
+};
+
+/* Physical and numerical parameters.  The reference hard-codes Cs = 0.2 (lbm_core4.cu:123), the MRT
+ * rates si = 1.1 (lbm_core4.cu:6) and the clamps (lbm_core4.cu:142-144); they are fields here. */
+typedef struct lbm_params {
+    int32_t W, H;   /* lattice size (totwidth, totheight) */
+    int32_t model;  /* enum lbm_model */
+    int32_t bc;     /* enum lbm_bc */
+    int32_t clamp;  /* 1: |u| <= 0.4 and rho in [0,1.5] before MRT collision (populations are always
+                       clamped to [0,10], lbm_core4.cu:203, lbm_core_smemaa.cu:220) */
+    float tau_inv;  /* relaxation rate of the stress moments (MRT) / BGK rate */
+    float cs;       /* Smagorinsky constant; 0 = plain MRT (no LES) */
+    float s_e, s_eps, s_q; /* MRT rates of e, eps and q_x/q_y (reference: 1.1) */
+} lbm_params;
+
+/* ABI version of this header (bumped on any signature change). */
+int lbm_abi_version(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char *lbm_last_error(void);
+
+/* Wait for all work on `stream`; returns the first asynchronous error, if any. */
+int lbm_sync(lbm_stream_t stream);
+
+/*
+ * One in-place AA collide-stream step of the whole lattice.
+ * Replaces the reference's per-step kernel (fused_lbmkernel of lbm_core4.cu:7-13) with a single buffer:
+ * parity 0 collides locally and stores reversed; parity 1 gathers from neighbours, collides and
+ * scatters back to canonical (bounce-back on solid sources/targets, x=0 Zou-He, W-1 zero-gradient ghost).
+ * ux/uy/rho may all be NULL (no macro output) or all non-NULL (written for every updated cell, as
+ * lbm_core4.cu:59-61,145-147 does).
+ */
+int lbm_step_aa(const lbm_params *p, const uint8_t *mask, float *f, const float *inlet_u, int parity,
+                float *ux, float *uy, float *rho, lbm_stream_t stream);
+
+/*
+ * nsteps consecutive lbm_step_aa launches on `stream` (parities parity, parity^1, ...), issued from C so
+ * that a long run costs no host-language overhead per step.  The state ends at parity (parity+nsteps)&1.
+ * Replaces the reference's replay loop of a captured 2-launch graph (lbm_mine2.py:73-94).
+ */
+int lbm_step_aa_n(const lbm_params *p, const uint8_t *mask, float *f, const float *inlet_u, int parity, int nsteps,
+                  lbm_stream_t stream);
+
+/*
+ * Reference-compatible two-buffer pull step: fused_lbmkernel of lbm_core4.cu:7-13
+ * (mask, f_now, f_out, ux, uy, rho, totwidth, totheight, tau_inv, u_in, tau_inv_bnd).  f_out receives
+ * post-collision populations, the W-1 column a copy of W-2, and ux/uy/rho are always written.
+ * `tau_inv_bnd` is unused by the reference (lbm_core4.cu:12) and is therefore not a parameter;
+ * u_in becomes the per-row inlet_u array.
+ */
+int lbm_step_pull(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out, float *ux,
+                  float *uy, float *rho, const float *inlet_u, lbm_stream_t stream);
+
+/*
+ * k collide-stream steps in one launch, temporally blocked in LDS (the fused_lbmkernel of
+ * lbm_core_smemaa.cu:12-15 / lbm_core_smemaa_endfused.cu:12-14, with T = k chosen at run time).
+ * f_now (parity `parity`) is read, f_out receives the state after k steps (parity (parity + k) & 1);
+ * f_now and f_out must not overlap.  The result is bit-identical to k calls of lbm_step_aa.
+ * 1 <= k <= lbm_fused_max_k().  ux/uy/rho: NULL or the macros of the last step's collision.
+ */
+int lbm_step_fused(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out,
+                   const float *inlet_u, int parity, int k, float *ux, float *uy, float *rho,
+                   lbm_stream_t stream);
+
+/* Largest k accepted by lbm_step_fused. */
+int lbm_fused_max_k(void);
+
+/*
+ * rho, ux, uy of a stored state (macro_kernel of lbm_core_smemaa_endfused.cu:377-379, extended to both
+ * parities): canonical states get Zou-He at x=0; reversed states are read through opp().  Only updated
+ * cells are written.
+ */
+int lbm_macro(const lbm_params *p, const uint8_t *mask, const float *f, const float *inlet_u, int parity,
+              float *ux, float *uy, float *rho, lbm_stream_t stream);
+
+/*
+ * Vorticity colour map into RGBA8 (visualizekernel of lbm_core4.cu:214-217): interior cells only,
+ * solids black, vort = ((uy[x+1]-uy[x-1]) - (ux[y+1]-ux[y-1])) * vort_scale.  rho is unused, as in the
+ * reference, and therefore not a parameter.
+ */
+int lbm_visualize(const lbm_params *p, const float *ux, const float *uy, const uint8_t *mask, uint8_t *rgba,
+                  float vort_scale, lbm_stream_t stream);
+
+/*
+ * FP64 diagnostics of a stored state, reduced deterministically on the GPU (no reference counterpart:
+ * the reference keeps no diagnostics; the build needs them for drag/lift and conservation checks).
+ * out[0] = mass, out[1..2] = momentum (same cells and Zou-He completion as lbm_macro), out[3..4] =
+ * momentum-exchange force 2 sum c_q f*_q over links from updated cells to interior solid cells
+ * (wall rows excluded).  `work` is a device buffer of lbm_diagnostics_work_size(p) bytes; `out` is a
+ * device buffer of 5 doubles.
+ */
+size_t lbm_diagnostics_work_size(const lbm_params *p);
+int lbm_diagnostics(const lbm_params *p, const uint8_t *mask, const float *f, const float *inlet_u, int parity,
+                    double *work, double *out, lbm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_H */

@@ -1,0 +1,541 @@
+// Single-step gfx950 kernels of liblbm and their C-ABI entry points:
+//   lbm_step_aa    in-place AA step (the plain-method hot path)
+//   lbm_step_pull  reference-compatible two-buffer pull step (lbm_core4.cu:7-212)
+//   lbm_macro      rho/u extraction (lbm_core_smemaa_endfused.cu:377-422)
+//   lbm_visualize  vorticity colour map (lbm_core4.cu:214-255)
+//   lbm_diagnostics deterministic FP64 mass / momentum / momentum-exchange force
+//
+// Layout: SoA planes f[q*plane + y*W + x]; a 256-thread block covers 64 x 4 cells, so each wave64 reads
+// and writes one 256-byte row segment per plane (coalesced).  Odd (B) steps read and write the same 9
+// locations per cell (x - c_q / x + c_q pairs), which is what makes the single-buffer update race free.
+#include "lbm_common.h"
+
+#include <cstring>
+
+namespace lbm {
+
+static thread_local char g_err[512];
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail((int)e, "%s: launch failed: %s", what, hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int make_kpar(const lbm_params *p, KPar *k)
+{
+    if (!p) return fail(LBM_ERR_ARG, "params is NULL");
+    if (p->W < 3 || p->H < 3) return fail(LBM_ERR_SIZE, "lattice %dx%d too small (need W, H >= 3)", p->W, p->H);
+    // element offsets q*plane + idx are kept in 32 bits
+    if ((long long)p->W * p->H * 9 >= (1LL << 31))
+        return fail(LBM_ERR_SIZE, "lattice %dx%d too large (9*W*H must be < 2^31)", p->W, p->H);
+    if (p->model != LBM_MRT_LES && p->model != LBM_BGK) return fail(LBM_ERR_ARG, "unknown model %d", p->model);
+    if (p->bc != LBM_BC_CHANNEL && p->bc != LBM_BC_PERIODIC) return fail(LBM_ERR_ARG, "unknown bc %d", p->bc);
+    if (!(p->tau_inv > 0.0f && p->tau_inv < 2.0f)) return fail(LBM_ERR_ARG, "tau_inv %g outside (0,2)", p->tau_inv);
+    if (!(p->cs >= 0.0f)) return fail(LBM_ERR_ARG, "cs %g must be >= 0", p->cs);
+    k->W = p->W;
+    k->H = p->H;
+    k->plane = p->W * p->H;
+    k->les = (p->model == LBM_MRT_LES && p->cs != 0.0f) ? 1 : 0;
+    k->clamp = p->clamp ? 1 : 0;
+    k->tau_inv = p->tau_inv;
+    volatile float one = 1.0f; // keep the host division in FP32, as the device would do it
+    k->tau0 = one / p->tau_inv;
+    k->tau0sq = k->tau0 * k->tau0;
+    k->c18cs2 = (18.0f * p->cs) * p->cs;
+    k->s_e = p->s_e;
+    k->s_eps = p->s_eps;
+    k->s_q = p->s_q;
+    return LBM_OK;
+}
+
+// ------------------------------------------------------------------------------------------------------
+// in-place AA step
+// ------------------------------------------------------------------------------------------------------
+
+// Index of the neighbour x + c_q (wrapped when periodic; only called for in-domain neighbours otherwise).
+template <bool PERIODIC>
+__device__ __forceinline__ int nbr_index(const KPar &P, int x, int y, int q)
+{
+    int nx = x + cx(q), ny = y + cy(q);
+    if constexpr (PERIODIC) {
+        nx = nx < 0 ? nx + P.W : (nx >= P.W ? nx - P.W : nx);
+        ny = ny < 0 ? ny + P.H : (ny >= P.H ? ny - P.H : ny);
+    }
+    return ny * P.W + nx;
+}
+
+template <int MODEL, int PARITY, bool PERIODIC, bool MACROS>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__restrict__ mask, float *__restrict__ f,
+                                                        const float *__restrict__ inlet_u, float *__restrict__ oux,
+                                                        float *__restrict__ ouy, float *__restrict__ orho)
+{
+    const int x = blockIdx.x * TILE_X + threadIdx.x;
+    const int y = blockIdx.y * TILE_Y + threadIdx.y;
+    if (x >= P.W || y >= P.H) return;
+    const int W = P.W, plane = P.plane;
+    const int idx = y * W + x;
+    bool inlet = false;
+    if constexpr (!PERIODIC) {
+        // wall rows and the outlet ghost are never updated (lbm_core4.cu:20-24)
+        if (y == 0 || y == P.H - 1 || x == W - 1) return;
+        inlet = x == 0;
+        if (!inlet && mask[idx]) return;
+    } else {
+        if (mask[idx]) return;
+    }
+    float g[9], mac[3];
+    int nb[9];
+    uint8_t solid[9];
+    if constexpr (PARITY == 0) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = f[q * plane + idx];
+    } else {
+        nb[0] = idx;
+        solid[0] = 0;
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            if (!PERIODIC && inlet && cx(q) < 0) { // x = -1 does not exist
+                nb[q] = idx;
+                solid[q] = 0;
+            } else {
+                nb[q] = nbr_index<PERIODIC>(P, x, y, q);
+                solid[q] = mask[nb[q]];
+            }
+        }
+        g[0] = f[idx];
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            const int o = opp(q);
+            if (!PERIODIC && inlet && cx(q) > 0)
+                g[q] = 0.0f; // arrives from x = -1; rebuilt by Zou-He
+            else
+                g[q] = solid[o] ? f[q * plane + idx] : f[o * plane + nb[o]];
+        }
+    }
+    collide<MODEL>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
+    if constexpr (PARITY == 0) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) f[opp(q) * plane + idx] = g[q];
+        if (!PERIODIC && x == W - 2) { // reversed copy into the outlet ghost (lbm_core4.cu:114-116)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) f[opp(q) * plane + idx + 1] = g[q];
+        }
+    } else {
+        f[idx] = g[0];
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            if (!PERIODIC && inlet && cx(q) < 0) continue; // leaves through the open inlet
+            if (solid[q])
+                f[opp(q) * plane + idx] = g[q];
+            else
+                f[q * plane + nb[q]] = g[q];
+        }
+        if (!PERIODIC && x == W - 2) {
+            // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
+            f[3 * plane + idx] = g[3];
+            if (!solid[2]) f[6 * plane + nb[2]] = g[6];
+            if (!solid[4]) f[7 * plane + nb[4]] = g[7];
+        }
+    }
+    if constexpr (MACROS) {
+        orho[idx] = mac[0];
+        oux[idx] = mac[1];
+        ouy[idx] = mac[2];
+    }
+}
+
+template <int MODEL, int PARITY, bool PERIODIC, bool MACROS>
+static void launch_aa(const KPar &P, dim3 grid, hipStream_t s, const uint8_t *mask, float *f, const float *inlet_u,
+                      float *ux, float *uy, float *rho)
+{
+    hipLaunchKernelGGL((k_aa<MODEL, PARITY, PERIODIC, MACROS>), grid, dim3(TILE_X, TILE_Y), 0, s, P, mask, f, inlet_u, ux,
+                       uy, rho);
+}
+
+template <int MODEL, int PARITY, bool PERIODIC>
+static void dispatch_aa_m(const KPar &P, dim3 grid, hipStream_t s, const uint8_t *mask, float *f, const float *inlet_u,
+                          float *ux, float *uy, float *rho)
+{
+    if (rho)
+        launch_aa<MODEL, PARITY, PERIODIC, true>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+    else
+        launch_aa<MODEL, PARITY, PERIODIC, false>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+}
+
+template <int MODEL, int PARITY>
+static void dispatch_aa_b(const KPar &P, int bc, dim3 grid, hipStream_t s, const uint8_t *mask, float *f,
+                          const float *inlet_u, float *ux, float *uy, float *rho)
+{
+    if (bc == LBM_BC_PERIODIC)
+        dispatch_aa_m<MODEL, PARITY, true>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+    else
+        dispatch_aa_m<MODEL, PARITY, false>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// reference-compatible pull step (lbm_core4.cu:120-211)
+// ------------------------------------------------------------------------------------------------------
+
+template <int MODEL, bool PERIODIC>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_pull(KPar P, const uint8_t *__restrict__ mask,
+                                                          const float *__restrict__ f_now, float *__restrict__ f_out,
+                                                          const float *__restrict__ inlet_u, float *__restrict__ oux,
+                                                          float *__restrict__ ouy, float *__restrict__ orho)
+{
+    const int x = blockIdx.x * TILE_X + threadIdx.x;
+    const int y = blockIdx.y * TILE_Y + threadIdx.y;
+    if (x >= P.W || y >= P.H) return;
+    const int W = P.W, plane = P.plane;
+    const int idx = y * W + x;
+    bool inlet = false;
+    if constexpr (!PERIODIC) {
+        if (y == 0 || y == P.H - 1 || x == W - 1) return;
+        inlet = x == 0;
+        if (!inlet && mask[idx]) return;
+    } else {
+        if (mask[idx]) return;
+    }
+    float g[9], mac[3];
+    g[0] = f_now[idx];
+#pragma unroll
+    for (int q = 1; q < 9; ++q) {
+        if (!PERIODIC && inlet && cx(q) > 0) {
+            g[q] = 0.0f; // lbm_core4.cu:36-37
+        } else {
+            const int s = nbr_index<PERIODIC>(P, x, y, opp(q)); // x - c_q
+            g[q] = mask[s] ? f_now[opp(q) * plane + idx] : f_now[q * plane + s];
+        }
+    }
+    collide<MODEL>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        f_out[q * plane + idx] = g[q];
+        if (!PERIODIC && x == W - 2) f_out[q * plane + idx + 1] = g[q];
+    }
+    orho[idx] = mac[0];
+    oux[idx] = mac[1];
+    ouy[idx] = mac[2];
+}
+
+// ------------------------------------------------------------------------------------------------------
+// macro extraction (lbm_core_smemaa_endfused.cu:377-422), both parities
+// ------------------------------------------------------------------------------------------------------
+
+template <bool PERIODIC>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_macro(KPar P, const uint8_t *__restrict__ mask,
+                                                           const float *__restrict__ f, const float *__restrict__ inlet_u,
+                                                           int parity, float *__restrict__ oux,
+                                                           float *__restrict__ ouy, float *__restrict__ orho)
+{
+    const int x = blockIdx.x * TILE_X + threadIdx.x;
+    const int y = blockIdx.y * TILE_Y + threadIdx.y;
+    if (x >= P.W || y >= P.H) return;
+    const int idx = y * P.W + x;
+    bool inlet = false;
+    if constexpr (!PERIODIC) {
+        if (y == 0 || y == P.H - 1 || x == P.W - 1) return;
+        inlet = x == 0;
+        if (!inlet && mask[idx]) return;
+    } else {
+        if (mask[idx]) return;
+    }
+    float g[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] = f[(parity ? opp(q) : q) * P.plane + idx];
+    float rho, ux, uy;
+    if (inlet && parity == 0) {
+        const float u = inlet_u[y];
+        rho = __fdiv_rn(fma_(2.0f, (g[3] + g[6]) + g[7], (g[0] + g[2]) + g[4]), 1.0f - u);
+        ux = u;
+        uy = 0.0f;
+    } else {
+        const float s13 = g[1] + g[3], s24 = g[2] + g[4], s57 = g[5] + g[7], s68 = g[6] + g[8];
+        const float a13 = g[1] - g[3], a24 = g[2] - g[4], a57 = g[5] - g[7], a86 = g[8] - g[6];
+        rho = (g[0] + (s13 + s24)) + (s57 + s68);
+        ux = __fdiv_rn(a13 + (a57 + a86), rho);
+        uy = __fdiv_rn(a24 + (a57 - a86), rho);
+    }
+    orho[idx] = rho;
+    oux[idx] = ux;
+    ouy[idx] = uy;
+}
+
+// ------------------------------------------------------------------------------------------------------
+// vorticity colour map (lbm_core4.cu:214-255)
+// ------------------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_visualize(int W, int H, const float *__restrict__ ux,
+                                                               const float *__restrict__ uy,
+                                                               const uint8_t *__restrict__ mask,
+                                                               uint8_t *__restrict__ image, float vort_scale)
+{
+    const int x = blockIdx.x * TILE_X + threadIdx.x;
+    const int y = blockIdx.y * TILE_Y + threadIdx.y;
+    if (x <= 0 || x >= W - 1 || y <= 0 || y >= H - 1) return;
+    const int pid = y * W + x;
+    image[pid * 4 + 3] = 255;
+    if (mask[pid]) {
+        image[pid * 4 + 0] = 0;
+        image[pid * 4 + 1] = 0;
+        image[pid * 4 + 2] = 0;
+        return;
+    }
+    const float vort = ((uy[pid + 1] - uy[pid - 1]) - (ux[pid + W] - ux[pid - W])) * vort_scale;
+    const float r = fminf(fmaxf(1.0f + vort, 0.0f), 1.0f);
+    const float g = fminf(fmaxf(1.0f - fabsf(vort), 0.0f), 1.0f);
+    const float b = fminf(fmaxf(1.0f - vort, 0.0f), 1.0f);
+    image[pid * 4 + 0] = (uint8_t)(r * 255.0f);
+    image[pid * 4 + 1] = (uint8_t)(g * 255.0f);
+    image[pid * 4 + 2] = (uint8_t)(b * 255.0f);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// FP64 diagnostics: fixed grid, fixed-order tree reductions -> bitwise reproducible
+// ------------------------------------------------------------------------------------------------------
+
+constexpr int DIAG_BLOCKS = 1024;
+constexpr int DIAG_THREADS = 256;
+constexpr int DIAG_N = 5;
+
+template <bool PERIODIC>
+__global__ __launch_bounds__(DIAG_THREADS) void k_diag_partial(KPar P, const uint8_t *__restrict__ mask,
+                                                                const float *__restrict__ f,
+                                                                const float *__restrict__ inlet_u, int parity,
+                                                                double *__restrict__ work)
+{
+    __shared__ double red[DIAG_N][DIAG_THREADS];
+    double acc[DIAG_N] = {0, 0, 0, 0, 0};
+    const int ncell = P.plane;
+    for (int i = blockIdx.x * DIAG_THREADS + threadIdx.x; i < ncell; i += DIAG_BLOCKS * DIAG_THREADS) {
+        const int y = i / P.W, x = i - y * P.W;
+        bool inlet = false;
+        if constexpr (!PERIODIC) {
+            if (y == 0 || y == P.H - 1 || x == P.W - 1) continue;
+            inlet = x == 0;
+            if (!inlet && mask[i]) continue;
+        } else {
+            if (mask[i]) continue;
+        }
+        double g[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = (double)f[(parity ? opp(q) : q) * P.plane + i];
+        if (inlet && parity == 0) {
+            const double u = (double)inlet_u[y];
+            const double rho = (g[0] + g[2] + g[4] + 2.0 * (g[3] + g[6] + g[7])) / (1.0 - u);
+            acc[0] += rho;
+            acc[1] += rho * u;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                acc[0] += g[q];
+                acc[1] += g[q] * cx(q);
+                acc[2] += g[q] * cy(q);
+            }
+        }
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            int nx = x + cx(q), ny = y + cy(q);
+            if constexpr (PERIODIC) {
+                nx = nx < 0 ? nx + P.W : (nx >= P.W ? nx - P.W : nx);
+                ny = ny < 0 ? ny + P.H : (ny >= P.H ? ny - P.H : ny);
+            } else {
+                if (nx < 0 || ny <= 0 || ny >= P.H - 1) continue; // outside, or a wall row
+            }
+            if (!mask[ny * P.W + nx]) continue;
+            const double fs = (double)f[opp(q) * P.plane + i];
+            acc[3] += 2.0 * cx(q) * fs;
+            acc[4] += 2.0 * cy(q) * fs;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < DIAG_N; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = DIAG_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int c = 0; c < DIAG_N; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < DIAG_N) work[blockIdx.x * DIAG_N + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(DIAG_THREADS) void k_diag_final(const double *__restrict__ work, double *__restrict__ out)
+{
+    __shared__ double red[DIAG_N][DIAG_THREADS];
+    double acc[DIAG_N] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < DIAG_BLOCKS; b += DIAG_THREADS) {
+#pragma unroll
+        for (int c = 0; c < DIAG_N; ++c) acc[c] += work[b * DIAG_N + c];
+    }
+#pragma unroll
+    for (int c = 0; c < DIAG_N; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = DIAG_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int c = 0; c < DIAG_N; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < DIAG_N) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+static dim3 step_grid(const KPar &P) { return dim3((P.W + TILE_X - 1) / TILE_X, (P.H + TILE_Y - 1) / TILE_Y); }
+
+} // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+int lbm_abi_version(void) { return 1; }
+
+const char *lbm_last_error(void) { return g_err; }
+
+int lbm_sync(lbm_stream_t stream)
+{
+    hipError_t e = hipStreamSynchronize(to_hip(stream));
+    if (e != hipSuccess) return fail((int)e, "stream synchronize: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_step_aa(const lbm_params *p, const uint8_t *mask, float *f, const float *inlet_u, int parity, float *ux,
+                float *uy, float *rho, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P);
+    if (rc) return rc;
+    if (!mask || !f) return fail(LBM_ERR_ARG, "lbm_step_aa: mask and f must be non-NULL");
+    if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_step_aa: channel mode needs inlet_u");
+    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_step_aa: parity must be 0 or 1");
+    if ((ux == nullptr) != (rho == nullptr) || (uy == nullptr) != (rho == nullptr))
+        return fail(LBM_ERR_ARG, "lbm_step_aa: ux, uy, rho must be all NULL or all non-NULL");
+    const dim3 grid = step_grid(P);
+    hipStream_t s = to_hip(stream);
+    if (p->model == LBM_BGK) {
+        if (parity == 0)
+            dispatch_aa_b<MODEL_BGK, 0>(P, p->bc, grid, s, mask, f, inlet_u, ux, uy, rho);
+        else
+            dispatch_aa_b<MODEL_BGK, 1>(P, p->bc, grid, s, mask, f, inlet_u, ux, uy, rho);
+    } else {
+        if (parity == 0)
+            dispatch_aa_b<MODEL_MRT, 0>(P, p->bc, grid, s, mask, f, inlet_u, ux, uy, rho);
+        else
+            dispatch_aa_b<MODEL_MRT, 1>(P, p->bc, grid, s, mask, f, inlet_u, ux, uy, rho);
+    }
+    return check_launch("lbm_step_aa");
+}
+
+int lbm_step_aa_n(const lbm_params *p, const uint8_t *mask, float *f, const float *inlet_u, int parity, int nsteps,
+                  lbm_stream_t stream)
+{
+    if (nsteps < 0) return fail(LBM_ERR_ARG, "lbm_step_aa_n: nsteps must be >= 0");
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = lbm_step_aa(p, mask, f, inlet_u, (parity + i) & 1, nullptr, nullptr, nullptr, stream);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
+int lbm_step_pull(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out, float *ux, float *uy,
+                  float *rho, const float *inlet_u, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P);
+    if (rc) return rc;
+    if (!mask || !f_now || !f_out || !ux || !uy || !rho)
+        return fail(LBM_ERR_ARG, "lbm_step_pull: mask, f_now, f_out, ux, uy, rho must be non-NULL");
+    if (f_now == f_out) return fail(LBM_ERR_ARG, "lbm_step_pull: f_now and f_out must differ");
+    if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_step_pull: channel mode needs inlet_u");
+    const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);
+    hipStream_t s = to_hip(stream);
+    const bool per = p->bc == LBM_BC_PERIODIC;
+    if (p->model == LBM_BGK) {
+        if (per)
+            hipLaunchKernelGGL((k_pull<MODEL_BGK, true>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy, rho);
+        else
+            hipLaunchKernelGGL((k_pull<MODEL_BGK, false>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy, rho);
+    } else {
+        if (per)
+            hipLaunchKernelGGL((k_pull<MODEL_MRT, true>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy, rho);
+        else
+            hipLaunchKernelGGL((k_pull<MODEL_MRT, false>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy, rho);
+    }
+    return check_launch("lbm_step_pull");
+}
+
+int lbm_macro(const lbm_params *p, const uint8_t *mask, const float *f, const float *inlet_u, int parity, float *ux,
+              float *uy, float *rho, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P);
+    if (rc) return rc;
+    if (!mask || !f || !ux || !uy || !rho) return fail(LBM_ERR_ARG, "lbm_macro: NULL pointer");
+    if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_macro: channel mode needs inlet_u");
+    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_macro: parity must be 0 or 1");
+    const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);
+    hipStream_t s = to_hip(stream);
+    if (p->bc == LBM_BC_PERIODIC)
+        hipLaunchKernelGGL((k_macro<true>), grid, block, 0, s, P, mask, f, inlet_u, parity, ux, uy, rho);
+    else
+        hipLaunchKernelGGL((k_macro<false>), grid, block, 0, s, P, mask, f, inlet_u, parity, ux, uy, rho);
+    return check_launch("lbm_macro");
+}
+
+int lbm_visualize(const lbm_params *p, const float *ux, const float *uy, const uint8_t *mask, uint8_t *rgba,
+                  float vort_scale, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P);
+    if (rc) return rc;
+    if (!ux || !uy || !mask || !rgba) return fail(LBM_ERR_ARG, "lbm_visualize: NULL pointer");
+    const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);
+    hipLaunchKernelGGL(k_visualize, grid, block, 0, to_hip(stream), P.W, P.H, ux, uy, mask, rgba, vort_scale);
+    return check_launch("lbm_visualize");
+}
+
+size_t lbm_diagnostics_work_size(const lbm_params *) { return sizeof(double) * DIAG_BLOCKS * DIAG_N; }
+
+int lbm_diagnostics(const lbm_params *p, const uint8_t *mask, const float *f, const float *inlet_u, int parity,
+                    double *work, double *out, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P);
+    if (rc) return rc;
+    if (!mask || !f || !work || !out) return fail(LBM_ERR_ARG, "lbm_diagnostics: NULL pointer");
+    if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_diagnostics: channel mode needs inlet_u");
+    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_diagnostics: parity must be 0 or 1");
+    hipStream_t s = to_hip(stream);
+    if (p->bc == LBM_BC_PERIODIC)
+        hipLaunchKernelGGL((k_diag_partial<true>), dim3(DIAG_BLOCKS), dim3(DIAG_THREADS), 0, s, P, mask, f, inlet_u, parity,
+                           work);
+    else
+        hipLaunchKernelGGL((k_diag_partial<false>), dim3(DIAG_BLOCKS), dim3(DIAG_THREADS), 0, s, P, mask, f, inlet_u,
+                           parity, work);
+    rc = check_launch("lbm_diagnostics partial");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(DIAG_THREADS), 0, s, work, out);
+    return check_launch("lbm_diagnostics final");
+}
+
+} // extern "C"

@@ -1,0 +1,92 @@
+"""ctypes binding of liblbm.so (declared in include/lbm.h).
+
+The library is built in-tree by ``lbm.build`` (hipcc, gfx950).  There is no fallback: if the shared
+library is missing or a symbol is absent, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm.so")
+
+MRT_LES, BGK = 0, 1
+CHANNEL, PERIODIC = 0, 1
+
+ABI_VERSION = 1
+
+
+class LbmParams(ctypes.Structure):
+    """Mirror of ``lbm_params`` (include/lbm.h)."""
+
+    _fields_ = [
+        ("W", ctypes.c_int32),
+        ("H", ctypes.c_int32),
+        ("model", ctypes.c_int32),
+        ("bc", ctypes.c_int32),
+        ("clamp", ctypes.c_int32),
+        ("tau_inv", ctypes.c_float),
+        ("cs", ctypes.c_float),
+        ("s_e", ctypes.c_float),
+        ("s_eps", ctypes.c_float),
+        ("s_q", ctypes.c_float),
+    ]
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+_vp = ctypes.c_void_p
+_P = ctypes.POINTER(LbmParams)
+_i = ctypes.c_int
+_f = ctypes.c_float
+
+# name -> (restype, argtypes); every symbol declared in include/lbm.h
+SIGNATURES = {
+    "lbm_abi_version": (_i, []),
+    "lbm_last_error": (ctypes.c_char_p, []),
+    "lbm_sync": (_i, [_vp]),
+    "lbm_step_aa": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "lbm_step_aa_n": (_i, [_P, _vp, _vp, _vp, _i, _i, _vp]),
+    "lbm_step_pull": (_i, [_P, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lbm_step_fused": (_i, [_P, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp]),
+    "lbm_fused_max_k": (_i, []),
+    "lbm_macro": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "lbm_visualize": (_i, [_P, _vp, _vp, _vp, _vp, _f, _vp]),
+    "lbm_diagnostics_work_size": (ctypes.c_size_t, [_P]),
+    "lbm_diagnostics": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load liblbm.so and bind every entry point (raises if anything is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LbmError(f"liblbm.so not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)  # AttributeError if the symbol is absent
+        fn.restype = res
+        fn.argtypes = args
+    v = L.lbm_abi_version()
+    if v != ABI_VERSION:
+        raise LbmError(f"liblbm ABI version {v}, expected {ABI_VERSION}")
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().lbm_last_error().decode(errors="replace")
+        raise LbmError(f"{what} failed ({rc}): {msg}")
+
+
+def params(W, H, tau_inv, cs=0.0, model=MRT_LES, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1):
+    return LbmParams(int(W), int(H), int(model), int(bc), int(clamp), float(tau_inv), float(cs), float(s_e),
+                     float(s_eps), float(s_q))

@@ -1,0 +1,156 @@
+"""Masks, inlet profiles and initial states, built in FP64 on the host.
+
+The reference builds its mask from an image (``lbm_smemaa_main.py:53-68``) and then forces the boundary
+roles: rows 0 and H-1 solid (``:61-62``), the inlet column x=0 and the outlet ghost column x=W-1 open on
+rows 1..H-2 (``:65-66``).  ``channel_mask`` applies the same forcing to an analytic geometry;
+``finalize_channel_mask`` applies it to any user mask (for example one read from a BMP by
+``lbm.io.mask_from_bmp``).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], dtype=np.int64)
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], dtype=np.int64)
+OPP = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6], dtype=np.int64)
+WEIGHTS = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, dtype=np.float64)
+
+
+def finalize_channel_mask(mask):
+    """Force the channel roles of lbm_smemaa_main.py:61-66 and check the outlet column.
+
+    Returns a C-contiguous uint8 [H, W] copy.  Raises if a solid cell sits on column W-2 (the cell that
+    feeds the zero-gradient ghost must be fluid)."""
+    m = np.array(mask, dtype=np.uint8, copy=True)
+    if m.ndim != 2:
+        raise ValueError("mask must be 2-D [H, W]")
+    H, W = m.shape
+    if H < 3 or W < 3:
+        raise ValueError(f"channel lattice needs W, H >= 3, got {W}x{H}")
+    m[m != 0] = 1
+    m[0, :] = 1
+    m[H - 1, :] = 1
+    m[1:H - 1, 0] = 0
+    m[1:H - 1, W - 1] = 0
+    if m[1:H - 1, W - 2].any():
+        raise ValueError("solid cells on column W-2 are not supported (it feeds the outlet ghost column)")
+    return np.ascontiguousarray(m)
+
+
+def channel_mask(W, H, cylinder=None):
+    """Channel with wall rows and an optional solid disc ``cylinder=(cx, cy, r)`` (cells with
+    (x-cx)^2 + (y-cy)^2 <= r^2 at integer cell coordinates)."""
+    m = np.zeros((H, W), np.uint8)
+    if cylinder is not None:
+        cx, cy, r = cylinder
+        yy, xx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+        m[(xx - cx) ** 2 + (yy - cy) ** 2 <= r * r] = 1
+    return finalize_channel_mask(m)
+
+
+def periodic_mask(W, H):
+    return np.zeros((H, W), np.uint8)
+
+
+def parabolic_profile(H, u_max):
+    """Poiseuille profile with half-way walls at y = 1/2 and y = H - 3/2:
+    u(y) = 4 u_max (y - 1/2)(H - 3/2 - y) / (H - 2)^2 for rows 1..H-2, 0 on the wall rows."""
+    y = np.arange(H, dtype=np.float64)
+    u = 4.0 * u_max * (y - 0.5) * (H - 1.5 - y) / float(H - 2) ** 2
+    u[0] = 0.0
+    u[H - 1] = 0.0
+    return u
+
+
+def cosine_ramp_profile(H, u_in, width=50.0):
+    """The reference's inlet speed u_in * s(y), s = 1/2 (1 - cos(3.14159 d / width)) for d < width, with
+    d = min(y, H-1-y)   (lbm_core4.cu:46-51), evaluated in FP64."""
+    y = np.arange(H, dtype=np.float64)
+    d = np.minimum(y, H - 1 - y)
+    s = np.where(d < width, 0.5 * (1.0 - np.cos(3.14159 * d / width)), 1.0)
+    return u_in * s
+
+
+def feq(rho, ux, uy):
+    """w_q rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) in FP64, shape [9, ...]   (lbm_smemaa_main.py:35-40)."""
+    rho = np.asarray(rho, np.float64)
+    ux = np.asarray(ux, np.float64)
+    uy = np.asarray(uy, np.float64)
+    shape = np.broadcast(rho, ux, uy).shape
+    usq = ux * ux + uy * uy
+    out = np.empty((9,) + shape)
+    for q in range(9):
+        cu = CX[q] * ux + CY[q] * uy
+        out[q] = WEIGHTS[q] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq)
+    return out
+
+
+def channel_initial_state(W, H, profile, noise=1e-4, seed=0):
+    """rho = 1, ux = inlet profile on every column, uy ~ U(-noise, noise) from default_rng(seed); f = feq."""
+    rng = np.random.default_rng(seed)
+    ux = np.broadcast_to(np.asarray(profile, np.float64)[:, None], (H, W))
+    uy = rng.uniform(-noise, noise, size=(H, W)) if noise else np.zeros((H, W))
+    return feq(np.ones((H, W)), ux, uy)
+
+
+def reynolds_tau_inv(u, length, re):
+    """tau_inv = 1/tau with nu = u L / Re and tau = 3 nu + 1/2 (lattice units)."""
+    nu = u * length / re
+    return 1.0 / (3.0 * nu + 0.5)
+
+
+def nu_tau_inv(nu):
+    return 1.0 / (3.0 * nu + 0.5)
+
+
+def shear_layer_velocity(W, H, u0=0.04, delta=80.0, amp=0.002, noise=1e-4, seed=3):
+    """Periodic double shear layer on cell centres (j + 1/2):
+    ux = u0 tanh(delta (y/H - 1/4)) for y < H/2, u0 tanh(delta (3/4 - y/H)) otherwise;
+    uy = amp sin(2 pi x / W); plus U(-noise, noise) on both components."""
+    rng = np.random.default_rng(seed)
+    yc = (np.arange(H, dtype=np.float64) + 0.5) / H
+    xc = np.arange(W, dtype=np.float64)
+    ux1 = np.where(yc < 0.5, u0 * np.tanh(delta * (yc - 0.25)), u0 * np.tanh(delta * (0.75 - yc)))
+    ux = np.broadcast_to(ux1[:, None], (H, W)).copy()
+    uy = np.broadcast_to(amp * np.sin(2.0 * math.pi * xc / W)[None, :], (H, W)).copy()
+    if noise:
+        ux += rng.uniform(-noise, noise, size=(H, W))
+        uy += rng.uniform(-noise, noise, size=(H, W))
+    return ux, uy
+
+
+def taylor_green_velocity(W, H, u0, t=0.0, nu=0.0):
+    """Decaying Taylor-Green vortex with kx = 2 pi / W, ky = 2 pi / H:
+    ux = -u0 (ky/kx)... normalised so that ux = -u0 cos(kx x) sin(ky y) * (ky/k), uy = u0 sin(kx x) cos(ky y) * (kx/k)
+    is divergence free; amplitude decays as exp(-nu (kx^2 + ky^2) t)."""
+    kx, ky = 2.0 * math.pi / W, 2.0 * math.pi / H
+    x = np.arange(W, dtype=np.float64)[None, :]
+    y = np.arange(H, dtype=np.float64)[:, None]
+    k = math.hypot(kx, ky)
+    decay = math.exp(-nu * (kx * kx + ky * ky) * t)
+    ux = -u0 * (ky / k) * np.cos(kx * x) * np.sin(ky * y) * decay
+    uy = u0 * (kx / k) * np.sin(kx * x) * np.cos(ky * y) * decay
+    return np.broadcast_to(ux, (H, W)).copy(), np.broadcast_to(uy, (H, W)).copy()
+
+
+def random_phase_velocity(W, H, k0, u_rms, seed, power=4.0):
+    """Solenoidal random-phase field from a stream function with E(k) ~ k^power exp(-2 (k/k0)^2),
+    rescaled to the given u_rms (mean of ux^2 + uy^2 over the box, square-rooted)."""
+    rng = np.random.default_rng(seed)
+    kx = np.fft.fftfreq(W) * W
+    ky = np.fft.fftfreq(H) * H
+    KX, KY = np.meshgrid(kx, ky)
+    k = np.hypot(KX, KY)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        # |psi_k|^2 ~ E(k) / (k^3) in 2-D (energy shell 2 pi k |u_k|^2, u = k psi)
+        amp = np.where(k > 0, np.sqrt(k ** power * np.exp(-2.0 * (k / k0) ** 2) / np.maximum(k, 1e-30) ** 3), 0.0)
+    phase = np.exp(2j * math.pi * rng.random((H, W)))
+    psi_k = amp * phase
+    psi = np.real(np.fft.ifft2(psi_k))
+    psi_k = np.fft.fft2(psi)
+    ux = np.real(np.fft.ifft2(1j * 2 * math.pi * KY / H * psi_k))
+    uy = np.real(np.fft.ifft2(-1j * 2 * math.pi * KX / W * psi_k))
+    scale = u_rms / math.sqrt(float(np.mean(ux * ux + uy * uy)))
+    return ux * scale, uy * scale

@@ -1,0 +1,177 @@
+"""Device-resident D2Q9 simulation driven through liblbm's C-ABI.
+
+Mirrors the flow of the reference drivers (lbm_smemaa_main.py:21-174): build the initial state and the
+mask on the host, move them to the GPU, then advance with single steps or k-step fused launches.  The
+state lives in torch tensors only for device memory and streams; all compute is liblbm's HIP kernels.
+
+Storage: ``f`` is float32 [9, H, W] (SoA planes, x fastest).  ``parity`` is the AA parity of ``f``
+(0 = canonical, 1 = reversed).  The fused kernel needs a second buffer, allocated on first use.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import geometry as geo
+
+_MODELS = {"mrt": _lib.MRT_LES, "mrt_les": _lib.MRT_LES, "bgk": _lib.BGK}
+_BCS = {"channel": _lib.CHANNEL, "periodic": _lib.PERIODIC}
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Simulation:
+    """One D2Q9 lattice on one GPU.
+
+    Parameters mirror the reference driver surface: lattice size (W, H), ``tau_inv``, Smagorinsky ``cs``,
+    the solid ``mask`` [H, W], the inlet speed (``inlet_u`` per row), steps-per-launch ``k`` (``step``), and
+    the AA parity of the stored state.
+    """
+
+    def __init__(self, W, H, tau_inv, *, cs=0.0, model="mrt", bc="channel", mask=None, inlet_u=None, f0=None,
+                 clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda"):
+        self.lib = _lib.load()
+        self.W, self.H = int(W), int(H)
+        self.model = _MODELS[model] if isinstance(model, str) else int(model)
+        self.bc = _BCS[bc] if isinstance(bc, str) else int(bc)
+        self.params = _lib.params(self.W, self.H, tau_inv, cs, self.model, self.bc, int(bool(clamp)), s_e, s_eps,
+                                  s_q)
+        self.device = torch.device(device)
+        if mask is None:
+            mask = geo.channel_mask(W, H) if self.bc == _lib.CHANNEL else geo.periodic_mask(W, H)
+        elif self.bc == _lib.CHANNEL:
+            mask = geo.finalize_channel_mask(mask)
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        if mask.shape != (self.H, self.W):
+            raise ValueError(f"mask shape {mask.shape} != ({self.H}, {self.W})")
+        self.mask_host = mask
+        self.mask = torch.from_numpy(mask).to(self.device)
+        if inlet_u is None:
+            inlet_u = np.zeros(self.H)
+        inlet_u = np.asarray(inlet_u, np.float64)
+        if inlet_u.shape != (self.H,):
+            raise ValueError("inlet_u must have H entries")
+        self.inlet_u_host = inlet_u.astype(np.float32)
+        self.inlet_u = torch.from_numpy(self.inlet_u_host).to(self.device)
+        if f0 is None:
+            f0 = geo.feq(np.ones((self.H, self.W)), 0.0, 0.0)
+        f0 = np.asarray(f0)
+        if f0.shape != (9, self.H, self.W):
+            raise ValueError(f"f0 shape {f0.shape} != (9, {self.H}, {self.W})")
+        self.f = torch.from_numpy(np.ascontiguousarray(f0, dtype=np.float32)).to(self.device)
+        self.f_alt = None
+        self.parity = int(parity)
+        self.steps = 0
+        self.rho = torch.zeros((self.H, self.W), dtype=torch.float32, device=self.device)
+        self.ux = torch.zeros_like(self.rho)
+        self.uy = torch.zeros_like(self.rho)
+        self._diag_work = None
+        self._diag_out = None
+
+    # ------------------------------------------------------------------ stepping
+    def step(self, n=1, k=1, macros=False):
+        """Advance n collide-stream steps; k > 1 uses the temporally blocked kernel (k steps per launch)."""
+        n = int(n)
+        k = int(k)
+        if k <= 1:
+            for _ in range(n):
+                self._aa(macros)
+            return self
+        if n % k:
+            raise ValueError(f"n={n} is not a multiple of k={k}")
+        for _ in range(n // k):
+            self._fused(k, macros)
+        return self
+
+    def _aa(self, macros=False):
+        m = (self.ux, self.uy, self.rho) if macros else (None, None, None)
+        rc = self.lib.lbm_step_aa(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
+                                  self.parity, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]), _stream())
+        _lib.check(rc, "lbm_step_aa")
+        self.parity ^= 1
+        self.steps += 1
+
+    def _fused(self, k, macros=False):
+        if self.f_alt is None:
+            self.f_alt = torch.empty_like(self.f)
+        m = (self.ux, self.uy, self.rho) if macros else (None, None, None)
+        rc = self.lib.lbm_step_fused(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.f_alt),
+                                     _ptr(self.inlet_u), self.parity, k, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]),
+                                     _stream())
+        _lib.check(rc, "lbm_step_fused")
+        self.f, self.f_alt = self.f_alt, self.f
+        self.parity = (self.parity + k) & 1
+        self.steps += k
+
+    # ------------------------------------------------------------------ observables
+    def macro(self):
+        """(rho, ux, uy) of the stored state (torch tensors on the device)."""
+        rc = self.lib.lbm_macro(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
+                                self.parity, _ptr(self.ux), _ptr(self.uy), _ptr(self.rho), _stream())
+        _lib.check(rc, "lbm_macro")
+        return self.rho, self.ux, self.uy
+
+    def diagnostics(self):
+        """FP64 (mass, jx, jy, Fx, Fy) reduced on the GPU."""
+        if self._diag_work is None:
+            nbytes = self.lib.lbm_diagnostics_work_size(ctypes.byref(self.params))
+            self._diag_work = torch.empty(nbytes // 8, dtype=torch.float64, device=self.device)
+            self._diag_out = torch.empty(5, dtype=torch.float64, device=self.device)
+        rc = self.lib.lbm_diagnostics(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
+                                      self.parity, _ptr(self._diag_work), _ptr(self._diag_out), _stream())
+        _lib.check(rc, "lbm_diagnostics")
+        return self._diag_out.cpu().numpy().copy()
+
+    def drag_lift(self, u_ref, length, rho_ref=1.0):
+        """(Cd, Cl) = 2 F / (rho_ref u_ref^2 length) from the momentum-exchange force."""
+        d = self.diagnostics()
+        scale = 2.0 / (rho_ref * u_ref * u_ref * length)
+        return d[3] * scale, d[4] * scale
+
+    def visualize(self, vort_scale=50.0):
+        """RGBA8 vorticity image [H, W, 4] (interior cells; computed from the current macro fields)."""
+        self.macro()
+        img = torch.zeros((self.H, self.W, 4), dtype=torch.uint8, device=self.device)
+        rc = self.lib.lbm_visualize(ctypes.byref(self.params), _ptr(self.ux), _ptr(self.uy), _ptr(self.mask),
+                                    _ptr(img), float(vort_scale), _stream())
+        _lib.check(rc, "lbm_visualize")
+        return img
+
+    def state(self):
+        """(f as numpy float32 [9, H, W], parity)."""
+        return self.f.cpu().numpy(), self.parity
+
+    def synchronize(self):
+        _lib.check(self.lib.lbm_sync(_stream()), "lbm_sync")
+
+    # ------------------------------------------------------------------ checkpoint / resume
+    def save(self, path):
+        """Write f, parity, step count and parameters to an .npz file."""
+        p = self.params
+        np.savez(path, f=self.f.cpu().numpy(), parity=self.parity, steps=self.steps, mask=self.mask_host,
+                 inlet_u=self.inlet_u_host,
+                 params=np.array([p.W, p.H, p.model, p.bc, p.clamp], np.int64),
+                 fparams=np.array([p.tau_inv, p.cs, p.s_e, p.s_eps, p.s_q], np.float32))
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        z = np.load(path, allow_pickle=False)
+        W, H, model, bc, clamp = (int(v) for v in z["params"])
+        tau_inv, cs, s_e, s_eps, s_q = (float(v) for v in z["fparams"])
+        sim = cls(W, H, tau_inv, cs=cs, model=model, bc=bc, mask=z["mask"], inlet_u=z["inlet_u"], f0=z["f"],
+                  clamp=bool(clamp), s_e=s_e, s_eps=s_eps, s_q=s_q, parity=int(z["parity"]), device=device)
+        sim.steps = int(z["steps"])
+        return sim

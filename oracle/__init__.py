@@ -1,0 +1,207 @@
+"""D2Q9 oracle -- test infrastructure only.
+
+ctypes front end of ``oracle/liblbm_oracle.so`` (the C restatement in ``lbm_oracle.c`` /
+``oracle_impl.h``) plus the independent numpy restatement in ``oracle/pyref.py``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` import this package, and only as the
+checker: the product (``lbm-cuda_amd/``) never imports it.
+
+Every function takes numpy arrays in the reference layout: populations ``f[q, y, x]`` (SoA, x fastest,
+``lbm_smemaa_main.py:35,43``), ``mask[y, x]`` as uint8 (``lbm_smemaa_main.py:68``), ``inlet_u[y]``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbm_oracle.so")
+
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], dtype=np.int64)
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], dtype=np.int64)
+OPP = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6], dtype=np.int64)
+W_EXACT = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, dtype=np.float64)
+
+MRT, BGK = 0, 1
+CHANNEL, PERIODIC = 0, 1
+
+
+class OracleParams(ctypes.Structure):
+    _fields_ = [
+        ("W", ctypes.c_int32),
+        ("H", ctypes.c_int32),
+        ("model", ctypes.c_int32),
+        ("bc", ctypes.c_int32),
+        ("clamp", ctypes.c_int32),
+        ("tau_inv", ctypes.c_float),
+        ("cs", ctypes.c_float),
+        ("s_e", ctypes.c_float),
+        ("s_eps", ctypes.c_float),
+        ("s_q", ctypes.c_float),
+    ]
+
+
+def params(W, H, tau_inv, cs=0.0, model=MRT, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1):
+    return OracleParams(int(W), int(H), int(model), int(bc), int(clamp), float(tau_inv), float(cs),
+                        float(s_e), float(s_eps), float(s_q))
+
+
+def build(verbose=False):
+    """Compile the C restatement with gcc (no GPU, no reference sources involved)."""
+    cmd = ["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-Wall",
+           "-o", LIB_PATH, os.path.join(_HERE, "lbm_oracle.c"), "-lm"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True, cwd=_HERE)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(OracleParams)
+        vp = ctypes.c_void_p
+        for suf in ("_f64", "_f32"):
+            getattr(L, "oracle_aa_step" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp, vp, vp]
+            getattr(L, "oracle_pull_step" + suf).argtypes = [P, vp, vp, vp, vp, vp, vp, vp]
+            getattr(L, "oracle_macro" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp, vp, vp]
+            real = ctypes.c_double if suf == "_f64" else ctypes.c_float
+            getattr(L, "oracle_collide" + suf).argtypes = [P, vp, ctypes.c_int, real, vp]
+        L.oracle_mrt_forward_f64.argtypes = [vp, vp]
+        L.oracle_mrt_inverse_f64.argtypes = [vp, vp]
+        L.oracle_diagnostics_f32.argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
+        _lib = L
+    return _lib
+
+
+def _suffix(dtype):
+    if dtype == np.float64:
+        return "_f64"
+    if dtype == np.float32:
+        return "_f32"
+    raise TypeError(f"oracle supports float32/float64, got {dtype}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check_state(p, mask, f, inlet_u):
+    W, H = p.W, p.H
+    assert f.shape == (9, H, W) and f.flags.c_contiguous, f.shape
+    assert mask.shape == (H, W) and mask.dtype == np.uint8 and mask.flags.c_contiguous
+    assert inlet_u.shape == (H,) and inlet_u.dtype == f.dtype
+
+
+def aa_step(p, mask, f, inlet_u, parity, macros=False):
+    """One AA step in place on ``f`` (float32 = FP32 mirror, float64 = FP64 oracle)."""
+    _check_state(p, mask, f, inlet_u)
+    mac = [np.zeros((p.H, p.W), f.dtype) for _ in range(3)] if macros else [None] * 3
+    getattr(lib(), "oracle_aa_step" + _suffix(f.dtype))(
+        ctypes.byref(p), _ptr(mask), _ptr(f), _ptr(inlet_u), int(parity), _ptr(mac[1]), _ptr(mac[2]), _ptr(mac[0]))
+    return tuple(mac) if macros else None
+
+
+def aa_steps(p, mask, f, inlet_u, parity, n):
+    """n AA steps in place; returns the parity after them."""
+    for _ in range(n):
+        aa_step(p, mask, f, inlet_u, parity)
+        parity ^= 1
+    return parity
+
+
+def pull_step(p, mask, f_now, f_out, inlet_u):
+    """Reference-compatible pull step (lbm_core4.cu:120-211); returns (rho, ux, uy)."""
+    _check_state(p, mask, f_now, inlet_u)
+    assert f_out.shape == f_now.shape and f_out.dtype == f_now.dtype
+    mac = [np.zeros((p.H, p.W), f_now.dtype) for _ in range(3)]
+    getattr(lib(), "oracle_pull_step" + _suffix(f_now.dtype))(
+        ctypes.byref(p), _ptr(mask), _ptr(f_now), _ptr(f_out), _ptr(inlet_u), _ptr(mac[1]), _ptr(mac[2]),
+        _ptr(mac[0]))
+    return tuple(mac)
+
+
+def macro(p, mask, f, inlet_u, parity):
+    _check_state(p, mask, f, inlet_u)
+    mac = [np.zeros((p.H, p.W), f.dtype) for _ in range(3)]
+    getattr(lib(), "oracle_macro" + _suffix(f.dtype))(
+        ctypes.byref(p), _ptr(mask), _ptr(f), _ptr(inlet_u), int(parity), _ptr(mac[1]), _ptr(mac[2]), _ptr(mac[0]))
+    return tuple(mac)
+
+
+def collide(p, f9, inlet=False, uin=0.0):
+    """Collision of one cell; returns (post-collision f[9], (rho, ux, uy))."""
+    g = np.array(f9, dtype=f9.dtype if isinstance(f9, np.ndarray) else np.float64).copy()
+    mac = np.zeros(3, g.dtype)
+    getattr(lib(), "oracle_collide" + _suffix(g.dtype))(ctypes.byref(p), _ptr(g), int(inlet), uin, _ptr(mac))
+    return g, mac
+
+
+def mrt_forward(f9):
+    f = np.ascontiguousarray(f9, dtype=np.float64)
+    m = np.zeros(9)
+    lib().oracle_mrt_forward_f64(_ptr(f), _ptr(m))
+    return m
+
+
+def mrt_inverse(m9):
+    m = np.ascontiguousarray(m9, dtype=np.float64)
+    f = np.zeros(9)
+    lib().oracle_mrt_inverse_f64(_ptr(m), _ptr(f))
+    return f
+
+
+def diagnostics(p, mask, f32, inlet_u32, parity):
+    """FP64 (mass, jx, jy, Fx, Fy) of a float32 state."""
+    _check_state(p, mask, f32, inlet_u32)
+    out = np.zeros(5)
+    lib().oracle_diagnostics_f32(ctypes.byref(p), _ptr(mask), _ptr(f32), _ptr(inlet_u32), int(parity), _ptr(out))
+    return out
+
+
+# --------------------------------------------------------------------------------------------------
+# State conventions shared by the tests
+# --------------------------------------------------------------------------------------------------
+
+def feq(rho, ux, uy):
+    """Second-order equilibrium w rho (1 + 3cu + 4.5cu^2 - 1.5u^2) in FP64 (lbm_smemaa_main.py:35-40)."""
+    rho = np.asarray(rho, np.float64)
+    ux = np.asarray(ux, np.float64)
+    uy = np.asarray(uy, np.float64)
+    usq = ux * ux + uy * uy
+    out = np.empty((9,) + np.broadcast(rho, ux, uy).shape)
+    for q in range(9):
+        cu = CX[q] * ux + CY[q] * uy
+        out[q] = W_EXACT[q] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq)
+    return out
+
+
+def observable(p, mask, parity):
+    """Boolean [9, H, W] selecting the populations that carry state (stale slots excluded):
+    updated cells only (no solids, wall rows or outlet ghost); in canonical form the inlet's q=1,5,8
+    (which arrive from outside and are rebuilt by Zou-He) are excluded."""
+    H, W = p.H, p.W
+    m = mask.astype(bool)
+    upd = ~m
+    if p.bc == CHANNEL:
+        upd[0, :] = False
+        upd[H - 1, :] = False
+        upd[:, W - 1] = False
+        upd[1:H - 1, 0] = True
+    sel = np.broadcast_to(upd, (9, H, W)).copy()
+    if p.bc == CHANNEL and parity == 0:
+        for q in (1, 5, 8):
+            sel[q, :, 0] = False
+    return sel
+
+
+def reverse(f):
+    """Read a reversed (odd-parity) state as post-collision populations: out[q] = f[opp q]."""
+    return f[OPP]

@@ -1,0 +1,410 @@
+/*
+ * CPU restatement of the D2Q9 collide-stream step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This header is included twice by lbm_oracle.c: once with REAL = double (the FP64 oracle) and once with
+ * REAL = float (the FP32 mirror, whose operation order and fused multiply-adds are the ones the gfx950
+ * kernels use, so that the two agree bit for bit).  Nothing in the product links or calls this code; only
+ * tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it, as the checker.
+ *
+ * What it restates (file:line in the reference tree):
+ *   - D2Q9 velocity set, opposite table, weights           lbm_core4.cu:1-6, lbm_core_smemaa.cu:1-7
+ *   - moments, clamps, MRT forward/equilibria/relax/inverse lbm_core4.cu:124-199
+ *   - Smagorinsky tau_eff                                   lbm_core4.cu:171-177
+ *   - population clamp [0,10] and outlet ghost copy         lbm_core4.cu:200-210
+ *   - Zou-He velocity inlet + inlet MRT (no LES, no clamps) lbm_core4.cu:29-118
+ *   - pull streaming with half-way bounce-back              lbm_core4.cu:129-138
+ *   - BGK collision of the temporally blocked family        lbm_core_smemaa.cu:194-220
+ *   - AA even (local, canonical->reversed) and odd (gather/scatter, reversed->canonical) steps,
+ *     bounce-back on solid source/destination, left-open inlet drop
+ *                                                           lbm_core_smemaa.cu:92-223, 293-445;
+ *                                                           lbm_core_smemaa_endfused.cu:330-369
+ *   - macro extraction with Zou-He at x=0                   lbm_core_smemaa_endfused.cu:377-422
+ *
+ * Deliberate definitions where the reference is inconsistent (each one is pinned by a test):
+ *   - corners (W-1,0) and (W-1,H-1) are walls (lbm_core4.cu:134 honours the mask; the AA kernels force the
+ *     whole W-1 column open, lbm_core_smemaa.cu:51-55);
+ *   - the zero-gradient outlet is the one of lbm_core4.cu:114-116 (ghost W-1 = post-collision copy of W-2).
+ *     In AA form this needs the W-2 cell to stand in for the ghost's scatter of q=3,6,7 on odd steps, which
+ *     the reference's AA kernels omit (their W-2 canonical slots 3,6,7 go stale, lbm_core_smemaa.cu:296-315);
+ *   - the inlet speed is a per-row array (the reference evaluates u_in*ramp(y) inline, lbm_core4.cu:46-51).
+ */
+
+#define CAT_(a, b) a##b
+#define CAT(a, b) CAT_(a, b)
+#define FN(name) CAT(name, SUFFIX)
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Derived constants                                                                                      */
+/* ---------------------------------------------------------------------------------------------------- */
+
+typedef struct {
+    int W, H, model, bc, clamp, les;
+    REAL tau_inv, tau0, tau0sq, c18cs2, s_e, s_eps, s_q;
+} FN(kpar_t);
+
+static void FN(make_kpar)(const oracle_params *p, FN(kpar_t) * k)
+{
+    k->W = p->W;
+    k->H = p->H;
+    k->model = p->model;
+    k->bc = p->bc;
+    k->clamp = p->clamp;
+    k->les = (p->model == ORACLE_MRT && p->cs != 0.0f) ? 1 : 0;
+    k->tau_inv = (REAL)p->tau_inv;
+    /* tau_0 = 1/tau_inv and 18*Cs*Cs as in lbm_core4.cu:175-176, evaluated once in the working precision */
+    k->tau0 = (REAL)1 / (REAL)p->tau_inv;
+    k->tau0sq = k->tau0 * k->tau0;
+    k->c18cs2 = ((REAL)18 * (REAL)p->cs) * (REAL)p->cs;
+    k->s_e = (REAL)p->s_e;
+    k->s_eps = (REAL)p->s_eps;
+    k->s_q = (REAL)p->s_q;
+}
+
+static inline REAL FN(clampr)(REAL v, REAL lo, REAL hi)
+{
+    /* fminf(hi, fmaxf(lo, v)) of lbm_core4.cu:142-144,203 */
+    REAL a = v > lo ? v : lo;
+    return a < hi ? a : hi;
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Collision K: 9 pre-collision populations -> 9 post-collision populations (clamped to [0,10]).          */
+/* inlet != 0 applies Zou-He with speed uin first and then the inlet collision of lbm_core4.cu:65-93.       */
+/* mac (may be NULL) receives (rho, ux, uy) as the reference writes them (lbm_core4.cu:59-61,145-147).     */
+/* ---------------------------------------------------------------------------------------------------- */
+
+static void FN(collide_mrt)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin, REAL *mac)
+{
+    REAL rho = 0, ux = 0, uy = 0, rc = 0;
+    int les = 0;
+    if (inlet) {
+        /* Zou-He, lbm_core4.cu:54-57: rho = (f0+f2+f4+2(f3+f6+f7))/(1-u) */
+        REAL num = FMA((REAL)2, (f[3] + f[6]) + f[7], (f[0] + f[2]) + f[4]);
+        rho = num / ((REAL)1 - uin);
+        REAL ru = rho * uin;
+        REAL h = (REAL)0.5 * (f[2] - f[4]);
+        f[1] = FMA(C_2_3, ru, f[3]);
+        f[5] = FMA(C_1_6, ru, f[7] - h);
+        f[8] = FMA(C_1_6, ru, f[6] + h);
+        ux = uin;
+        uy = (REAL)0;
+        rc = rho;
+        les = 0;
+    }
+    REAL s13 = f[1] + f[3], s24 = f[2] + f[4], s57 = f[5] + f[7], s68 = f[6] + f[8];
+    REAL f14 = s13 + s24, f58 = s57 + s68;
+    REAL a13 = f[1] - f[3], a24 = f[2] - f[4], a57 = f[5] - f[7], a86 = f[8] - f[6];
+    REAL sx = a57 + a86; /* f5 - f6 - f7 + f8 */
+    REAL sy = a57 - a86; /* f5 + f6 - f7 - f8 */
+    if (!inlet) {
+        /* moments, lbm_core4.cu:130-144 */
+        rho = (f[0] + f14) + f58;
+        REAL jx = a13 + sx;
+        REAL jy = a24 + sy;
+        REAL inv_rho = (REAL)1 / rho;
+        ux = jx * inv_rho;
+        uy = jy * inv_rho;
+        rc = rho;
+        if (P->clamp) {
+            ux = FN(clampr)(ux, (REAL)-0.4, (REAL)0.4);
+            uy = FN(clampr)(uy, (REAL)-0.4, (REAL)0.4);
+            rc = FN(clampr)(rho, (REAL)0, (REAL)1.5);
+        }
+        les = P->les;
+    }
+    if (mac) {
+        mac[0] = rc;
+        mac[1] = ux;
+        mac[2] = uy;
+    }
+    /* MRT forward, Lallemand-Luo order (rho, e, eps, jx, qx, jy, qy, pxx, pxy), lbm_core4.cu:152-165 */
+    REAL m0 = rc;
+    REAL m1 = FMA((REAL)2, f58, FMA((REAL)-4, f[0], -f14));
+    REAL m2 = FMA((REAL)-2, f14, FMA((REAL)4, f[0], f58));
+    REAL m3 = rc * ux;
+    REAL m4 = FMA((REAL)-2, a13, sx);
+    REAL m5 = rc * uy;
+    REAL m6 = FMA((REAL)-2, a24, sy);
+    REAL m7 = s13 - s24;
+    REAL m8 = s57 - s68;
+    /* equilibria, lbm_core4.cu:166-170 */
+    REAL ux2 = ux * ux, uy2 = uy * uy;
+    REAL usq = ux2 + uy2;
+    REAL m1eq = FMA((REAL)3, usq, (REAL)-2) * rc;
+    REAL m2eq = FMA((REAL)-3, usq, (REAL)1) * rc;
+    REAL m7eq = rc * (ux2 - uy2);
+    REAL m8eq = m3 * uy;
+    REAL n7 = m7 - m7eq;
+    REAL n8 = m8 - m8eq;
+    REAL w = P->tau_inv;
+    if (les) {
+        /* Smagorinsky, lbm_core4.cu:172-177 */
+        REAL Q = SQRT(FMA(n7, n7, FMA(n8, n8, (REAL)1e-8)));
+        REAL inv_rc = (REAL)1 / rc; /* equals 1/rho unless the density clamp engaged */
+        REAL d = FMA(P->c18cs2, Q * inv_rc, P->tau0sq);
+        REAL tau_eff = (REAL)0.5 * (P->tau0 + SQRT(d));
+        w = (REAL)1 / tau_eff;
+    }
+    /* relaxation, lbm_core4.cu:180-185 (m4, m6 relax towards -m3, -m5) */
+    m1 = FMA(-P->s_e, m1 - m1eq, m1);
+    m2 = FMA(-P->s_eps, m2 - m2eq, m2);
+    m4 = FMA(-P->s_q, m4 + m3, m4);
+    m6 = FMA(-P->s_q, m6 + m5, m6);
+    m7 = FMA(-w, n7, m7);
+    m8 = FMA(-w, n8, m8);
+    /* inverse, lbm_core4.cu:187-199 */
+    REAL t0 = (m0 - m1) + m2;
+    REAL t1 = FMA((REAL)-2, m2, FMA((REAL)4, m0, -m1));
+    REAL t2 = FMA((REAL)4, m0, FMA((REAL)2, m1, m2));
+    REAL A = (m3 - m4) * (REAL)6;
+    REAL B = (m5 - m6) * (REAL)6;
+    REAL Pp = m7 * (REAL)9;
+    REAL X = FMA((REAL)6, m3, m4 * (REAL)3);
+    REAL Y = FMA((REAL)6, m5, m6 * (REAL)3);
+    REAL Z = m8 * (REAL)9;
+    REAL t1p = t1 + Pp, t1m = t1 - Pp;
+    REAL u = t2 + Z, v = t2 - Z;
+    REAL XpY = X + Y, XmY = X - Y;
+    f[0] = t0 * C_INV9;
+    f[1] = (t1p + A) * C_INV36;
+    f[2] = (t1m + B) * C_INV36;
+    f[3] = (t1p - A) * C_INV36;
+    f[4] = (t1m - B) * C_INV36;
+    f[5] = (u + XpY) * C_INV36;
+    f[6] = (v - XmY) * C_INV36;
+    f[7] = (u - XpY) * C_INV36;
+    f[8] = (v + XmY) * C_INV36;
+    for (int q = 0; q < 9; ++q) f[q] = FN(clampr)(f[q], (REAL)0, (REAL)10);
+}
+
+static void FN(collide_bgk)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin, REAL *mac)
+{
+    REAL rho, ux, uy;
+    if (inlet) {
+        REAL num = FMA((REAL)2, (f[3] + f[6]) + f[7], (f[0] + f[2]) + f[4]);
+        rho = num / ((REAL)1 - uin);
+        REAL ru = rho * uin;
+        REAL h = (REAL)0.5 * (f[2] - f[4]);
+        f[1] = FMA(C_2_3, ru, f[3]);
+        f[5] = FMA(C_1_6, ru, f[7] - h);
+        f[8] = FMA(C_1_6, ru, f[6] + h);
+        ux = uin;
+        uy = (REAL)0;
+    } else {
+        /* plain moments without clamps, lbm_core_smemaa.cu:194-201 */
+        REAL s13 = f[1] + f[3], s24 = f[2] + f[4], s57 = f[5] + f[7], s68 = f[6] + f[8];
+        REAL a13 = f[1] - f[3], a24 = f[2] - f[4], a57 = f[5] - f[7], a86 = f[8] - f[6];
+        rho = (f[0] + (s13 + s24)) + (s57 + s68);
+        REAL inv_rho = (REAL)1 / rho;
+        ux = (a13 + (a57 + a86)) * inv_rho;
+        uy = (a24 + (a57 - a86)) * inv_rho;
+    }
+    if (mac) {
+        mac[0] = rho;
+        mac[1] = ux;
+        mac[2] = uy;
+    }
+    /* feq = w rho (1 + 3cu + 4.5cu^2 - 1.5u^2); f += tau_inv (feq - f)   (lbm_core_smemaa.cu:206-212) */
+    REAL usq = FMA(uy, uy, ux * ux);
+    REAL base = FMA((REAL)-1.5, usq, (REAL)1);
+    REAL wr0 = C_W0 * rho, wr1 = C_W1 * rho, wr5 = C_W5 * rho;
+    REAL cu[9];
+    cu[0] = (REAL)0;
+    cu[1] = ux;
+    cu[2] = uy;
+    cu[3] = -ux;
+    cu[4] = -uy;
+    cu[5] = ux + uy;
+    cu[6] = uy - ux;
+    cu[7] = -(ux + uy);
+    cu[8] = ux - uy;
+    for (int q = 0; q < 9; ++q) {
+        REAL wr = q == 0 ? wr0 : (q < 5 ? wr1 : wr5);
+        REAL poly = q == 0 ? base : FMA(cu[q], FMA((REAL)4.5, cu[q], (REAL)3), base);
+        REAL feq = wr * poly;
+        f[q] = FMA(P->tau_inv, feq - f[q], f[q]);
+        f[q] = FN(clampr)(f[q], (REAL)0, (REAL)10);
+    }
+}
+
+static inline void FN(collide)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin, REAL *mac)
+{
+    if (P->model == ORACLE_BGK)
+        FN(collide_bgk)(P, f, inlet, uin, mac);
+    else
+        FN(collide_mrt)(P, f, inlet, uin, mac);
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
+/* Streaming                                                                                              */
+/* ---------------------------------------------------------------------------------------------------- */
+
+/* neighbour of (x,y) displaced by (dx,dy); returns -1 when it leaves a channel domain */
+static inline long FN(nbr)(const FN(kpar_t) * P, int x, int y, int dx, int dy)
+{
+    int nx = x + dx, ny = y + dy;
+    if (P->bc == ORACLE_PERIODIC) {
+        if (nx < 0) nx += P->W;
+        if (nx >= P->W) nx -= P->W;
+        if (ny < 0) ny += P->H;
+        if (ny >= P->H) ny -= P->H;
+    } else if (nx < 0 || nx >= P->W || ny < 0 || ny >= P->H) {
+        return -1;
+    }
+    return (long)ny * P->W + nx;
+}
+
+/* 0 = not updated (solid, wall row, outlet ghost), 1 = fluid, 2 = inlet   (lbm_core4.cu:20-29,121) */
+static inline int FN(role)(const FN(kpar_t) * P, const uint8_t *mask, int x, int y)
+{
+    if (P->bc == ORACLE_PERIODIC) return mask[(long)y * P->W + x] ? 0 : 1;
+    if (y <= 0 || y >= P->H - 1 || x >= P->W - 1) return 0;
+    if (x == 0) return 2;
+    return mask[(long)y * P->W + x] ? 0 : 1;
+}
+
+/* One AA step in place.  parity 0: canonical -> reversed (local).  parity 1: reversed -> canonical. */
+void FN(oracle_aa_step)(const oracle_params *pp, const uint8_t *mask, REAL *f, const REAL *inlet_u, int parity,
+                        REAL *oux, REAL *ouy, REAL *orho)
+{
+    FN(kpar_t) P;
+    FN(make_kpar)(pp, &P);
+    const long plane = (long)P.W * P.H;
+    const int channel = P.bc == ORACLE_CHANNEL;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < P.H; ++y) {
+        for (int x = 0; x < P.W; ++x) {
+            int r = FN(role)(&P, mask, x, y);
+            if (!r) continue;
+            const int inlet = r == 2;
+            const long idx = (long)y * P.W + x;
+            REAL g[9], mac[3];
+            if (parity == 0) {
+                for (int q = 0; q < 9; ++q) g[q] = f[q * plane + idx];
+            } else {
+                for (int q = 0; q < 9; ++q) {
+                    if (inlet && CX[q] == 1) {
+                        g[q] = (REAL)0; /* from x = -1: rebuilt by Zou-He */
+                        continue;
+                    }
+                    long s = FN(nbr)(&P, x, y, -CX[q], -CY[q]);
+                    g[q] = mask[s] ? f[q * plane + idx] : f[OPP[q] * plane + s];
+                }
+            }
+            FN(collide)(&P, g, inlet, inlet ? inlet_u[y] : (REAL)0, mac);
+            if (parity == 0) {
+                for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx] = g[q];
+                if (channel && x == P.W - 2)
+                    for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx + 1] = g[q];
+            } else {
+                for (int q = 0; q < 9; ++q) {
+                    if (inlet && CX[q] == -1) continue; /* leaves through the open inlet */
+                    long d = FN(nbr)(&P, x, y, CX[q], CY[q]);
+                    if (mask[d])
+                        f[OPP[q] * plane + idx] = g[q];
+                    else
+                        f[q * plane + d] = g[q];
+                }
+                if (channel && x == P.W - 2) {
+                    /* the ghost (W-1,y) holds a copy of this cell; stream its q=3,6,7 into column W-2 */
+                    for (int q = 0; q < 9; ++q) {
+                        if (CX[q] != -1) continue;
+                        long d = (long)(y + CY[q]) * P.W + x;
+                        if (!mask[d]) f[q * plane + d] = g[q];
+                    }
+                }
+            }
+            if (orho) {
+                orho[idx] = mac[0];
+                oux[idx] = mac[1];
+                ouy[idx] = mac[2];
+            }
+        }
+    }
+}
+
+/* Reference-compatible pull step (lbm_core4.cu:120-211): f_out holds post-collision populations. */
+void FN(oracle_pull_step)(const oracle_params *pp, const uint8_t *mask, const REAL *f_now, REAL *f_out,
+                          const REAL *inlet_u, REAL *oux, REAL *ouy, REAL *orho)
+{
+    FN(kpar_t) P;
+    FN(make_kpar)(pp, &P);
+    const long plane = (long)P.W * P.H;
+    const int channel = P.bc == ORACLE_CHANNEL;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < P.H; ++y) {
+        for (int x = 0; x < P.W; ++x) {
+            int r = FN(role)(&P, mask, x, y);
+            if (!r) continue;
+            const int inlet = r == 2;
+            const long idx = (long)y * P.W + x;
+            REAL g[9], mac[3];
+            for (int q = 0; q < 9; ++q) {
+                if (inlet && CX[q] == 1) {
+                    g[q] = (REAL)0;
+                    continue;
+                }
+                long s = FN(nbr)(&P, x, y, -CX[q], -CY[q]);
+                g[q] = mask[s] ? f_now[OPP[q] * plane + idx] : f_now[q * plane + s];
+            }
+            FN(collide)(&P, g, inlet, inlet ? inlet_u[y] : (REAL)0, mac);
+            for (int q = 0; q < 9; ++q) {
+                f_out[q * plane + idx] = g[q];
+                if (channel && x == P.W - 2) f_out[q * plane + idx + 1] = g[q];
+            }
+            if (orho) {
+                orho[idx] = mac[0];
+                oux[idx] = mac[1];
+                ouy[idx] = mac[2];
+            }
+        }
+    }
+}
+
+/* rho, u of the stored state (lbm_core_smemaa_endfused.cu:377-422): canonical states get Zou-He at x=0,
+ * reversed states are read through opp().  Cells that are not updated are left untouched. */
+void FN(oracle_macro)(const oracle_params *pp, const uint8_t *mask, const REAL *f, const REAL *inlet_u, int parity,
+                      REAL *oux, REAL *ouy, REAL *orho)
+{
+    FN(kpar_t) P;
+    FN(make_kpar)(pp, &P);
+    const long plane = (long)P.W * P.H;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < P.H; ++y) {
+        for (int x = 0; x < P.W; ++x) {
+            int r = FN(role)(&P, mask, x, y);
+            if (!r) continue;
+            const long idx = (long)y * P.W + x;
+            REAL g[9];
+            for (int q = 0; q < 9; ++q) g[q] = f[(parity ? OPP[q] : q) * plane + idx];
+            REAL rho, ux, uy;
+            if (r == 2 && parity == 0) {
+                REAL u = inlet_u[y];
+                rho = FMA((REAL)2, (g[3] + g[6]) + g[7], (g[0] + g[2]) + g[4]) / ((REAL)1 - u);
+                ux = u;
+                uy = (REAL)0;
+            } else {
+                REAL s13 = g[1] + g[3], s24 = g[2] + g[4], s57 = g[5] + g[7], s68 = g[6] + g[8];
+                REAL a13 = g[1] - g[3], a24 = g[2] - g[4], a57 = g[5] - g[7], a86 = g[8] - g[6];
+                rho = (g[0] + (s13 + s24)) + (s57 + s68);
+                ux = (a13 + (a57 + a86)) / rho;
+                uy = (a24 + (a57 - a86)) / rho;
+            }
+            orho[idx] = rho;
+            oux[idx] = ux;
+            ouy[idx] = uy;
+        }
+    }
+}
+
+/* Exposed collision on one cell, for unit tests. */
+void FN(oracle_collide)(const oracle_params *pp, REAL *f9, int inlet, REAL uin, REAL *mac3)
+{
+    FN(kpar_t) P;
+    FN(make_kpar)(pp, &P);
+    FN(collide)(&P, f9, inlet, uin, mac3);
+}
+
+#undef FN
+#undef CAT
+#undef CAT_

@@ -1,0 +1,55 @@
+"""CPU checks of the C-ABI boundary: liblbm.so loads, exports every function include/lbm.h declares,
+the ctypes structure matches the C layout, and argument errors come back as codes (no GPU call made)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import lbm
+from lbm import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lbm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    for must in ("lbm_step_aa", "lbm_step_pull", "lbm_step_fused", "lbm_macro", "lbm_visualize",
+                 "lbm_diagnostics", "lbm_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = lbm.load()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_params_struct_layout():
+    # int32 W, H, model, bc, clamp; float tau_inv, cs, s_e, s_eps, s_q
+    assert ctypes.sizeof(_lib.LbmParams) == 40
+    assert _lib.LbmParams.tau_inv.offset == 20
+
+
+def test_argument_errors_return_codes():
+    L = lbm.load()
+    bad = _lib.params(2, 2, 1.5)
+    rc = L.lbm_step_aa(ctypes.byref(bad), None, None, None, 0, None, None, None, None)
+    assert rc == -2
+    assert b"too small" in L.lbm_last_error()
+    big = _lib.params(40000, 40000, 1.5)
+    assert L.lbm_step_aa(ctypes.byref(big), None, None, None, 0, None, None, None, None) == -2
+    p = _lib.params(16, 8, 2.5)
+    assert L.lbm_step_aa(ctypes.byref(p), None, None, None, 0, None, None, None, None) == -1
+    p = _lib.params(16, 8, 1.5)
+    assert L.lbm_step_aa(ctypes.byref(p), None, None, None, 0, None, None, None, None) == -1  # NULL mask
+    with pytest.raises(lbm.LbmError):
+        _lib.check(L.lbm_step_aa(ctypes.byref(p), None, None, None, 0, None, None, None, None), "lbm_step_aa")

@@ -1,0 +1,197 @@
+"""GPU parity of the single-step kernels against the oracle (FP32 mirror bit for bit, FP64 within the
+stated tolerance).  Everything goes through liblbm's C-ABI via lbm.Simulation / ctypes."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lbm import configs
+from lbm import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL_FP64 = 1e-6  # FP32 single step vs FP64 oracle, per population, per step
+
+
+def _oparams(case_or_sim, **kw):
+    s = case_or_sim
+    p = s.params
+    return O.params(p.W, p.H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc, clamp=p.clamp, s_e=p.s_e, s_eps=p.s_eps,
+                    s_q=p.s_q)
+
+
+def _sim(case, **kw):
+    from lbm.sim import Simulation
+    return Simulation(case.W, case.H, case.tau_inv, **case.sim_kwargs(), **kw)
+
+
+def _small_channel(W=96, H=40, model="mrt", cs=0.1, seed=4):
+    from lbm.sim import Simulation
+    mask = G.channel_mask(W, H, cylinder=(W // 4, H // 2, 5))
+    prof = G.parabolic_profile(H, 0.05)
+    f0 = G.channel_initial_state(W, H, prof, noise=1e-3, seed=seed)
+    return Simulation(W, H, 1.8, cs=cs, model=model, bc="channel", mask=mask, inlet_u=prof, f0=f0)
+
+
+def _small_periodic(W=80, H=48, cs=0.12, model="mrt"):
+    from lbm.sim import Simulation
+    mask = G.periodic_mask(W, H)
+    mask[20:26, 30:37] = 1
+    ux, uy = G.random_phase_velocity(W, H, k0=4, u_rms=0.04, seed=5)
+    return Simulation(W, H, 1.7, cs=cs, model=model, bc="periodic", mask=mask, f0=G.feq(np.ones((H, W)), ux, uy))
+
+
+def _check_steps_bitwise(gpu, sim, nsteps, macros=False):
+    """Every step: the GPU's output equals the FP32 mirror applied to the GPU's previous state."""
+    p = _oparams(sim)
+    mask = sim.mask_host
+    prof = sim.inlet_u_host
+    f_prev, par = sim.state()
+    for n in range(nsteps):
+        sim.step(1, macros=macros)
+        f_gpu, par_gpu = sim.state()
+        ref = f_prev.copy()
+        mac = O.aa_step(p, mask, ref, prof, par, macros=macros)
+        sel = O.observable(p, mask, par ^ 1)
+        assert par_gpu == par ^ 1
+        bad = np.count_nonzero(f_gpu[sel] != ref[sel])
+        assert bad == 0, f"step {n} parity {par}: {bad} populations differ, max {np.abs(f_gpu - ref)[sel].max()}"
+        if macros:
+            upd = sel[0] | O.observable(p, mask, 1)[0]
+            for a, b in zip(mac, (sim.rho, sim.ux, sim.uy)):
+                assert np.array_equal(a[upd], b.cpu().numpy()[upd]), f"macros step {n}"
+        f_prev, par = f_gpu, par_gpu
+
+
+@pytest.mark.parametrize("model,cs", [("mrt", 0.1), ("mrt", 0.0), ("bgk", 0.0)])
+def test_aa_step_bitwise_channel(gpu, model, cs):
+    _check_steps_bitwise(gpu, _small_channel(model=model, cs=cs), 12)
+
+
+def test_aa_step_macros_bitwise(gpu):
+    _check_steps_bitwise(gpu, _small_channel(), 6, macros=True)
+
+
+@pytest.mark.parametrize("model", ["mrt", "bgk"])
+def test_aa_step_bitwise_periodic(gpu, model):
+    _check_steps_bitwise(gpu, _small_periodic(model=model, cs=0.12 if model == "mrt" else 0.0), 12)
+
+
+def test_aa_step_odd_sizes(gpu):
+    """Lattices that do not fill the 64 x 4 tiles (ragged right and bottom edges)."""
+    from lbm.sim import Simulation
+    for W, H in ((3, 3), (65, 5), (131, 7), (70, 129)):
+        mask = G.channel_mask(W, H)
+        prof = G.parabolic_profile(H, 0.04)
+        f0 = G.channel_initial_state(W, H, prof, noise=1e-3, seed=W)
+        sim = Simulation(W, H, 1.6, cs=0.1, mask=mask, inlet_u=prof, f0=f0)
+        _check_steps_bitwise(gpu, sim, 4)
+
+
+def test_cylinder_small_1000_steps_vs_fp64(gpu):
+    """512 x 128 cylinder, 1000 single AA steps: at every step the GPU's FP32 output is within 1e-6
+    relative per population of the FP64 oracle advanced one step from the GPU's previous state, and equal
+    bit for bit to the FP32 mirror.  The free-running FP32 vs FP64 drift is reported, not gated."""
+    case = configs.cylinder_small()
+    sim = _sim(case)
+    p = _oparams(sim)
+    mask, prof32 = sim.mask_host, sim.inlet_u_host
+    prof64 = prof32.astype(np.float64)
+    f_prev, par = sim.state()
+    free64 = f_prev.astype(np.float64)
+    worst = 0.0
+    for n in range(case.steps):
+        sim.step(1)
+        f_gpu, par_gpu = sim.state()
+        sel = O.observable(p, mask, par ^ 1)
+        r64 = f_prev.astype(np.float64)
+        O.aa_step(p, mask, r64, prof64, par)
+        rel = np.abs(f_gpu[sel] - r64[sel]) / np.abs(r64[sel])
+        worst = max(worst, float(rel.max()))
+        assert rel.max() <= REL_TOL_FP64, f"step {n}: {rel.max():.3e}"
+        if n % 50 == 0 or n == case.steps - 1:
+            r32 = f_prev.copy()
+            O.aa_step(p, mask, r32, prof32, par)
+            assert np.array_equal(f_gpu[sel], r32[sel]), f"step {n}: FP32 mirror differs"
+        O.aa_step(p, mask, free64, prof64, par)
+        f_prev, par = f_gpu, par_gpu
+    sel = O.observable(p, mask, par)
+    drift = np.abs(f_prev[sel] - free64[sel]) / np.abs(free64[sel])
+    print(f"\nper-step worst rel err {worst:.3e}; free-running 1000-step drift max {drift.max():.3e}")
+
+
+def test_pull_step_bitwise(gpu):
+    """Reference-compatible two-buffer step (lbm_core4.cu semantics) vs the FP32 mirror, with macros."""
+    import torch
+    from lbm import _lib
+    sim = _small_channel()
+    p = _oparams(sim)
+    L = _lib.load()
+    f_now = sim.f.clone()
+    f_out = f_now.clone()
+    ux = torch.zeros((sim.H, sim.W), device=sim.device)
+    uy = torch.zeros_like(ux)
+    rho = torch.zeros_like(ux)
+    ref = f_now.cpu().numpy()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for n in range(8):
+        rc = L.lbm_step_pull(ctypes.byref(sim.params), ctypes.c_void_p(sim.mask.data_ptr()),
+                             ctypes.c_void_p(f_now.data_ptr()), ctypes.c_void_p(f_out.data_ptr()),
+                             ctypes.c_void_p(ux.data_ptr()), ctypes.c_void_p(uy.data_ptr()),
+                             ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(sim.inlet_u.data_ptr()), st)
+        _lib.check(rc, "lbm_step_pull")
+        ref_out = ref.copy()
+        mac = O.pull_step(p, sim.mask_host, ref, ref_out, sim.inlet_u_host)
+        sel = O.observable(p, sim.mask_host, 1)
+        got = f_out.cpu().numpy()
+        assert np.array_equal(got[sel], ref_out[sel]), f"step {n}"
+        for a, b in zip(mac, (rho, ux, uy)):
+            assert np.array_equal(a[sel[0]], b.cpu().numpy()[sel[0]])
+        f_now, f_out = f_out, f_now
+        ref = got
+
+
+@pytest.mark.parametrize("parity", [0, 1])
+def test_macro_bitwise(gpu, parity):
+    sim = _small_channel()
+    sim.step(5 + parity)
+    rho, ux, uy = (t.cpu().numpy() for t in sim.macro())
+    f, par = sim.state()
+    assert par == (5 + parity) & 1
+    p = _oparams(sim)
+    r, a, b = O.macro(p, sim.mask_host, f, sim.inlet_u_host, par)
+    upd = O.observable(p, sim.mask_host, 1)[0]
+    assert np.array_equal(r[upd], rho[upd]) and np.array_equal(a[upd], ux[upd]) and np.array_equal(b[upd], uy[upd])
+
+
+def test_visualize_matches_numpy(gpu):
+    sim = _small_channel()
+    sim.step(20)
+    img = sim.visualize(50.0).cpu().numpy()
+    ux, uy = sim.ux.cpu().numpy(), sim.uy.cpu().numpy()
+    H, W = ux.shape
+    vort = ((uy[1:-1, 2:] - uy[1:-1, :-2]) - (ux[2:, 1:-1] - ux[:-2, 1:-1])) * np.float32(50.0)
+    one = np.float32(1)
+    r = np.minimum(np.maximum(one + vort, 0), 1)
+    g = np.minimum(np.maximum(one - np.abs(vort), 0), 1)
+    b = np.minimum(np.maximum(one - vort, 0), 1)
+    want = np.stack([(r * np.float32(255)).astype(np.uint8), (g * np.float32(255)).astype(np.uint8),
+                     (b * np.float32(255)).astype(np.uint8)], -1)
+    solid = sim.mask_host[1:-1, 1:-1].astype(bool)
+    want[solid] = 0
+    assert np.array_equal(img[1:-1, 1:-1, :3], want)
+    assert np.all(img[1:-1, 1:-1, 3] == 255)
+
+
+@pytest.mark.parametrize("parity", [0, 1])
+def test_diagnostics_fp64(gpu, parity):
+    sim = _small_channel(W=200, H=64)
+    sim.step(30 + parity)
+    d = sim.diagnostics()
+    f, par = sim.state()
+    ref = O.diagnostics(_oparams(sim), sim.mask_host, f, sim.inlet_u_host, par)
+    assert abs(d[0] - ref[0]) <= 1e-12 * abs(ref[0])
+    assert np.all(np.abs(d[1:] - ref[1:]) <= 1e-12 * (1 + np.abs(ref[1:])))
+    d2 = sim.diagnostics()
+    assert np.array_equal(d, d2)  # deterministic reduction
