@@ -1,0 +1,233 @@
+"""CPU emulation of the temporally blocked kernel's schedule (lbm-cuda_amd/csrc/lbm_fused.hip, k_fused).
+
+Test infrastructure: it replays, for every workgroup, the kernel's sweep -- ring slots, per-step row lags
+and column/row margins, the load / step / write-back order inside an iteration -- vectorised over the strip
+columns, with bounds assertions on every ring and lattice index, and an FP64 collision from
+oracle/pyref.py.  Comparing its output with k FP64 oracle AA steps checks the schedule (which values feed
+which collision) on the CPU before the kernel runs on a GPU.
+"""
+import numpy as np
+
+from oracle import pyref
+
+LW = 128
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+OPP = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6])
+F_SOLID, F_COMPUTE, F_INLET, F_OUTADJ = 1, 2, 4, 8
+
+
+class Sched:
+    def __init__(self, K, P0):
+        self.K, self.P0 = K, P0
+        self.D0 = 1 + P0
+        self.LASTB = (P0 + K - 1) & 1
+        self.DLAST = self.D0 + 2 * (K - 1)
+        self.DOUT = self.DLAST + 1 + self.LASTB
+        self.R = self.DOUT + 1
+        self.HALO = 2 * self.nodd_before(K)
+        self.CW = LW - 2 * self.HALO
+
+    def d(self, s):
+        return self.D0 + 2 * s
+
+    def nodd_before(self, s):
+        return ((self.P0 + s) >> 1) - (self.P0 >> 1)
+
+    def margin(self, s):
+        return 2 * self.nodd_before(s) + ((self.P0 + s) & 1)
+
+
+def pick_chunk(W, H, k):
+    CW = LW - 2 * (k + 1)
+    strips = (W + CW - 1) // CW
+    ch = 256
+    while ch > 16 and strips * ((H + ch - 1) // ch) < 512:
+        ch //= 2
+    return ch if ch < H else H
+
+
+def _collide(g, inlet, uin, p):
+    """FP64 collision of a [9, n] block (pyref's transcription of lbm_core4.cu / lbm_core_smemaa.cu)."""
+    tau_inv, cs, model, clamp = p["tau_inv"], p["cs"], p["model"], p["clamp"]
+    out = np.empty_like(g)
+    fl = ~inlet
+    if fl.any():
+        gi = g[:, fl]
+        rho = gi.sum(0)
+        ux = (gi * CX[:, None]).sum(0) / rho
+        uy = (gi * CY[:, None]).sum(0) / rho
+        if model == 1:
+            out[:, fl] = _bgk(gi, rho, ux, uy, tau_inv)
+        else:
+            if clamp:
+                ux = np.clip(ux, -0.4, 0.4)
+                uy = np.clip(uy, -0.4, 0.4)
+                rho = np.clip(rho, 0.0, 1.5)
+            om = np.full(rho.shape, tau_inv)
+            if cs != 0.0:
+                m7 = gi[1] - gi[2] + gi[3] - gi[4]
+                m8 = gi[5] - gi[6] + gi[7] - gi[8]
+                n7 = m7 - rho * (ux * ux - uy * uy)
+                n8 = m8 - rho * ux * uy
+                Q = np.sqrt(n7 * n7 + n8 * n8 + 1e-8)
+                t0 = 1.0 / tau_inv
+                om = 1.0 / (0.5 * (t0 + np.sqrt(t0 * t0 + 18.0 * cs * cs * Q / rho)))
+            out[:, fl] = pyref._mrt(gi, rho, ux, uy, om, p["s_e"], p["s_eps"], p["s_q"])
+    if inlet.any():
+        gz, rz = pyref._zou_he(g[:, inlet], uin[inlet])
+        u = uin[inlet]
+        if model == 1:
+            out[:, inlet] = _bgk(gz, rz, u, np.zeros_like(u), tau_inv)
+        else:
+            out[:, inlet] = pyref._mrt(gz, rz, u, np.zeros_like(u), tau_inv, p["s_e"], p["s_eps"], p["s_q"])
+    return out
+
+
+def _bgk(f, rho, ux, uy, tau_inv):
+    usq = ux * ux + uy * uy
+    out = np.empty_like(f)
+    for q in range(9):
+        cu = CX[q] * ux + CY[q] * uy
+        feq = pyref.WQ[q] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq)
+        out[q] = f[q] + tau_inv * (feq - f[q])
+    return np.minimum(10.0, np.maximum(0.0, out))
+
+
+def fused(f_in, mask, inlet_u, parity, K, p, periodic):
+    """k_fused on the whole grid; returns f_out (cells not written keep NaN)."""
+    _, H, W = f_in.shape
+    S = Sched(K, parity)
+    CH = pick_chunk(W, H, K)
+    out = np.full_like(f_in, np.nan)
+    gx = (W + S.CW - 1) // S.CW
+    gy_n = (H + CH - 1) // CH
+    for bx in range(gx):
+        for by in range(gy_n):
+            _block(f_in, out, mask, inlet_u, S, CH, bx, by, p, periodic)
+    return out
+
+
+def _block(fin, fout, mask, inlet_u, S, CH, bx, by, p, periodic):
+    _, H, W = fin.shape
+    R, HALO, CW, K, P0 = S.R, S.HALO, S.CW, S.K, S.P0
+    sf = np.full((9, R, LW), np.nan)
+    sflag = np.zeros((R, LW), np.uint8)
+    xs = bx * CW - HALO
+    y0 = by * CH
+    ch = min(CH, H - y0)
+    ybeg, yend = y0 - HALO, y0 + ch + HALO - 1
+    niter = ch + HALO + S.DOUT
+    cols = np.arange(LW)
+
+    def load_row(j):
+        y = ybeg + j
+        slot = j % R
+        assert 0 <= slot < R
+        rowin = periodic or (0 <= y < H)
+        gy = y % H if periodic else y
+        x = xs + cols
+        if periodic:
+            x = x % W
+            colin = np.ones(LW, bool)
+        else:
+            colin = (x >= 0) & (x < W)
+        vals = np.zeros((9, LW))
+        fl = np.zeros(LW, np.uint8)
+        if rowin:
+            xi = x[colin]
+            assert xi.min(initial=0) >= 0 and xi.max(initial=0) < W and 0 <= gy < H
+            vals[:, colin] = fin[:, gy, xi]
+            m = np.zeros(LW, bool)
+            m[colin] = mask[gy, xi] != 0
+            if periodic:
+                fl[:] = np.where(m, F_SOLID, F_COMPUTE)
+            else:
+                fl[colin & m] |= F_SOLID
+                if 1 <= gy <= H - 2:
+                    inner = colin & (x <= W - 2)
+                    fl[inner & (x == 0)] |= F_COMPUTE | F_INLET
+                    fl[inner & (x != 0) & ~m] |= F_COMPUTE
+                    fl[inner & (x == W - 2)] |= F_OUTADJ
+        sf[:, slot, :] = vals
+        sflag[slot, :] = fl
+
+    for i in range(niter):
+        if i <= yend - ybeg:
+            load_row(i)
+        # all steps of this iteration read the ring state left by the previous iteration's barrier;
+        # rows touched by different steps are disjoint, so applying them in order is equivalent
+        touched = {}
+        for s in range(K):
+            jrow = i - S.d(s)
+            m = S.margin(s)
+            odd = (P0 + s) & 1
+            if not (jrow >= m and ybeg + jrow <= yend - m):
+                continue
+            rows = (jrow - 1, jrow, jrow + 1) if odd else (jrow,)
+            for r in rows:
+                assert r not in touched, f"iteration {i}: row {r} touched by steps {touched[r]} and {s}"
+                touched[r] = s
+            c = np.arange(m, LW - m)
+            sl = jrow % R
+            fl = sflag[sl, c]
+            comp = (fl & F_COMPUTE) != 0
+            c = c[comp]
+            fl = fl[comp]
+            if c.size == 0:
+                continue
+            inlet = (fl & F_INLET) != 0
+            y = ybeg + jrow
+            gy = y % H if periodic else y
+            uin = np.where(inlet, inlet_u[gy] if 0 <= gy < H else 0.0, 0.0)
+            if not odd:
+                g = sf[:, sl, c].copy()
+                o = _collide(g, inlet, uin, p)
+                sf[OPP[:, None], sl, c[None, :]] = o
+                oa = (fl & F_OUTADJ) != 0
+                if not periodic and oa.any():
+                    cc = c[oa] + 1
+                    keep = cc < LW
+                    sf[OPP[:, None], sl, cc[keep][None, :]] = o[:, oa][:, keep]
+            else:
+                su, sd = (jrow + 1) % R, (jrow - 1) % R
+                rs = [sl if CY[q] == 0 else (su if CY[q] > 0 else sd) for q in range(9)]
+                assert c.min() >= 1 and c.max() <= LW - 2
+                solid = np.zeros((9, c.size), bool)
+                for q in range(1, 9):
+                    sk = (not periodic) & inlet & (CX[q] < 0)
+                    solid[q] = np.where(sk, False, (sflag[rs[q], c + CX[q]] & F_SOLID) != 0)
+                g = np.empty((9, c.size))
+                g[0] = sf[0, sl, c]
+                for q in range(1, 9):
+                    o_ = OPP[q]
+                    v = np.where(solid[o_], sf[q, sl, c], sf[o_, rs[o_], c + CX[o_]])
+                    if not periodic:
+                        v = np.where(inlet & (CX[q] > 0), 0.0, v)
+                    g[q] = v
+                o = _collide(g, inlet, uin, p)
+                sf[0, sl, c] = o[0]
+                for q in range(1, 9):
+                    drop = (inlet & (CX[q] < 0)) if not periodic else np.zeros(c.size, bool)
+                    bb = solid[q] & ~drop
+                    nb = ~solid[q] & ~drop
+                    sf[OPP[q], sl, c[bb]] = o[q, bb]
+                    sf[q, rs[q], c[nb] + CX[q]] = o[q, nb]
+                oa = (fl & F_OUTADJ) != 0
+                if not periodic and oa.any():
+                    sf[3, sl, c[oa]] = o[3, oa]
+                    nu = oa & ~solid[2]
+                    sf[6, su, c[nu]] = o[6, nu]
+                    nd = oa & ~solid[4]
+                    sf[7, sd, c[nd]] = o[7, nd]
+        assert i not in touched, f"iteration {i}: the row being loaded is also stepped"
+        jout = i - S.DOUT
+        if HALO <= jout < HALO + ch:
+            assert jout not in touched
+            sl = jout % R
+            y = ybeg + jout
+            cc = np.arange(CW)
+            x = xs + HALO + cc
+            ok = x < W
+            assert 0 <= y < H
+            fout[:, y, x[ok]] = sf[:, sl, HALO + cc[ok]]

@@ -53,3 +53,7 @@ def test_argument_errors_return_codes():
     assert L.lbm_step_aa(ctypes.byref(p), None, None, None, 0, None, None, None, None) == -1  # NULL mask
     with pytest.raises(lbm.LbmError):
         _lib.check(L.lbm_step_aa(ctypes.byref(p), None, None, None, 0, None, None, None, None), "lbm_step_aa")
+
+
+def test_fused_max_k():
+    assert lbm.load().lbm_fused_max_k() == 8

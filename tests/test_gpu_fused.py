@@ -1,0 +1,81 @@
+"""GPU: the temporally blocked kernel equals k single AA steps bit for bit (observable state)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from lbm import configs
+from lbm import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(W, H, periodic, model, seed=3):
+    from lbm.sim import Simulation
+    if periodic:
+        mask = G.periodic_mask(W, H)
+        mask[H // 3:H // 3 + 5, W // 2:W // 2 + 6] = 1
+        ux, uy = G.random_phase_velocity(W, H, 4, 0.04, seed=seed)
+        kw = dict(bc="periodic", mask=mask, f0=G.feq(np.ones((H, W)), ux, uy))
+    else:
+        mask = G.channel_mask(W, H, cylinder=(W // 4, H // 2, max(2, H // 10)))
+        prof = G.parabolic_profile(H, 0.05)
+        kw = dict(bc="channel", mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, seed))
+    cs = 0.12 if model == "mrt" else 0.0
+    return (Simulation(W, H, 1.75, cs=cs, model=model, **kw), Simulation(W, H, 1.75, cs=cs, model=model, **kw))
+
+
+def _sel(sim):
+    p = sim.params
+    op = O.params(p.W, p.H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc)
+    return [torch.from_numpy(O.observable(op, sim.mask_host, par)).to(sim.device) for par in (0, 1)]
+
+
+def _assert_equal(a, b, sel, what):
+    diff = (a.f != b.f) & sel[a.parity]
+    n = int(diff.sum())
+    assert a.parity == b.parity
+    assert n == 0, f"{what}: {n} populations differ"
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("periodic,model", [(False, "mrt"), (True, "mrt"), (False, "bgk")])
+def test_fused_equals_single_small(gpu, k, periodic, model):
+    W, H = (333, 97) if not periodic else (300, 70)
+    single, fusedsim = _pair(W, H, periodic, model)
+    sel = _sel(single)
+    for start_parity in (0, 1):
+        if start_parity == 1:
+            single.step(1)
+            fusedsim.step(1)
+        for launch in range(3):
+            single.step(k)
+            fusedsim.step(k, k=k)
+            _assert_equal(single, fusedsim, sel, f"k={k} launch {launch} start parity {start_parity}")
+
+
+def test_fused_macros_match_single(gpu):
+    single, fusedsim = _pair(200, 64, False, "mrt")
+    single.step(3)
+    single.step(1, macros=True)
+    fusedsim.step(4, k=4, macros=True)
+    upd = _sel(single)[1][0]
+    for a, b in ((single.rho, fusedsim.rho), (single.ux, fusedsim.ux), (single.uy, fusedsim.uy)):
+        assert torch.equal(a[upd], b[upd])
+
+
+@pytest.mark.parametrize("k", [2, 4, 8])
+def test_fused_equals_single_cylinder_large(gpu, k):
+    """4096 x 1024 cylinder, 10000 steps: bitwise equal to single steps at every launch boundary."""
+    from lbm.sim import Simulation
+    case = configs.cylinder_large()
+    a = Simulation(case.W, case.H, case.tau_inv, **case.sim_kwargs())
+    b = Simulation(case.W, case.H, case.tau_inv, **case.sim_kwargs())
+    sel = _sel(a)
+    for launch in range(case.steps // k):
+        a.step(k)
+        b.step(k, k=k)
+        if launch % 25 == 0 or launch == case.steps // k - 1:
+            _assert_equal(a, b, sel, f"launch {launch}")
+    d = a.diagnostics()
+    assert np.isfinite(d).all()
