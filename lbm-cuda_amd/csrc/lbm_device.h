@@ -2,8 +2,9 @@
 //
 // One __device__ collide function is used by the single-step, pull and temporally blocked kernels, so
 // that k fused steps are bit-identical to k single steps.  The code is compiled with -ffp-contract=off:
-// every fused multiply-add is written out (__builtin_fmaf), divisions and square roots are the correctly
-// rounded __fdiv_rn / __fsqrt_rn, and nothing is reassociated.
+// every fused multiply-add is written out (__builtin_fmaf), divisions are the correctly rounded __fdiv_rn
+// and square roots sqrtf (which hipcc expands to a correctly rounded sequence on gfx950; __fsqrt_rn lowers
+// to the bare v_sqrt_f32, which is not), and nothing is reassociated.
 //
 // Physics follows lbm_core4.cu:29-210 (MRT + Smagorinsky, Zou-He inlet, clamps) and
 // lbm_core_smemaa.cu:159-220 (BGK); the operation order is this project's own (fewer VALU operations:
@@ -115,10 +116,10 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float f[9], bool inle
     float w = P.tau_inv;
     if (les) {
         // Smagorinsky (lbm_core4.cu:172-177)
-        const float Q = __fsqrt_rn(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
+        const float Q = sqrtf(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
         const float inv_rc = __fdiv_rn(1.0f, rc);
         const float d = fma_(P.c18cs2, Q * inv_rc, P.tau0sq);
-        const float tau_eff = 0.5f * (P.tau0 + __fsqrt_rn(d));
+        const float tau_eff = 0.5f * (P.tau0 + sqrtf(d));
         w = __fdiv_rn(1.0f, tau_eff);
     }
     // relaxation (lbm_core4.cu:180-185)
