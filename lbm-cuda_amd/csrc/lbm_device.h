@@ -46,69 +46,81 @@ struct KPar {
 __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
 
-// Zou-He velocity inlet (lbm_core4.cu:54-57): rebuilds f1, f5, f8 from the known populations.
-__device__ __forceinline__ float zou_he(float f[9], float u)
+// The collision works on deviations h_q = f_q - w_q from the rest state (rho = 1, u = 0).  The rest
+// state's moments (1, -2, 1, 0, ..., 0) are carried exactly and only the small remainders are rounded,
+// which keeps the FP32 result within a few 1e-8 of FP64 and is the form FP16/BF16 storage keeps in memory.
+// Every formula below is the reference's (cited), rewritten for deviations.
+__device__ __forceinline__ float wq(int q) { return q == 0 ? LBM_W0 : (q < 5 ? LBM_W1 : LBM_W5); }
+
+// Zou-He velocity inlet (lbm_core4.cu:54-57) on deviations: rebuilds h1, h5, h8; returns rho - 1.
+// f0+f2+f4+2(f3+f6+f7) = 1 + (h0+h2+h4+2(h3+h6+h7)), so rho - 1 = (H + u) / (1 - u).
+__device__ __forceinline__ float zou_he(float h[9], float u, float &rho)
 {
-    float num = fma_(2.0f, (f[3] + f[6]) + f[7], (f[0] + f[2]) + f[4]);
-    float rho = __fdiv_rn(num, 1.0f - u);
-    float ru = rho * u;
-    float h = 0.5f * (f[2] - f[4]);
-    f[1] = fma_(LBM_C_2_3, ru, f[3]);
-    f[5] = fma_(LBM_C_1_6, ru, f[7] - h);
-    f[8] = fma_(LBM_C_1_6, ru, f[6] + h);
-    return rho;
+    const float Hs = fma_(2.0f, (h[3] + h[6]) + h[7], (h[0] + h[2]) + h[4]);
+    const float drho = __fdiv_rn(Hs + u, 1.0f - u);
+    rho = 1.0f + drho;
+    const float ru = rho * u;
+    const float hh = 0.5f * (h[2] - h[4]);
+    h[1] = fma_(LBM_C_2_3, ru, h[3]);
+    h[5] = fma_(LBM_C_1_6, ru, h[7] - hh);
+    h[8] = fma_(LBM_C_1_6, ru, h[6] + hh);
+    return drho;
 }
 
-// MRT + Smagorinsky collision (lbm_core4.cu:124-210).  `inlet` selects Zou-He and the inlet collision
-// of lbm_core4.cu:54-117 (no LES, no clamps, m3 = rho u, m5 = 0).  mac = (rho, ux, uy) as written by the
-// reference.  f is replaced by the clamped post-collision populations.
-__device__ __forceinline__ void collide_mrt(const KPar &P, float f[9], bool inlet, float uin, float mac[3])
+// MRT + Smagorinsky collision (lbm_core4.cu:124-199) on deviations, h -> h' (unclamped).  `inlet`
+// selects Zou-He and the inlet collision of lbm_core4.cu:54-93 (no LES, no clamps, m3 = rho u, m5 = 0).
+// mac = (rho, ux, uy) as written by the reference (lbm_core4.cu:59-61,145-147).
+__device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3])
 {
-    float rho = 0.f, ux = 0.f, uy = 0.f, rc = 0.f;
+    float rho = 0.f, ux = 0.f, uy = 0.f, drc = 0.f, rc = 0.f;
     bool les = false;
     if (inlet) {
-        rho = zou_he(f, uin);
+        drc = zou_he(h, uin, rho);
+        rc = rho;
         ux = uin;
         uy = 0.0f;
-        rc = rho;
     }
-    const float s13 = f[1] + f[3], s24 = f[2] + f[4], s57 = f[5] + f[7], s68 = f[6] + f[8];
-    const float f14 = s13 + s24, f58 = s57 + s68;
-    const float a13 = f[1] - f[3], a24 = f[2] - f[4], a57 = f[5] - f[7], a86 = f[8] - f[6];
+    const float s13 = h[1] + h[3], s24 = h[2] + h[4], s57 = h[5] + h[7], s68 = h[6] + h[8];
+    const float h14 = s13 + s24, h58 = s57 + s68;
+    const float a13 = h[1] - h[3], a24 = h[2] - h[4], a57 = h[5] - h[7], a86 = h[8] - h[6];
     const float sx = a57 + a86; // f5 - f6 - f7 + f8
     const float sy = a57 - a86; // f5 + f6 - f7 - f8
     if (!inlet) {
-        rho = (f[0] + f14) + f58;
+        // moments and clamps (lbm_core4.cu:130-144); rho = 1 + drho
+        const float drho = (h[0] + h14) + h58;
+        rho = 1.0f + drho;
         const float jx = a13 + sx, jy = a24 + sy;
         const float inv_rho = __fdiv_rn(1.0f, rho);
         ux = jx * inv_rho;
         uy = jy * inv_rho;
-        rc = rho;
+        drc = drho;
         if (P.clamp) {
             ux = clampf(ux, -0.4f, 0.4f);
             uy = clampf(uy, -0.4f, 0.4f);
-            rc = clampf(rho, 0.0f, 1.5f);
+            drc = clampf(drho, -1.0f, 0.5f); // rho in [0, 1.5]
         }
+        rc = 1.0f + drc;
         les = P.les != 0;
     }
     mac[0] = rc;
     mac[1] = ux;
     mac[2] = uy;
-    // forward transform, Lallemand-Luo order (lbm_core4.cu:152-165)
-    const float m0 = rc;
-    float m1 = fma_(2.0f, f58, fma_(-4.0f, f[0], -f14));
-    float m2 = fma_(-2.0f, f14, fma_(4.0f, f[0], f58));
+    // forward transform (lbm_core4.cu:152-165): m1 = -2 + m1d, m2 = 1 + m2d, the rest carry no rest part
+    float m1d = fma_(2.0f, h58, fma_(-4.0f, h[0], -h14));
+    float m2d = fma_(-2.0f, h14, fma_(4.0f, h[0], h58));
     const float m3 = rc * ux;
     float m4 = fma_(-2.0f, a13, sx);
     const float m5 = rc * uy;
     float m6 = fma_(-2.0f, a24, sy);
     float m7 = s13 - s24;
     float m8 = s57 - s68;
-    // equilibria (lbm_core4.cu:166-170)
+    // non-equilibrium parts (lbm_core4.cu:166-170): m1 - (3u^2 - 2) rho = m1d + 2 (rho - 1) - 3u^2 rho,
+    // m2 - (1 - 3u^2) rho = m2d - (rho - 1) + 3u^2 rho
     const float ux2 = ux * ux, uy2 = uy * uy;
     const float usq = ux2 + uy2;
-    const float m1eq = fma_(3.0f, usq, -2.0f) * rc;
-    const float m2eq = fma_(-3.0f, usq, 1.0f) * rc;
+    const float u3 = 3.0f * usq;
+    const float n1 = fma_(-u3, rc, fma_(2.0f, drc, m1d));
+    const float n2 = fma_(u3, rc, m2d - drc);
     const float m7eq = rc * (ux2 - uy2);
     const float m8eq = m3 * uy;
     const float n7 = m7 - m7eq;
@@ -123,16 +135,16 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float f[9], bool inle
         w = __fdiv_rn(1.0f, tau_eff);
     }
     // relaxation (lbm_core4.cu:180-185)
-    m1 = fma_(-P.s_e, m1 - m1eq, m1);
-    m2 = fma_(-P.s_eps, m2 - m2eq, m2);
+    m1d = fma_(-P.s_e, n1, m1d);
+    m2d = fma_(-P.s_eps, n2, m2d);
     m4 = fma_(-P.s_q, m4 + m3, m4);
     m6 = fma_(-P.s_q, m6 + m5, m6);
     m7 = fma_(-w, n7, m7);
     m8 = fma_(-w, n8, m8);
-    // inverse (lbm_core4.cu:187-199)
-    const float t0 = (m0 - m1) + m2;
-    const float t1 = fma_(-2.0f, m2, fma_(4.0f, m0, -m1));
-    const float t2 = fma_(4.0f, m0, fma_(2.0f, m1, m2));
+    // inverse (lbm_core4.cu:187-199) of the deviation moments (rho - 1, m1d, m2d, m3, ..., m8)
+    const float t0 = (drc - m1d) + m2d;
+    const float t1 = fma_(-2.0f, m2d, fma_(4.0f, drc, -m1d));
+    const float t2 = fma_(4.0f, drc, fma_(2.0f, m1d, m2d));
     const float A = (m3 - m4) * 6.0f;
     const float B = (m5 - m6) * 6.0f;
     const float Pp = m7 * 9.0f;
@@ -142,29 +154,31 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float f[9], bool inle
     const float t1p = t1 + Pp, t1m = t1 - Pp;
     const float u = t2 + Z, v = t2 - Z;
     const float XpY = X + Y, XmY = X - Y;
-    f[0] = clampf(t0 * LBM_INV9, 0.0f, 10.0f);
-    f[1] = clampf((t1p + A) * LBM_INV36, 0.0f, 10.0f);
-    f[2] = clampf((t1m + B) * LBM_INV36, 0.0f, 10.0f);
-    f[3] = clampf((t1p - A) * LBM_INV36, 0.0f, 10.0f);
-    f[4] = clampf((t1m - B) * LBM_INV36, 0.0f, 10.0f);
-    f[5] = clampf((u + XpY) * LBM_INV36, 0.0f, 10.0f);
-    f[6] = clampf((v - XmY) * LBM_INV36, 0.0f, 10.0f);
-    f[7] = clampf((u - XpY) * LBM_INV36, 0.0f, 10.0f);
-    f[8] = clampf((v + XmY) * LBM_INV36, 0.0f, 10.0f);
+    h[0] = t0 * LBM_INV9;
+    h[1] = (t1p + A) * LBM_INV36;
+    h[2] = (t1m + B) * LBM_INV36;
+    h[3] = (t1p - A) * LBM_INV36;
+    h[4] = (t1m - B) * LBM_INV36;
+    h[5] = (u + XpY) * LBM_INV36;
+    h[6] = (v - XmY) * LBM_INV36;
+    h[7] = (u - XpY) * LBM_INV36;
+    h[8] = (v + XmY) * LBM_INV36;
 }
 
-// BGK collision (lbm_core_smemaa.cu:159-220): no velocity/density clamps, populations clamped to [0,10].
-__device__ __forceinline__ void collide_bgk(const KPar &P, float f[9], bool inlet, float uin, float mac[3])
+// BGK collision (lbm_core_smemaa.cu:159-212) on deviations: h' = h + tau_inv (heq - h) with
+// heq_q = feq_q - w_q = w_q (drho + rho (3cu + 4.5cu^2 - 1.5u^2)).  No velocity/density clamps.
+__device__ __forceinline__ void collide_bgk(const KPar &P, float h[9], bool inlet, float uin, float mac[3])
 {
-    float rho, ux, uy;
+    float rho, drho, ux, uy;
     if (inlet) {
-        rho = zou_he(f, uin);
+        drho = zou_he(h, uin, rho);
         ux = uin;
         uy = 0.0f;
     } else {
-        const float s13 = f[1] + f[3], s24 = f[2] + f[4], s57 = f[5] + f[7], s68 = f[6] + f[8];
-        const float a13 = f[1] - f[3], a24 = f[2] - f[4], a57 = f[5] - f[7], a86 = f[8] - f[6];
-        rho = (f[0] + (s13 + s24)) + (s57 + s68);
+        const float s13 = h[1] + h[3], s24 = h[2] + h[4], s57 = h[5] + h[7], s68 = h[6] + h[8];
+        const float a13 = h[1] - h[3], a24 = h[2] - h[4], a57 = h[5] - h[7], a86 = h[8] - h[6];
+        drho = (h[0] + (s13 + s24)) + (s57 + s68);
+        rho = 1.0f + drho;
         const float inv_rho = __fdiv_rn(1.0f, rho);
         ux = (a13 + (a57 + a86)) * inv_rho;
         uy = (a24 + (a57 - a86)) * inv_rho;
@@ -172,26 +186,34 @@ __device__ __forceinline__ void collide_bgk(const KPar &P, float f[9], bool inle
     mac[0] = rho;
     mac[1] = ux;
     mac[2] = uy;
-    const float usq = fma_(uy, uy, ux * ux);
-    const float base = fma_(-1.5f, usq, 1.0f);
-    const float wr0 = LBM_W0 * rho, wr1 = LBM_W1 * rho, wr5 = LBM_W5 * rho;
+    const float usq15 = 1.5f * fma_(uy, uy, ux * ux);
     const float cu[9] = {0.0f, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy};
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        const float wr = q == 0 ? wr0 : (q < 5 ? wr1 : wr5);
-        const float poly = q == 0 ? base : fma_(cu[q], fma_(4.5f, cu[q], 3.0f), base);
-        const float feq = wr * poly;
-        f[q] = clampf(fma_(P.tau_inv, feq - f[q], f[q]), 0.0f, 10.0f);
+        const float poly = q == 0 ? -usq15 : fma_(cu[q], fma_(4.5f, cu[q], 3.0f), -usq15);
+        const float heq = wq(q) * fma_(rho, poly, drho);
+        h[q] = fma_(P.tau_inv, heq - h[q], h[q]);
     }
 }
 
 template <int MODEL>
-__device__ __forceinline__ void collide(const KPar &P, float f[9], bool inlet, float uin, float mac[3])
+__device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inlet, float uin, float mac[3])
 {
     if constexpr (MODEL == MODEL_BGK)
-        collide_bgk(P, f, inlet, uin, mac);
+        collide_bgk(P, h, inlet, uin, mac);
     else
-        collide_mrt(P, f, inlet, uin, mac);
+        collide_mrt(P, h, inlet, uin, mac);
+}
+
+// FP32 storage: f -> h = f - w -> collide -> f' = clamp(w + h', 0, 10)   (lbm_core4.cu:203)
+template <int MODEL>
+__device__ __forceinline__ void collide(const KPar &P, float f[9], bool inlet, float uin, float mac[3])
+{
+#pragma unroll
+    for (int q = 0; q < 9; ++q) f[q] = f[q] - wq(q);
+    collide_dev<MODEL>(P, f, inlet, uin, mac);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) f[q] = clampf(wq(q) + f[q], 0.0f, 10.0f);
 }
 
 } // namespace lbm

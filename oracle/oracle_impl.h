@@ -73,43 +73,54 @@ static inline REAL FN(clampr)(REAL v, REAL lo, REAL hi)
 /* mac (may be NULL) receives (rho, ux, uy) as the reference writes them (lbm_core4.cu:59-61,145-147).     */
 /* ---------------------------------------------------------------------------------------------------- */
 
-static void FN(collide_mrt)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin, REAL *mac)
+/* Deviation form: h_q = f_q - w_q (rest state rho = 1, u = 0, whose moments 1, -2, 1, 0.. are exact). */
+static inline REAL FN(wq)(int q) { return q == 0 ? C_W0 : (q < 5 ? C_W1 : C_W5); }
+
+/* Zou-He (lbm_core4.cu:54-57) on deviations; returns rho - 1 and sets *rho. */
+static inline REAL FN(zou_he)(REAL *h, REAL uin, REAL *rho)
 {
-    REAL rho = 0, ux = 0, uy = 0, rc = 0;
+    REAL Hs = FMA((REAL)2, (h[3] + h[6]) + h[7], (h[0] + h[2]) + h[4]);
+    REAL drho = (Hs + uin) / ((REAL)1 - uin);
+    *rho = (REAL)1 + drho;
+    REAL ru = *rho * uin;
+    REAL hh = (REAL)0.5 * (h[2] - h[4]);
+    h[1] = FMA(C_2_3, ru, h[3]);
+    h[5] = FMA(C_1_6, ru, h[7] - hh);
+    h[8] = FMA(C_1_6, ru, h[6] + hh);
+    return drho;
+}
+
+static void FN(collide_mrt)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin, REAL *mac)
+{
+    REAL rho = 0, ux = 0, uy = 0, drc = 0, rc = 0;
     int les = 0;
     if (inlet) {
-        /* Zou-He, lbm_core4.cu:54-57: rho = (f0+f2+f4+2(f3+f6+f7))/(1-u) */
-        REAL num = FMA((REAL)2, (f[3] + f[6]) + f[7], (f[0] + f[2]) + f[4]);
-        rho = num / ((REAL)1 - uin);
-        REAL ru = rho * uin;
-        REAL h = (REAL)0.5 * (f[2] - f[4]);
-        f[1] = FMA(C_2_3, ru, f[3]);
-        f[5] = FMA(C_1_6, ru, f[7] - h);
-        f[8] = FMA(C_1_6, ru, f[6] + h);
+        drc = FN(zou_he)(h, uin, &rho);
+        rc = rho;
         ux = uin;
         uy = (REAL)0;
-        rc = rho;
-        les = 0;
     }
-    REAL s13 = f[1] + f[3], s24 = f[2] + f[4], s57 = f[5] + f[7], s68 = f[6] + f[8];
-    REAL f14 = s13 + s24, f58 = s57 + s68;
-    REAL a13 = f[1] - f[3], a24 = f[2] - f[4], a57 = f[5] - f[7], a86 = f[8] - f[6];
+    REAL s13 = h[1] + h[3], s24 = h[2] + h[4], s57 = h[5] + h[7], s68 = h[6] + h[8];
+    REAL h14 = s13 + s24, h58 = s57 + s68;
+    REAL a13 = h[1] - h[3], a24 = h[2] - h[4], a57 = h[5] - h[7], a86 = h[8] - h[6];
     REAL sx = a57 + a86; /* f5 - f6 - f7 + f8 */
     REAL sy = a57 - a86; /* f5 + f6 - f7 - f8 */
     if (!inlet) {
-        /* moments, lbm_core4.cu:130-144 */
-        rho = (f[0] + f14) + f58;
+        /* moments and clamps, lbm_core4.cu:130-144 (rho = 1 + drho) */
+        REAL drho = (h[0] + h14) + h58;
+        rho = (REAL)1 + drho;
         REAL jx = a13 + sx;
         REAL jy = a24 + sy;
         REAL inv_rho = (REAL)1 / rho;
         ux = jx * inv_rho;
         uy = jy * inv_rho;
-        rc = rho;
+        drc = drho;
         if (P->clamp) {
             ux = FN(clampr)(ux, (REAL)-0.4, (REAL)0.4);
             uy = FN(clampr)(uy, (REAL)-0.4, (REAL)0.4);
-            rc = FN(clampr)(rho, (REAL)0, (REAL)1.5);
+            drc = FN(clampr)(drho, (REAL)-1, (REAL)0.5); /* rho in [0, 1.5] */
         }
+        rc = (REAL)1 + drc;
         les = P->les;
     }
     if (mac) {
@@ -117,21 +128,22 @@ static void FN(collide_mrt)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin
         mac[1] = ux;
         mac[2] = uy;
     }
-    /* MRT forward, Lallemand-Luo order (rho, e, eps, jx, qx, jy, qy, pxx, pxy), lbm_core4.cu:152-165 */
-    REAL m0 = rc;
-    REAL m1 = FMA((REAL)2, f58, FMA((REAL)-4, f[0], -f14));
-    REAL m2 = FMA((REAL)-2, f14, FMA((REAL)4, f[0], f58));
+    /* MRT forward, Lallemand-Luo order (rho, e, eps, jx, qx, jy, qy, pxx, pxy), lbm_core4.cu:152-165;
+       e = -2 + m1d, eps = 1 + m2d */
+    REAL m1d = FMA((REAL)2, h58, FMA((REAL)-4, h[0], -h14));
+    REAL m2d = FMA((REAL)-2, h14, FMA((REAL)4, h[0], h58));
     REAL m3 = rc * ux;
     REAL m4 = FMA((REAL)-2, a13, sx);
     REAL m5 = rc * uy;
     REAL m6 = FMA((REAL)-2, a24, sy);
     REAL m7 = s13 - s24;
     REAL m8 = s57 - s68;
-    /* equilibria, lbm_core4.cu:166-170 */
+    /* non-equilibrium parts, lbm_core4.cu:166-170 */
     REAL ux2 = ux * ux, uy2 = uy * uy;
     REAL usq = ux2 + uy2;
-    REAL m1eq = FMA((REAL)3, usq, (REAL)-2) * rc;
-    REAL m2eq = FMA((REAL)-3, usq, (REAL)1) * rc;
+    REAL u3 = (REAL)3 * usq;
+    REAL n1 = FMA(-u3, rc, FMA((REAL)2, drc, m1d)); /* e - (3u^2 - 2) rho */
+    REAL n2 = FMA(u3, rc, m2d - drc);              /* eps - (1 - 3u^2) rho */
     REAL m7eq = rc * (ux2 - uy2);
     REAL m8eq = m3 * uy;
     REAL n7 = m7 - m7eq;
@@ -140,22 +152,22 @@ static void FN(collide_mrt)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin
     if (les) {
         /* Smagorinsky, lbm_core4.cu:172-177 */
         REAL Q = SQRT(FMA(n7, n7, FMA(n8, n8, (REAL)1e-8)));
-        REAL inv_rc = (REAL)1 / rc; /* equals 1/rho unless the density clamp engaged */
+        REAL inv_rc = (REAL)1 / rc;
         REAL d = FMA(P->c18cs2, Q * inv_rc, P->tau0sq);
         REAL tau_eff = (REAL)0.5 * (P->tau0 + SQRT(d));
         w = (REAL)1 / tau_eff;
     }
     /* relaxation, lbm_core4.cu:180-185 (m4, m6 relax towards -m3, -m5) */
-    m1 = FMA(-P->s_e, m1 - m1eq, m1);
-    m2 = FMA(-P->s_eps, m2 - m2eq, m2);
+    m1d = FMA(-P->s_e, n1, m1d);
+    m2d = FMA(-P->s_eps, n2, m2d);
     m4 = FMA(-P->s_q, m4 + m3, m4);
     m6 = FMA(-P->s_q, m6 + m5, m6);
     m7 = FMA(-w, n7, m7);
     m8 = FMA(-w, n8, m8);
-    /* inverse, lbm_core4.cu:187-199 */
-    REAL t0 = (m0 - m1) + m2;
-    REAL t1 = FMA((REAL)-2, m2, FMA((REAL)4, m0, -m1));
-    REAL t2 = FMA((REAL)4, m0, FMA((REAL)2, m1, m2));
+    /* inverse, lbm_core4.cu:187-199, of the deviation moments (rho - 1, m1d, m2d, m3, ..., m8) */
+    REAL t0 = (drc - m1d) + m2d;
+    REAL t1 = FMA((REAL)-2, m2d, FMA((REAL)4, drc, -m1d));
+    REAL t2 = FMA((REAL)4, drc, FMA((REAL)2, m1d, m2d));
     REAL A = (m3 - m4) * (REAL)6;
     REAL B = (m5 - m6) * (REAL)6;
     REAL Pp = m7 * (REAL)9;
@@ -165,36 +177,30 @@ static void FN(collide_mrt)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin
     REAL t1p = t1 + Pp, t1m = t1 - Pp;
     REAL u = t2 + Z, v = t2 - Z;
     REAL XpY = X + Y, XmY = X - Y;
-    f[0] = t0 * C_INV9;
-    f[1] = (t1p + A) * C_INV36;
-    f[2] = (t1m + B) * C_INV36;
-    f[3] = (t1p - A) * C_INV36;
-    f[4] = (t1m - B) * C_INV36;
-    f[5] = (u + XpY) * C_INV36;
-    f[6] = (v - XmY) * C_INV36;
-    f[7] = (u - XpY) * C_INV36;
-    f[8] = (v + XmY) * C_INV36;
-    for (int q = 0; q < 9; ++q) f[q] = FN(clampr)(f[q], (REAL)0, (REAL)10);
+    h[0] = t0 * C_INV9;
+    h[1] = (t1p + A) * C_INV36;
+    h[2] = (t1m + B) * C_INV36;
+    h[3] = (t1p - A) * C_INV36;
+    h[4] = (t1m - B) * C_INV36;
+    h[5] = (u + XpY) * C_INV36;
+    h[6] = (v - XmY) * C_INV36;
+    h[7] = (u - XpY) * C_INV36;
+    h[8] = (v + XmY) * C_INV36;
 }
 
-static void FN(collide_bgk)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin, REAL *mac)
+/* BGK of lbm_core_smemaa.cu:194-212 on deviations: heq_q = w_q (drho + rho (3cu + 4.5cu^2 - 1.5u^2)) */
+static void FN(collide_bgk)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin, REAL *mac)
 {
-    REAL rho, ux, uy;
+    REAL rho, drho, ux, uy;
     if (inlet) {
-        REAL num = FMA((REAL)2, (f[3] + f[6]) + f[7], (f[0] + f[2]) + f[4]);
-        rho = num / ((REAL)1 - uin);
-        REAL ru = rho * uin;
-        REAL h = (REAL)0.5 * (f[2] - f[4]);
-        f[1] = FMA(C_2_3, ru, f[3]);
-        f[5] = FMA(C_1_6, ru, f[7] - h);
-        f[8] = FMA(C_1_6, ru, f[6] + h);
+        drho = FN(zou_he)(h, uin, &rho);
         ux = uin;
         uy = (REAL)0;
     } else {
-        /* plain moments without clamps, lbm_core_smemaa.cu:194-201 */
-        REAL s13 = f[1] + f[3], s24 = f[2] + f[4], s57 = f[5] + f[7], s68 = f[6] + f[8];
-        REAL a13 = f[1] - f[3], a24 = f[2] - f[4], a57 = f[5] - f[7], a86 = f[8] - f[6];
-        rho = (f[0] + (s13 + s24)) + (s57 + s68);
+        REAL s13 = h[1] + h[3], s24 = h[2] + h[4], s57 = h[5] + h[7], s68 = h[6] + h[8];
+        REAL a13 = h[1] - h[3], a24 = h[2] - h[4], a57 = h[5] - h[7], a86 = h[8] - h[6];
+        drho = (h[0] + (s13 + s24)) + (s57 + s68);
+        rho = (REAL)1 + drho;
         REAL inv_rho = (REAL)1 / rho;
         ux = (a13 + (a57 + a86)) * inv_rho;
         uy = (a24 + (a57 - a86)) * inv_rho;
@@ -204,10 +210,7 @@ static void FN(collide_bgk)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin
         mac[1] = ux;
         mac[2] = uy;
     }
-    /* feq = w rho (1 + 3cu + 4.5cu^2 - 1.5u^2); f += tau_inv (feq - f)   (lbm_core_smemaa.cu:206-212) */
-    REAL usq = FMA(uy, uy, ux * ux);
-    REAL base = FMA((REAL)-1.5, usq, (REAL)1);
-    REAL wr0 = C_W0 * rho, wr1 = C_W1 * rho, wr5 = C_W5 * rho;
+    REAL usq15 = (REAL)1.5 * FMA(uy, uy, ux * ux);
     REAL cu[9];
     cu[0] = (REAL)0;
     cu[1] = ux;
@@ -219,20 +222,21 @@ static void FN(collide_bgk)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin
     cu[7] = -(ux + uy);
     cu[8] = ux - uy;
     for (int q = 0; q < 9; ++q) {
-        REAL wr = q == 0 ? wr0 : (q < 5 ? wr1 : wr5);
-        REAL poly = q == 0 ? base : FMA(cu[q], FMA((REAL)4.5, cu[q], (REAL)3), base);
-        REAL feq = wr * poly;
-        f[q] = FMA(P->tau_inv, feq - f[q], f[q]);
-        f[q] = FN(clampr)(f[q], (REAL)0, (REAL)10);
+        REAL poly = q == 0 ? -usq15 : FMA(cu[q], FMA((REAL)4.5, cu[q], (REAL)3), -usq15);
+        REAL heq = FN(wq)(q) * FMA(rho, poly, drho);
+        h[q] = FMA(P->tau_inv, heq - h[q], h[q]);
     }
 }
 
+/* Collision of FP32/FP64-stored populations: h = f - w, collide, f' = clamp(w + h', 0, 10). */
 static inline void FN(collide)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin, REAL *mac)
 {
+    for (int q = 0; q < 9; ++q) f[q] = f[q] - FN(wq)(q);
     if (P->model == ORACLE_BGK)
         FN(collide_bgk)(P, f, inlet, uin, mac);
     else
         FN(collide_mrt)(P, f, inlet, uin, mac);
+    for (int q = 0; q < 9; ++q) f[q] = FN(clampr)(FN(wq)(q) + f[q], (REAL)0, (REAL)10);
 }
 
 /* ---------------------------------------------------------------------------------------------------- */
