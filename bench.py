@@ -88,20 +88,29 @@ def main():
         f"{glups:.2f} GLUPS, {achieved:.0f} GB/s")
     forms = {"aa_fp32": {"glups": glups, "us_per_step": kern_ms * 1e3, "achieved_gbs": achieved}}
 
-    # ---------------- fused form (k steps per launch), if built
-    if L.lbm_fused_max_k() >= args.k >= 2:
-        nlaunch = max(1, args.steps // args.k)
-        sim.step(args.k * 2, k=args.k)
+    # ---------------- fused form: autotuned (k, strip, chunk), then timed like the plain form
+    if L.lbm_fused_max_k() >= 2 and args.k >= 2:
+        from lbm import autotune
+        ks = tuple(k for k in autotune.KS if k <= args.k)
+        best, table = autotune.tune_fused(sim, ks=ks, use_cache=False)
+        for k, strip, chunk, us in table:
+            log(f"tuned k={k}: strip={strip} chunk={chunk} {us:.2f} us/step")
+        k, strip, chunk, _ = best
+        nlaunch = max(1, args.steps // k)
+        sim.step(2 * k, k=k, cfg=(strip, chunk))
         torch.cuda.synchronize()
         e0.record()
-        sim.step(nlaunch * args.k, k=args.k)
+        sim.step(nlaunch * k, k=k, cfg=(strip, chunk))
         e1.record()
         torch.cuda.synchronize()
-        fk_ms = e0.elapsed_time(e1) / (nlaunch * args.k)
+        fk_ms = e0.elapsed_time(e1) / (nlaunch * k)
         fg = cells / (fk_ms * 1e-3) / 1e9
-        forms[f"fused_fp32_k{args.k}"] = {"glups": fg, "us_per_step": fk_ms * 1e3,
-                                          "x_plain": fg / glups}
-        log(f"fused fp32 k={args.k}: {fk_ms * 1e3:.1f} us/step, {fg:.2f} GLUPS ({fg / glups:.2f}x plain)")
+        forms[f"fused_fp32_k{k}"] = {"glups": fg, "us_per_step": fk_ms * 1e3, "x_plain": fg / glups,
+                                     "strip": strip, "chunk": chunk,
+                                     "tuned": {f"k{r[0]}": {"strip": r[1], "chunk": r[2], "us_per_step": r[3]}
+                                               for r in table}}
+        log(f"fused fp32 k={k} strip={strip} chunk={chunk}: {fk_ms * 1e3:.1f} us/step, {fg:.2f} GLUPS "
+            f"({fg / glups:.2f}x plain)")
 
     # ---------------- L2 error of the FP32 GPU run against the FP64 oracle (same initial state)
     l2 = None

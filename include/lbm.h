@@ -117,6 +117,22 @@ int lbm_step_fused(const lbm_params *p, const uint8_t *mask, const float *f_now,
 /* Largest k accepted by lbm_step_fused. */
 int lbm_fused_max_k(void);
 
+/* Launch configuration of the k-step kernel (no reference counterpart: the reference fixes its tile and
+ * T at compile time, lbm_core_smemaa.cu:8-9).  strip = LDS columns per workgroup including the halo
+ * (64 or 128); chunk = rows per workgroup. */
+typedef struct lbm_fused_cfg {
+    int32_t strip;
+    int32_t chunk;
+} lbm_fused_cfg;
+
+/* The configuration lbm_step_fused uses when none is given. */
+int lbm_fused_default_cfg(const lbm_params *p, int k, lbm_fused_cfg *cfg);
+
+/* lbm_step_fused with an explicit configuration (NULL = default); used by the autotuner. */
+int lbm_step_fused_cfg(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out,
+                       const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
+                       float *rho, lbm_stream_t stream);
+
 /*
  * rho, ux, uy of a stored state (macro_kernel of lbm_core_smemaa_endfused.cu:377-379, extended to both
  * parities): canonical states get Zou-He at x=0; reversed states are read through opp().  Only updated

@@ -20,12 +20,12 @@ namespace lbm {
 
 namespace fused {
 
-constexpr int LW = 128; // strip width in LDS, including the 2k halo columns
 constexpr int MAXK = 8;
+constexpr int PF = 4; // rows of HBM loads kept in flight ahead of the sweep (registers)
 
 enum : uint8_t { F_SOLID = 1, F_COMPUTE = 2, F_INLET = 4, F_OUTADJ = 8 };
 
-template <int K, int P0> struct Sched {
+template <int K, int P0, int LW> struct Sched {
     static constexpr int D0 = 1 + P0;                   // lag of step 0 behind the load
     static constexpr int LASTB = (P0 + K - 1) & 1;      // last step is an odd (B) step
     static constexpr int DLAST = D0 + 2 * (K - 1);
@@ -42,23 +42,27 @@ template <int K, int P0> struct Sched {
     static constexpr int HALO = 2 * nodd_before(K);
     static constexpr int CW = LW - 2 * HALO; // core columns written per strip
 };
-static_assert(Sched<8, 0>::HALO == 8 && Sched<8, 1>::HALO == 8 && Sched<1, 0>::HALO == 0 && Sched<1, 1>::HALO == 2,
+static_assert(Sched<8, 0, 128>::HALO == 8 && Sched<8, 1, 128>::HALO == 8 && Sched<1, 0, 128>::HALO == 0 &&
+                  Sched<1, 1, 128>::HALO == 2,
               "halo");
-static_assert(Sched<8, 0>::margin(7) == 7 && Sched<8, 1>::margin(7) == 8 && Sched<8, 1>::margin(0) == 1, "margin");
-static_assert(Sched<8, 0>::R == 18 && Sched<8, 1>::R == 18, "ring");
+static_assert(Sched<8, 0, 128>::margin(7) == 7 && Sched<8, 1, 128>::margin(7) == 8 &&
+                  Sched<8, 1, 128>::margin(0) == 1,
+              "margin");
+static_assert(Sched<8, 0, 128>::R == 18 && Sched<8, 1, 128>::R == 18, "ring");
 
 } // namespace fused
 
 using namespace fused;
 
-template <int MODEL, bool PERIODIC, int K, int P0, bool MACROS>
-__global__ __launch_bounds__((Sched<K, P0>::NT)) void k_fused(KPar P, const uint8_t *__restrict__ mask,
-                                                              const float *__restrict__ fin, float *__restrict__ fout,
-                                                              const float *__restrict__ inlet_u, int CH,
-                                                              float *__restrict__ oux, float *__restrict__ ouy,
-                                                              float *__restrict__ orho)
+template <int MODEL, bool PERIODIC, int K, int P0, int LW>
+__global__ __launch_bounds__((Sched<K, P0, LW>::NT)) void k_fused(KPar P, const uint8_t *__restrict__ mask,
+                                                                  const float *__restrict__ fin,
+                                                                  float *__restrict__ fout,
+                                                                  const float *__restrict__ inlet_u, int CH, int macros,
+                                                                  float *__restrict__ oux, float *__restrict__ ouy,
+                                                                  float *__restrict__ orho)
 {
-    using S = Sched<K, P0>;
+    using S = Sched<K, P0, LW>;
     constexpr int R = S::R;
     constexpr int NT = S::NT;
     constexpr int CW = S::CW;
@@ -68,24 +72,26 @@ __global__ __launch_bounds__((Sched<K, P0>::NT)) void k_fused(KPar P, const uint
 
     const int W = P.W, H = P.H, plane = P.plane;
     const int t = threadIdx.x;
-    const int xs = (int)blockIdx.x * CW - HALO; // global column of strip column 0
-    const int y0 = (int)blockIdx.y * CH;     // first output row
+    int bx, by;
+    xcd_tile(bx, by);
+    const int xs = bx * CW - HALO; // global column of strip column 0
+    const int y0 = by * CH;        // first output row
     const int ch = min(CH, H - y0);
     const int ybeg = y0 - HALO;              // first row needed at level 0
     const int yend = y0 + ch + HALO - 1;     // last row needed at level 0
     const int niter = ch + HALO + S::DOUT;
 
-    // ---- prefetch registers for the next row (level 0) -------------------------------------------
-    float pre[S::NLOAD];
-    uint8_t preflag = 0;
-    auto issue_load = [&](int j) { // global row ybeg + j
+    // ---- level-0 rows travel HBM -> registers (PF rows ahead) -> LDS ring ------------------------
+    float pre[PF][S::NLOAD];
+    uint8_t pfl[PF];
+    auto issue_load = [&](int j, float(&buf)[S::NLOAD], uint8_t &preflag) { // global row ybeg + j
         const int y = ybeg + j;
         bool rowin = PERIODIC ? true : (y >= 0 && y < H);
         const int gy = PERIODIC ? ((y % H) + H) % H : y;
 #pragma unroll
         for (int n = 0; n < S::NLOAD; ++n) {
             const int e = t + n * NT;
-            pre[n] = 0.0f;
+            buf[n] = 0.0f;
             if (e < 9 * LW && rowin && j <= yend - ybeg) {
                 const int q = e / LW, c = e - q * LW;
                 int x = xs + c;
@@ -95,7 +101,7 @@ __global__ __launch_bounds__((Sched<K, P0>::NT)) void k_fused(KPar P, const uint
                 } else {
                     colin = x >= 0 && x < W;
                 }
-                if (colin) pre[n] = fin[q * plane + gy * W + x];
+                if (colin) buf[n] = fin[q * plane + gy * W + x];
             }
         }
         if (t < LW) {
@@ -117,24 +123,34 @@ __global__ __launch_bounds__((Sched<K, P0>::NT)) void k_fused(KPar P, const uint
             preflag = fl;
         }
     };
-    auto commit_load = [&](int j) {
+    auto commit_load = [&](int j, const float(&buf)[S::NLOAD], uint8_t preflag) {
         const int slot = j % R;
 #pragma unroll
         for (int n = 0; n < S::NLOAD; ++n) {
             const int e = t + n * NT;
             if (e < 9 * LW) {
                 const int q = e / LW, c = e - q * LW;
-                sf[q][slot][c] = pre[n];
+                sf[q][slot][c] = buf[n];
             }
         }
         if (t < LW) sflag[slot][t] = preflag;
     };
 
-    issue_load(0);
-    for (int i = 0; i < niter; ++i) {
-        // (1) row i enters the ring; (2) prefetch row i + 1
-        if (i <= yend - ybeg) commit_load(i);
-        if (i + 1 <= yend - ybeg) issue_load(i + 1);
+    const int nload = yend - ybeg + 1;
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+        if (u < nload) issue_load(u, pre[u], pfl[u]);
+    // the u loop is unrolled so that every prefetch buffer is addressed statically (row i lives in
+    // pre[i % PF]); the barrier at the end of each iteration waits for LDS only, so up to PF rows of HBM
+    // loads stay in flight across iterations
+    for (int i0 = 0; i0 < niter; i0 += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int i = i0 + u;
+        if (i >= niter) break;
+        // (1) row i enters the ring; (2) row i + PF is requested from HBM
+        if (i < nload) commit_load(i, pre[u], pfl[u]);
+        if (i + PF < nload) issue_load(i + PF, pre[u], pfl[u]);
 
         // (3) one cell of one step per thread
         const int s = t / LW, c = t - (t / LW) * LW;
@@ -197,7 +213,7 @@ __global__ __launch_bounds__((Sched<K, P0>::NT)) void k_fused(KPar P, const uint
                             if (!solid[4]) sf[7][sd][c] = g[7];
                         }
                     }
-                    if (MACROS && s == K - 1) {
+                    if (macros && s == K - 1) {
                         const int x = xs + c;
                         if (c >= HALO && c < LW - HALO && x < W && y >= y0 && y < y0 + ch) {
                             const int idx = y * W + x;
@@ -223,66 +239,70 @@ __global__ __launch_bounds__((Sched<K, P0>::NT)) void k_fused(KPar P, const uint
         }
         __syncthreads();
     }
+    }
 }
 
-template <int MODEL, bool PERIODIC, int K, int P0, bool MACROS>
+template <int MODEL, bool PERIODIC, int K, int P0, int LW>
 static void launch_fused_t(const KPar &P, hipStream_t st, const uint8_t *mask, const float *fin, float *fout,
                            const float *inlet_u, int CH, float *ux, float *uy, float *rho)
 {
-    using S = Sched<K, P0>;
-    const int CW = S::CW;
-    dim3 grid((P.W + CW - 1) / CW, (P.H + CH - 1) / CH);
-    hipLaunchKernelGGL((k_fused<MODEL, PERIODIC, K, P0, MACROS>), grid, dim3(S::NT), 0, st, P, mask, fin, fout, inlet_u,
-                       CH, ux, uy, rho);
+    using S = Sched<K, P0, LW>;
+    dim3 grid((P.W + S::CW - 1) / S::CW, (P.H + CH - 1) / CH);
+    hipLaunchKernelGGL((k_fused<MODEL, PERIODIC, K, P0, LW>), grid, dim3(S::NT), 0, st, P, mask, fin, fout, inlet_u, CH,
+                       rho != nullptr, ux, uy, rho);
 }
 
-template <int MODEL, bool PERIODIC, int K, int P0>
-static void launch_fused_m(const KPar &P, hipStream_t st, const uint8_t *mask, const float *fin, float *fout,
-                           const float *inlet_u, int CH, float *ux, float *uy, float *rho)
-{
-    if (rho)
-        launch_fused_t<MODEL, PERIODIC, K, P0, true>(P, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
-    else
-        launch_fused_t<MODEL, PERIODIC, K, P0, false>(P, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
-}
-
-template <int MODEL, bool PERIODIC, int K>
+template <int MODEL, bool PERIODIC, int K, int LW>
 static void launch_fused_p(const KPar &P, int parity, hipStream_t st, const uint8_t *mask, const float *fin, float *fout,
                            const float *inlet_u, int CH, float *ux, float *uy, float *rho)
 {
     if (parity)
-        launch_fused_m<MODEL, PERIODIC, K, 1>(P, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
+        launch_fused_t<MODEL, PERIODIC, K, 1, LW>(P, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
     else
-        launch_fused_m<MODEL, PERIODIC, K, 0>(P, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
+        launch_fused_t<MODEL, PERIODIC, K, 0, LW>(P, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
+}
+
+template <int MODEL, bool PERIODIC, int K>
+static void launch_fused_w(const KPar &P, int lw, int parity, hipStream_t st, const uint8_t *mask, const float *fin,
+                           float *fout, const float *inlet_u, int CH, float *ux, float *uy, float *rho)
+{
+    if (lw == 64)
+        launch_fused_p<MODEL, PERIODIC, K, 64>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
+    else
+        launch_fused_p<MODEL, PERIODIC, K, 128>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
 }
 
 template <int MODEL, bool PERIODIC>
-static int launch_fused_k(const KPar &P, int k, int parity, hipStream_t st, const uint8_t *mask, const float *fin,
-                          float *fout, const float *inlet_u, int CH, float *ux, float *uy, float *rho)
+static int launch_fused_k(const KPar &P, int k, int lw, int parity, hipStream_t st, const uint8_t *mask,
+                          const float *fin, float *fout, const float *inlet_u, int CH, float *ux, float *uy, float *rho)
 {
+#define LBM_FUSED_CASE(KK)                                                                                              \
+    case KK: launch_fused_w<MODEL, PERIODIC, KK>(P, lw, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
     switch (k) {
-    case 1: launch_fused_p<MODEL, PERIODIC, 1>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
-    case 2: launch_fused_p<MODEL, PERIODIC, 2>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
-    case 3: launch_fused_p<MODEL, PERIODIC, 3>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
-    case 4: launch_fused_p<MODEL, PERIODIC, 4>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
-    case 5: launch_fused_p<MODEL, PERIODIC, 5>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
-    case 6: launch_fused_p<MODEL, PERIODIC, 6>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
-    case 7: launch_fused_p<MODEL, PERIODIC, 7>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
-    case 8: launch_fused_p<MODEL, PERIODIC, 8>(P, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho); break;
+        LBM_FUSED_CASE(1)
+        LBM_FUSED_CASE(2)
+        LBM_FUSED_CASE(3)
+        LBM_FUSED_CASE(4)
+        LBM_FUSED_CASE(5)
+        LBM_FUSED_CASE(6)
+        LBM_FUSED_CASE(7)
+        LBM_FUSED_CASE(8)
     default: return fail(LBM_ERR_ARG, "lbm_step_fused: k=%d outside [1, %d]", k, MAXK);
     }
+#undef LBM_FUSED_CASE
     return LBM_OK;
 }
 
-// Rows per workgroup chunk: enough chunks that the grid fills the chip (>= 2 workgroups per CU on 256
-// CUs) while keeping the k-row warm-up at each chunk end small.
-static int pick_chunk(int W, int H, int k)
+// Default launch configuration: 128-column strips, and the tallest chunk that still gives at least 512
+// workgroups (two per CU on 256 CUs) so the k-row warm-up at each chunk end stays small.
+static void default_cfg(int W, int H, int k, lbm_fused_cfg *c)
 {
-    const int CW = LW - 2 * (k + 1);
+    c->strip = 128;
+    const int CW = 128 - 2 * (k + 1);
     const int strips = (W + CW - 1) / CW;
     int ch = 256;
     while (ch > 16 && (long long)strips * ((H + ch - 1) / ch) < 512) ch /= 2;
-    return ch < H ? ch : H;
+    c->chunk = ch < H ? ch : H;
 }
 
 } // namespace lbm
@@ -293,14 +313,21 @@ extern "C" {
 
 int lbm_fused_max_k(void) { return fused::MAXK; }
 
-int lbm_step_fused(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out, const float *inlet_u,
-                   int parity, int k, float *ux, float *uy, float *rho, lbm_stream_t stream)
+int lbm_fused_default_cfg(const lbm_params *p, int k, lbm_fused_cfg *cfg)
+{
+    if (!p || !cfg) return fail(LBM_ERR_ARG, "lbm_fused_default_cfg: NULL pointer");
+    default_cfg(p->W, p->H, k, cfg);
+    return LBM_OK;
+}
+
+int lbm_step_fused_cfg(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out,
+                       const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
+                       float *rho, lbm_stream_t stream)
 {
     KPar P;
     int rc = make_kpar(p, &P);
     if (rc) return rc;
     if (!mask || !f_now || !f_out) return fail(LBM_ERR_ARG, "lbm_step_fused: mask, f_now, f_out must be non-NULL");
-    if (f_now == f_out) return fail(LBM_ERR_ARG, "lbm_step_fused: f_now and f_out must differ");
     {
         const char *a = reinterpret_cast<const char *>(f_now), *b = reinterpret_cast<const char *>(f_out);
         const size_t n = sizeof(float) * 9 * (size_t)P.plane;
@@ -311,17 +338,35 @@ int lbm_step_fused(const lbm_params *p, const uint8_t *mask, const float *f_now,
     if ((ux == nullptr) != (rho == nullptr) || (uy == nullptr) != (rho == nullptr))
         return fail(LBM_ERR_ARG, "lbm_step_fused: ux, uy, rho must be all NULL or all non-NULL");
     if (k < 1 || k > fused::MAXK) return fail(LBM_ERR_ARG, "lbm_step_fused: k=%d outside [1, %d]", k, fused::MAXK);
-    const int CH = pick_chunk(P.W, P.H, k);
+    lbm_fused_cfg c;
+    if (cfg)
+        c = *cfg;
+    else
+        default_cfg(P.W, P.H, k, &c);
+    if (c.strip != 64 && c.strip != 128) return fail(LBM_ERR_ARG, "lbm_step_fused: strip must be 64 or 128");
+    if (c.chunk < 1) return fail(LBM_ERR_ARG, "lbm_step_fused: chunk must be >= 1");
+    if (c.strip - 2 * (k + 1) < 16) return fail(LBM_ERR_ARG, "lbm_step_fused: strip %d too narrow for k=%d", c.strip, k);
+    if (c.chunk > P.H) c.chunk = P.H;
     hipStream_t st = to_hip(stream);
     const bool per = p->bc == LBM_BC_PERIODIC;
     if (p->model == LBM_BGK)
-        rc = per ? launch_fused_k<MODEL_BGK, true>(P, k, parity, st, mask, f_now, f_out, inlet_u, CH, ux, uy, rho)
-                 : launch_fused_k<MODEL_BGK, false>(P, k, parity, st, mask, f_now, f_out, inlet_u, CH, ux, uy, rho);
+        rc = per ? launch_fused_k<MODEL_BGK, true>(P, k, c.strip, parity, st, mask, f_now, f_out, inlet_u, c.chunk, ux,
+                                                   uy, rho)
+                 : launch_fused_k<MODEL_BGK, false>(P, k, c.strip, parity, st, mask, f_now, f_out, inlet_u, c.chunk, ux,
+                                                    uy, rho);
     else
-        rc = per ? launch_fused_k<MODEL_MRT, true>(P, k, parity, st, mask, f_now, f_out, inlet_u, CH, ux, uy, rho)
-                 : launch_fused_k<MODEL_MRT, false>(P, k, parity, st, mask, f_now, f_out, inlet_u, CH, ux, uy, rho);
+        rc = per ? launch_fused_k<MODEL_MRT, true>(P, k, c.strip, parity, st, mask, f_now, f_out, inlet_u, c.chunk, ux,
+                                                   uy, rho)
+                 : launch_fused_k<MODEL_MRT, false>(P, k, c.strip, parity, st, mask, f_now, f_out, inlet_u, c.chunk, ux,
+                                                    uy, rho);
     if (rc) return rc;
     return check_launch("lbm_step_fused");
+}
+
+int lbm_step_fused(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out, const float *inlet_u,
+                   int parity, int k, float *ux, float *uy, float *rho, lbm_stream_t stream)
+{
+    return lbm_step_fused_cfg(p, mask, f_now, f_out, inlet_u, parity, k, nullptr, ux, uy, rho, stream);
 }
 
 } // extern "C"

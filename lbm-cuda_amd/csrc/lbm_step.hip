@@ -5,9 +5,10 @@
 //   lbm_visualize  vorticity colour map (lbm_core4.cu:214-255)
 //   lbm_diagnostics deterministic FP64 mass / momentum / momentum-exchange force
 //
-// Layout: SoA planes f[q*plane + y*W + x]; a 256-thread block covers 64 x 4 cells, so each wave64 reads
-// and writes one 256-byte row segment per plane (coalesced).  Odd (B) steps read and write the same 9
-// locations per cell (x - c_q / x + c_q pairs), which is what makes the single-buffer update race free.
+// Layout: SoA planes f[q*plane + y*W + x]; a 256-thread block covers 128 x 2 cells, so each wave64 reads
+// and writes one 256-byte row segment per plane (coalesced); blocks are renumbered per XCD (xcd_tile).
+// Odd (B) steps read and write the same 9 locations per cell (x - c_q / x + c_q pairs), which is what
+// makes the single-buffer update race free.
 #include "lbm_common.h"
 
 #include <cstring>
@@ -88,8 +89,10 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__
                                                         const float *__restrict__ inlet_u, float *__restrict__ oux,
                                                         float *__restrict__ ouy, float *__restrict__ orho)
 {
-    const int x = blockIdx.x * TILE_X + threadIdx.x;
-    const int y = blockIdx.y * TILE_Y + threadIdx.y;
+    int bx, by;
+    xcd_tile(bx, by);
+    const int x = bx * TILE_X + threadIdx.x;
+    const int y = by * TILE_Y + threadIdx.y;
     if (x >= P.W || y >= P.H) return;
     const int W = P.W, plane = P.plane;
     const int idx = y * W + x;
@@ -201,8 +204,10 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_pull(KPar P, const uint8_t *
                                                           const float *__restrict__ inlet_u, float *__restrict__ oux,
                                                           float *__restrict__ ouy, float *__restrict__ orho)
 {
-    const int x = blockIdx.x * TILE_X + threadIdx.x;
-    const int y = blockIdx.y * TILE_Y + threadIdx.y;
+    int bx, by;
+    xcd_tile(bx, by);
+    const int x = bx * TILE_X + threadIdx.x;
+    const int y = by * TILE_Y + threadIdx.y;
     if (x >= P.W || y >= P.H) return;
     const int W = P.W, plane = P.plane;
     const int idx = y * W + x;

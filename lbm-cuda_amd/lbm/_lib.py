@@ -34,6 +34,12 @@ class LbmParams(ctypes.Structure):
     ]
 
 
+class LbmFusedCfg(ctypes.Structure):
+    """Mirror of ``lbm_fused_cfg`` (include/lbm.h)."""
+
+    _fields_ = [("strip", ctypes.c_int32), ("chunk", ctypes.c_int32)]
+
+
 class LbmError(RuntimeError):
     pass
 
@@ -53,6 +59,8 @@ SIGNATURES = {
     "lbm_step_pull": (_i, [_P, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lbm_step_fused": (_i, [_P, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp]),
     "lbm_fused_max_k": (_i, []),
+    "lbm_fused_default_cfg": (_i, [_P, _i, _vp]),
+    "lbm_step_fused_cfg": (_i, [_P, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "lbm_macro": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "lbm_visualize": (_i, [_P, _vp, _vp, _vp, _vp, _f, _vp]),
     "lbm_diagnostics_work_size": (ctypes.c_size_t, [_P]),

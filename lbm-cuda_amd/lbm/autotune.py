@@ -1,0 +1,120 @@
+"""Autotuner for the temporally blocked kernel: picks the strip width, the chunk height and k for a lattice.
+
+The reference fixes its tile (SQUARE_A = 24) and T at compile time and mirrors them by hand in the driver
+(lbm_smemaa_main.py:93-95) -- its compile-time traversal-order table (lbm_core_smemaa_policy.cu:12-24) is
+the only tuning knob.  Here every candidate is timed on the GPU with HIP events on scratch copies of the
+state (the simulation itself is not advanced) and the winner is cached per lattice size class.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+import torch
+
+from . import _lib
+
+STRIPS = (64, 128)
+CHUNKS = (16, 32, 64, 128, 256)
+KS = (2, 4, 6, 8)
+
+_CACHE_ENV = "LBM_TUNE_CACHE"
+
+
+def _cache_path():
+    return os.environ.get(_CACHE_ENV, os.path.join(os.path.expanduser("~"), ".cache", "lbm-mi355x", "fused.json"))
+
+
+def _key(sim, k):
+    p = sim.params
+    dev = torch.cuda.get_device_name(sim.device)
+    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|k{k}"
+
+
+def load_cache():
+    try:
+        with open(_cache_path()) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(cache):
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path + ".tmp", "w") as fh:
+            json.dump(cache, fh, indent=1, sort_keys=True)
+        os.replace(path + ".tmp", path)
+    except OSError:
+        pass  # a read-only home is not an error: tuning still returns its result
+
+
+def time_config(sim, k, cfg, reps=6, bufs=None):
+    """Mean microseconds per collide-stream step of lbm_step_fused_cfg with `cfg` (strip, chunk)."""
+    L = sim.lib
+    if bufs is None:
+        bufs = (sim.f.clone(), torch.empty_like(sim.f))
+    a, b = bufs
+    c = _lib.LbmFusedCfg(int(cfg[0]), int(cfg[1]))
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    par = sim.parity
+
+    def launch(src, dst, parity):
+        rc = L.lbm_step_fused_cfg(ctypes.byref(sim.params), ctypes.c_void_p(sim.mask.data_ptr()),
+                                  ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                  ctypes.c_void_p(sim.inlet_u.data_ptr()), parity, k, ctypes.byref(c), None, None,
+                                  None, st)
+        _lib.check(rc, "lbm_step_fused_cfg")
+
+    launch(a, b, par)  # warm-up
+    launch(b, a, (par + k) & 1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for r in range(reps):
+        src, dst = (a, b) if r % 2 == 0 else (b, a)
+        launch(src, dst, (par + r * k) & 1)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / (reps * k)
+
+
+def candidates(sim, k):
+    H = sim.params.H
+    out = []
+    for strip in STRIPS:
+        if strip - 2 * (k + 1) < 16:
+            continue
+        for ch in CHUNKS:
+            if ch > 2 * H:
+                continue
+            out.append((strip, min(ch, H)))
+    return sorted(set(out))
+
+
+def tune_fused(sim, ks=KS, use_cache=True, reps=6, verbose=False):
+    """Best (k, strip, chunk, us_per_step) over ks for `sim`'s lattice; returns (best, table)."""
+    cache = load_cache() if use_cache else {}
+    bufs = (sim.f.clone(), torch.empty_like(sim.f))
+    table = []
+    for k in ks:
+        key = _key(sim, k)
+        if key in cache:
+            strip, chunk, us = cache[key]
+            table.append((k, strip, chunk, us))
+            continue
+        best = None
+        for cfg in candidates(sim, k):
+            us = time_config(sim, k, cfg, reps=reps, bufs=bufs)
+            if verbose:
+                print(f"  k={k} strip={cfg[0]} chunk={cfg[1]}: {us:.2f} us/step", flush=True)
+            if best is None or us < best[2]:
+                best = (cfg[0], cfg[1], us)
+        cache[key] = list(best)
+        table.append((k,) + tuple(best))
+    if use_cache:
+        save_cache(cache)
+    best = min(table, key=lambda r: r[3])
+    del bufs
+    return best, table

@@ -82,8 +82,9 @@ class Simulation:
         self._diag_out = None
 
     # ------------------------------------------------------------------ stepping
-    def step(self, n=1, k=1, macros=False):
-        """Advance n collide-stream steps; k > 1 uses the temporally blocked kernel (k steps per launch)."""
+    def step(self, n=1, k=1, macros=False, cfg=None):
+        """Advance n collide-stream steps; k > 1 uses the temporally blocked kernel (k steps per launch).
+        ``cfg`` = (strip, chunk) selects its launch configuration (default: the library's choice)."""
         n = int(n)
         k = int(k)
         if k <= 1:
@@ -93,7 +94,7 @@ class Simulation:
         if n % k:
             raise ValueError(f"n={n} is not a multiple of k={k}")
         for _ in range(n // k):
-            self._fused(k, macros)
+            self._fused(k, macros, cfg)
         return self
 
     def _aa(self, macros=False):
@@ -104,13 +105,14 @@ class Simulation:
         self.parity ^= 1
         self.steps += 1
 
-    def _fused(self, k, macros=False):
+    def _fused(self, k, macros=False, cfg=None):
         if self.f_alt is None:
             self.f_alt = torch.empty_like(self.f)
         m = (self.ux, self.uy, self.rho) if macros else (None, None, None)
-        rc = self.lib.lbm_step_fused(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.f_alt),
-                                     _ptr(self.inlet_u), self.parity, k, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]),
-                                     _stream())
+        c = None if cfg is None else ctypes.byref(_lib.LbmFusedCfg(int(cfg[0]), int(cfg[1])))
+        rc = self.lib.lbm_step_fused_cfg(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.f_alt),
+                                         _ptr(self.inlet_u), self.parity, k, c, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]),
+                                         _stream())
         _lib.check(rc, "lbm_step_fused")
         self.f, self.f_alt = self.f_alt, self.f
         self.parity = (self.parity + k) & 1

@@ -10,7 +10,6 @@ import numpy as np
 
 from oracle import pyref
 
-LW = 128
 CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
 CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
 OPP = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6])
@@ -18,15 +17,15 @@ F_SOLID, F_COMPUTE, F_INLET, F_OUTADJ = 1, 2, 4, 8
 
 
 class Sched:
-    def __init__(self, K, P0):
-        self.K, self.P0 = K, P0
+    def __init__(self, K, P0, LW=128):
+        self.K, self.P0, self.LW = K, P0, LW
         self.D0 = 1 + P0
         self.LASTB = (P0 + K - 1) & 1
         self.DLAST = self.D0 + 2 * (K - 1)
         self.DOUT = self.DLAST + 1 + self.LASTB
         self.R = self.DOUT + 1
         self.HALO = 2 * self.nodd_before(K)
-        self.CW = LW - 2 * self.HALO
+        self.CW = self.LW - 2 * self.HALO
 
     def d(self, s):
         return self.D0 + 2 * s
@@ -39,7 +38,7 @@ class Sched:
 
 
 def pick_chunk(W, H, k):
-    CW = LW - 2 * (k + 1)
+    CW = 128 - 2 * (k + 1)
     strips = (W + CW - 1) // CW
     ch = 256
     while ch > 16 and strips * ((H + ch - 1) // ch) < 512:
@@ -94,11 +93,11 @@ def _bgk(f, rho, ux, uy, tau_inv):
     return np.minimum(10.0, np.maximum(0.0, out))
 
 
-def fused(f_in, mask, inlet_u, parity, K, p, periodic):
+def fused(f_in, mask, inlet_u, parity, K, p, periodic, strip=128, chunk=None):
     """k_fused on the whole grid; returns f_out (cells not written keep NaN)."""
     _, H, W = f_in.shape
-    S = Sched(K, parity)
-    CH = pick_chunk(W, H, K)
+    S = Sched(K, parity, strip)
+    CH = pick_chunk(W, H, K) if chunk is None else min(chunk, H)
     out = np.full_like(f_in, np.nan)
     gx = (W + S.CW - 1) // S.CW
     gy_n = (H + CH - 1) // CH
@@ -110,7 +109,7 @@ def fused(f_in, mask, inlet_u, parity, K, p, periodic):
 
 def _block(fin, fout, mask, inlet_u, S, CH, bx, by, p, periodic):
     _, H, W = fin.shape
-    R, HALO, CW, K, P0 = S.R, S.HALO, S.CW, S.K, S.P0
+    R, HALO, CW, K, P0, LW = S.R, S.HALO, S.CW, S.K, S.P0, S.LW
     sf = np.full((9, R, LW), np.nan)
     sflag = np.zeros((R, LW), np.uint8)
     xs = bx * CW - HALO

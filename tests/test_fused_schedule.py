@@ -29,8 +29,10 @@ def _case(W, H, periodic, par, op):
 
 @pytest.mark.parametrize("K", [1, 2, 3, 8])
 @pytest.mark.parametrize("par", [0, 1])
-@pytest.mark.parametrize("periodic,model", [(False, O.MRT), (True, O.MRT), (False, O.BGK)])
-def test_fused_schedule_matches_single_steps(oracle_lib, K, par, periodic, model):
+@pytest.mark.parametrize("periodic,model,strip,chunk", [(False, O.MRT, 128, None), (True, O.MRT, 128, None),
+                                                        (False, O.BGK, 128, None), (False, O.MRT, 64, 24),
+                                                        (True, O.MRT, 64, 7)])
+def test_fused_schedule_matches_single_steps(oracle_lib, K, par, periodic, model, strip, chunk):
     W, H = (150, 40) if not periodic else (140, 36)
     cs = 0.1 if model == O.MRT else 0.0
     op = O.params(W, H, 1.7, cs=cs, model=model, bc=O.PERIODIC if periodic else O.CHANNEL)
@@ -38,7 +40,7 @@ def test_fused_schedule_matches_single_steps(oracle_lib, K, par, periodic, model
     ref = f.copy()
     rpar = O.aa_steps(op, mask, ref, prof, par, K)
     p = dict(tau_inv=F32(1.7), cs=F32(cs), model=model, clamp=1, s_e=F32(1.1), s_eps=F32(1.1), s_q=F32(1.1))
-    out = E.fused(f.copy(), mask, prof, par, K, p, periodic)
+    out = E.fused(f.copy(), mask, prof, par, K, p, periodic, strip=strip, chunk=chunk)
     sel = O.observable(op, mask, rpar)
     assert not np.isnan(out[sel]).any()
     assert np.abs(out - ref)[sel].max() < 1e-13

@@ -54,6 +54,33 @@ def test_fused_equals_single_small(gpu, k, periodic, model):
             _assert_equal(single, fusedsim, sel, f"k={k} launch {launch} start parity {start_parity}")
 
 
+@pytest.mark.parametrize("k,cfg", [(3, (64, 24)), (8, (64, 7)), (8, (128, 33)), (2, (64, 256)), (5, (128, 16))])
+def test_fused_explicit_configs(gpu, k, cfg):
+    single, fusedsim = _pair(301, 90, False, "mrt")
+    sel = _sel(single)
+    for launch in range(2):
+        single.step(k)
+        fusedsim.step(k, k=k, cfg=cfg)
+        _assert_equal(single, fusedsim, sel, f"k={k} cfg={cfg} launch {launch}")
+
+
+def test_autotuner_picks_a_valid_config(gpu, tmp_path, monkeypatch):
+    from lbm import autotune
+    monkeypatch.setenv("LBM_TUNE_CACHE", str(tmp_path / "tune.json"))
+    single, fusedsim = _pair(512, 256, True, "mrt")
+    before = fusedsim.f.clone()
+    best, table = autotune.tune_fused(fusedsim, ks=(2, 4))
+    assert torch.equal(before, fusedsim.f)  # tuning works on scratch copies
+    k, strip, chunk, us = best
+    assert k in (2, 4) and strip in (64, 128) and us > 0
+    sel = _sel(single)
+    single.step(2 * k)
+    fusedsim.step(2 * k, k=k, cfg=(strip, chunk))
+    _assert_equal(single, fusedsim, sel, "tuned config")
+    best2, _ = autotune.tune_fused(fusedsim, ks=(2, 4))
+    assert best2 == best  # served from the cache
+
+
 def test_fused_macros_match_single(gpu):
     single, fusedsim = _pair(200, 64, False, "mrt")
     single.step(3)
