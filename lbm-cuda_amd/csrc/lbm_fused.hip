@@ -67,6 +67,7 @@ __global__ __launch_bounds__((Sched<K, P0, LW>::NT)) void k_fused(KPar P, const 
     constexpr int NT = S::NT;
     constexpr int CW = S::CW;
     constexpr int HALO = S::HALO;
+    constexpr bool FINAL_REVERSED = ((P0 + K) & 1) != 0;
     __shared__ float sf[9][R][LW];
     __shared__ uint8_t sflag[R][LW];
 
@@ -234,7 +235,20 @@ __global__ __launch_bounds__((Sched<K, P0, LW>::NT)) void k_fused(KPar P, const 
             for (int e = t; e < 9 * CW; e += NT) {
                 const int q = e / CW, cc = e - q * CW;
                 const int x = xs + HALO + cc;
-                if (x < W) fout[q * plane + y * W + x] = sf[q][sl][HALO + cc];
+                if (x >= W) continue;
+                // The outlet ghost W-1 is part of cell W-2's state (its post-collision copy after an even
+                // step), so the workgroup that owns W-2 writes it: the owner of W-1 may not have kept
+                // W-2 valid through the last step.  A canonical ghost is never read and only written if
+                // present.
+                if (!PERIODIC && x == W - 1) continue;
+                const float v = sf[q][sl][HALO + cc];
+                fout[q * plane + y * W + x] = v;
+                if (!PERIODIC && x == W - 2) {
+                    if (FINAL_REVERSED && y >= 1 && y <= H - 2)
+                        fout[q * plane + y * W + x + 1] = v;
+                    else if (HALO + cc + 1 < LW)
+                        fout[q * plane + y * W + x + 1] = sf[q][sl][HALO + cc + 1];
+                }
             }
         }
         __syncthreads();

@@ -183,10 +183,11 @@ def feq(rho, ux, uy):
     return out
 
 
-def observable(p, mask, parity):
+def observable(p, mask, parity, ghost=False):
     """Boolean [9, H, W] selecting the populations that carry state (stale slots excluded):
     updated cells only (no solids, wall rows or outlet ghost); in canonical form the inlet's q=1,5,8
-    (which arrive from outside and are rebuilt by Zou-He) are excluded."""
+    (which arrive from outside and are rebuilt by Zou-He) are excluded.  ghost=True also selects the
+    outlet ghost column (rows 1..H-2) of a reversed state, which the next odd step reads."""
     H, W = p.H, p.W
     m = mask.astype(bool)
     upd = ~m
@@ -199,6 +200,8 @@ def observable(p, mask, parity):
     if p.bc == CHANNEL and parity == 0:
         for q in (1, 5, 8):
             sel[q, :, 0] = False
+    if ghost and p.bc == CHANNEL and parity == 1:
+        sel[:, 1:H - 1, W - 1] = True
     return sel
 
 

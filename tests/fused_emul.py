@@ -228,5 +228,16 @@ def _block(fin, fout, mask, inlet_u, S, CH, bx, by, p, periodic):
             cc = np.arange(CW)
             x = xs + HALO + cc
             ok = x < W
+            if not periodic:
+                ok &= x != W - 1  # the ghost is written by the owner of W-2
             assert 0 <= y < H
             fout[:, y, x[ok]] = sf[:, sl, HALO + cc[ok]]
+            if not periodic:
+                own = np.nonzero(ok & (x == W - 2))[0]
+                if own.size:
+                    c2 = HALO + own[0]
+                    final_reversed = ((P0 + K) & 1) == 1
+                    if final_reversed and 1 <= y <= H - 2:
+                        fout[:, y, W - 1] = sf[:, sl, c2]
+                    elif c2 + 1 < LW:
+                        fout[:, y, W - 1] = sf[:, sl, c2 + 1]

@@ -44,3 +44,22 @@ def test_fused_schedule_matches_single_steps(oracle_lib, K, par, periodic, model
     sel = O.observable(op, mask, rpar)
     assert not np.isnan(out[sel]).any()
     assert np.abs(out - ref)[sel].max() < 1e-13
+
+
+@pytest.mark.parametrize("K,strip,chunk", [(3, 64, 24), (1, 128, 16), (2, 64, 9), (5, 128, 40)])
+def test_fused_chained_launches_keep_the_ghost(oracle_lib, K, strip, chunk):
+    """Consecutive launches (each reading the previous one's output) match single steps, including the
+    outlet ghost of reversed states; W = 301 puts strip core boundaries between W-2 and W-1."""
+    W, H = 301, 40
+    op = O.params(W, H, 1.7, cs=0.1)
+    mask, prof, f = _case(W, H, False, 0, op)
+    p = dict(tau_inv=F32(1.7), cs=F32(0.1), model=O.MRT, clamp=1, s_e=F32(1.1), s_eps=F32(1.1), s_q=F32(1.1))
+    ref = f.copy()
+    par = 0
+    for launch in range(3):
+        out = E.fused(f, mask, prof, par, K, p, False, strip=strip, chunk=chunk)
+        rpar = O.aa_steps(op, mask, ref, prof, par, K)
+        sel = O.observable(op, mask, rpar, ghost=True)
+        assert np.abs(out - ref)[sel].max() < 1e-13, f"launch {launch}"
+        f = np.where(np.isnan(out), 0.0, out)
+        par = rpar

@@ -28,7 +28,7 @@ def _pair(W, H, periodic, model, seed=3):
 def _sel(sim):
     p = sim.params
     op = O.params(p.W, p.H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc)
-    return [torch.from_numpy(O.observable(op, sim.mask_host, par)).to(sim.device) for par in (0, 1)]
+    return [torch.from_numpy(O.observable(op, sim.mask_host, par, ghost=True)).to(sim.device) for par in (0, 1)]
 
 
 def _assert_equal(a, b, sel, what):

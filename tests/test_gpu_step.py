@@ -53,7 +53,7 @@ def _check_steps_bitwise(gpu, sim, nsteps, macros=False):
         f_gpu, par_gpu = sim.state()
         ref = f_prev.copy()
         mac = O.aa_step(p, mask, ref, prof, par, macros=macros)
-        sel = O.observable(p, mask, par ^ 1)
+        sel = O.observable(p, mask, par ^ 1, ghost=True)
         assert par_gpu == par ^ 1
         bad = np.count_nonzero(f_gpu[sel] != ref[sel])
         assert bad == 0, f"step {n} parity {par}: {bad} populations differ, max {np.abs(f_gpu - ref)[sel].max()}"
