@@ -42,6 +42,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--l2-steps", type=int, default=20, help="steps for the FP32-vs-FP64 L2 error (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=("all", "aa", "fused"), default="all",
+                    help="run only the plain single step or only the fused form (for profiling)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy a device buffer of this many MiB (a known-size kernel for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine runs one lattice per GPU and has no multi-GPU path "
@@ -56,6 +60,14 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
     case = configs.cylinder_large()
     W, H = case.W, case.H
     cells = W * H
@@ -70,6 +82,8 @@ def main():
         sim.parity = (sim.parity + n) & 1
         sim.steps += n
 
+    if args.only == "fused":
+        args.warmup, args.steps = 2, 2
     # ---------------- plain FP32 single AA steps
     run_aa(args.warmup)
     torch.cuda.synchronize()
@@ -89,7 +103,7 @@ def main():
     forms = {"aa_fp32": {"glups": glups, "us_per_step": kern_ms * 1e3, "achieved_gbs": achieved}}
 
     # ---------------- fused form: autotuned (k, strip, chunk), then timed like the plain form
-    if L.lbm_fused_max_k() >= 2 and args.k >= 2:
+    if L.lbm_fused_max_k() >= 2 and args.k >= 2 and args.only != "aa":
         from lbm import autotune
         ks = tuple(k for k in autotune.KS if k <= args.k)
         best, table = autotune.tune_fused(sim, ks=ks, use_cache=False)
