@@ -45,6 +45,15 @@ enum lbm_bc {
     LBM_BC_PERIODIC = 1 /* both directions periodic; mask cells are bounce-back obstacles      */
 };
 
+/* Storage format of the populations.  LBM_FMT_F32 stores f; the 16-bit formats store deviations
+ * h = f - w_q from the rest state (w_q the FP32 weights 0.444444444f, 0.1111111111f, 0.0277777778f of
+ * lbm_core_smemaa.cu:4-5), rounded to nearest even; the collision always computes in FP32. */
+enum lbm_format {
+    LBM_FMT_F32 = 0,
+    LBM_FMT_F16 = 1, /* IEEE half */
+    LBM_FMT_BF16 = 2 /* bfloat16 */
+};
+
 enum lbm_status {
     LBM_OK = 0,
     LBM_ERR_ARG = -1,      /* invalid argument (see lbm_last_error) */
@@ -132,6 +141,27 @@ int lbm_fused_default_cfg(const lbm_params *p, int k, lbm_fused_cfg *cfg);
 int lbm_step_fused_cfg(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out,
                        const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
                        float *rho, lbm_stream_t stream);
+
+/*
+ * 16-bit deviation storage (no reference counterpart: the reference stores FP32 only).  Same semantics as
+ * lbm_step_aa / lbm_step_aa_n / lbm_step_fused_cfg / lbm_macro on h = f - w stored as `fmt`
+ * (LBM_FMT_F16 or LBM_FMT_BF16); stores clamp h to [-w, 10 - w], the population clamp of lbm_core4.cu:203.
+ */
+int lbm_step_aa_h16(const lbm_params *p, const uint8_t *mask, uint16_t *h, int fmt, const float *inlet_u,
+                    int parity, float *ux, float *uy, float *rho, lbm_stream_t stream);
+int lbm_step_aa_h16_n(const lbm_params *p, const uint8_t *mask, uint16_t *h, int fmt, const float *inlet_u,
+                      int parity, int nsteps, lbm_stream_t stream);
+int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h_now, uint16_t *h_out, int fmt,
+                       const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
+                       float *rho, lbm_stream_t stream);
+int lbm_macro_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h, int fmt, const float *inlet_u,
+                  int parity, float *ux, float *uy, float *rho, lbm_stream_t stream);
+
+/*
+ * Convert a whole 9-plane state between formats (FP32 f <-> 16-bit h = f - w; 16-bit -> 16-bit through
+ * FP32).  Round to nearest even, no clamping.  src and dst must not overlap.
+ */
+int lbm_convert(const lbm_params *p, const void *src, int src_fmt, void *dst, int dst_fmt, lbm_stream_t stream);
 
 /*
  * rho, ux, uy of a stored state (macro_kernel of lbm_core_smemaa_endfused.cu:377-379, extended to both

@@ -24,6 +24,8 @@ inline hipStream_t to_hip(lbm_stream_t s) { return reinterpret_cast<hipStream_t>
 
 // Single-step tiles: 256-thread blocks of 128 x 2 cells (each wave64 covers half a 512-byte row segment
 // per plane), measured fastest among 32x8 .. 256x2 on MI355X.
+constexpr int FUSED_MAXK = 8; // largest k of the temporally blocked kernel
+
 constexpr int TILE_X = 128;
 constexpr int TILE_Y = 2;
 

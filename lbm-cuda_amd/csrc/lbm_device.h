@@ -42,6 +42,7 @@ struct KPar {
 #define LBM_W0 0.444444444f
 #define LBM_W1 0.1111111111f
 #define LBM_W5 0.0277777778f
+#define LBM_C_1_12 0.0833333333f
 
 __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
@@ -72,7 +73,7 @@ __device__ __forceinline__ float zou_he(float h[9], float u, float &rho)
 // mac = (rho, ux, uy) as written by the reference (lbm_core4.cu:59-61,145-147).
 __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3])
 {
-    float rho = 0.f, ux = 0.f, uy = 0.f, drc = 0.f, rc = 0.f;
+    float rho = 0.f, ux = 0.f, uy = 0.f, drc = 0.f, rc = 0.f, inv_rc = 0.f;
     bool les = false;
     if (inlet) {
         drc = zou_he(h, uin, rho);
@@ -100,6 +101,8 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
             drc = clampf(drho, -1.0f, 0.5f); // rho in [0, 1.5]
         }
         rc = 1.0f + drc;
+        inv_rc = inv_rho; // 1/rc == 1/rho bit for bit unless the density clamp engaged
+        if (drc != drho) inv_rc = __fdiv_rn(1.0f, rc);
         les = P.les != 0;
     }
     mac[0] = rc;
@@ -129,7 +132,6 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
     if (les) {
         // Smagorinsky (lbm_core4.cu:172-177)
         const float Q = sqrtf(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
-        const float inv_rc = __fdiv_rn(1.0f, rc);
         const float d = fma_(P.c18cs2, Q * inv_rc, P.tau0sq);
         const float tau_eff = 0.5f * (P.tau0 + sqrtf(d));
         w = __fdiv_rn(1.0f, tau_eff);
@@ -141,28 +143,28 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
     m6 = fma_(-P.s_q, m6 + m5, m6);
     m7 = fma_(-w, n7, m7);
     m8 = fma_(-w, n8, m8);
-    // inverse (lbm_core4.cu:187-199) of the deviation moments (rho - 1, m1d, m2d, m3, ..., m8)
+    // inverse (lbm_core4.cu:187-199) of the deviation moments (rho - 1, m1d, m2d, m3, ..., m8), with the
+    // common 1/36 folded into the coefficients: 6/36 = 1/6, 3/36 = 1/12, 9/36 = 1/4
     const float t0 = (drc - m1d) + m2d;
     const float t1 = fma_(-2.0f, m2d, fma_(4.0f, drc, -m1d));
     const float t2 = fma_(4.0f, drc, fma_(2.0f, m1d, m2d));
-    const float A = (m3 - m4) * 6.0f;
-    const float B = (m5 - m6) * 6.0f;
-    const float Pp = m7 * 9.0f;
-    const float X = fma_(6.0f, m3, m4 * 3.0f);
-    const float Y = fma_(6.0f, m5, m6 * 3.0f);
-    const float Z = m8 * 9.0f;
-    const float t1p = t1 + Pp, t1m = t1 - Pp;
-    const float u = t2 + Z, v = t2 - Z;
+    const float q7 = m7 * 0.25f;
+    const float q8 = m8 * 0.25f;
+    const float t1p = fma_(t1, LBM_INV36, q7), t1m = fma_(t1, LBM_INV36, -q7);
+    const float d34 = m3 - m4, d56 = m5 - m6;
+    const float X = fma_(LBM_C_1_6, m3, m4 * LBM_C_1_12);
+    const float Y = fma_(LBM_C_1_6, m5, m6 * LBM_C_1_12);
+    const float u = fma_(t2, LBM_INV36, q8), v = fma_(t2, LBM_INV36, -q8);
     const float XpY = X + Y, XmY = X - Y;
     h[0] = t0 * LBM_INV9;
-    h[1] = (t1p + A) * LBM_INV36;
-    h[2] = (t1m + B) * LBM_INV36;
-    h[3] = (t1p - A) * LBM_INV36;
-    h[4] = (t1m - B) * LBM_INV36;
-    h[5] = (u + XpY) * LBM_INV36;
-    h[6] = (v - XmY) * LBM_INV36;
-    h[7] = (u - XpY) * LBM_INV36;
-    h[8] = (v + XmY) * LBM_INV36;
+    h[1] = fma_(d34, LBM_C_1_6, t1p);
+    h[2] = fma_(d56, LBM_C_1_6, t1m);
+    h[3] = fma_(d34, -LBM_C_1_6, t1p);
+    h[4] = fma_(d56, -LBM_C_1_6, t1m);
+    h[5] = u + XpY;
+    h[6] = v - XmY;
+    h[7] = u - XpY;
+    h[8] = v + XmY;
 }
 
 // BGK collision (lbm_core_smemaa.cu:159-212) on deviations: h' = h + tau_inv (heq - h) with
@@ -205,7 +207,83 @@ __device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inle
         collide_mrt(P, h, inlet, uin, mac);
 }
 
-// FP32 storage: f -> h = f - w -> collide -> f' = clamp(w + h', 0, 10)   (lbm_core4.cu:203)
+// ------------------------------------------------------------------------------------------------------
+// Storage formats.  The kernels move deviations h = f - w between memory and the collision:
+//   FMT_F32   memory holds f (FP32):              h = f - w,            f' = clamp(w + h', 0, 10)
+//   FMT_F16   memory holds h as IEEE half:        h = (float)half,      half(clamp(h', -w, 10 - w))
+//   FMT_BF16  memory holds h as bfloat16:         h = bf16 << 16,       bf16(clamp(h', -w, 10 - w))
+// Both 16-bit conversions round to nearest even (v_cvt_f16_f32, v_cvt_pk_bf16_f32).  The population clamp
+// [0, 10] of lbm_core4.cu:203 becomes the same interval shifted by -w.
+// Memory is addressed through buffer resources: a scalar base, the plane offset as scalar offset and one
+// 32-bit vector byte offset per access (no 64-bit vector address arithmetic); accesses past the end of
+// the resource read 0 and write nothing.
+// ------------------------------------------------------------------------------------------------------
+enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2 };
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned nbytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)nbytes, 0x00020000);
+}
+
+// Register <-> storage conversions of one population (the storage word holds f for FMT_F32, h otherwise).
+template <int FMT> struct Conv;
+
+template <> struct Conv<FMT_F32> {
+    using T = float;
+    static constexpr unsigned BYTES = 4;
+    __device__ static float ld(T v, int q) { return v - wq(q); }
+    __device__ static T st(float h, int q) { return clampf(wq(q) + h, 0.0f, 10.0f); }
+    __device__ static float pop(T v, int) { return v; } // the population f itself
+    __device__ static T from_pop(float f, int) { return f; }
+};
+
+template <> struct Conv<FMT_F16> {
+    using T = uint16_t;
+    static constexpr unsigned BYTES = 2;
+    __device__ static float ld(T v, int) { return (float)__builtin_bit_cast(_Float16, v); }
+    __device__ static T st(float h, int q)
+    {
+        return __builtin_bit_cast(uint16_t, (_Float16)clampf(h, -wq(q), 10.0f - wq(q)));
+    }
+    __device__ static float pop(T v, int q) { return wq(q) + ld(v, q); }
+    __device__ static T from_pop(float f, int q) { return __builtin_bit_cast(uint16_t, (_Float16)(f - wq(q))); }
+};
+
+template <> struct Conv<FMT_BF16> {
+    using T = uint16_t;
+    static constexpr unsigned BYTES = 2;
+    __device__ static float ld(T v, int) { return __builtin_bit_cast(float, (unsigned)v << 16); }
+    __device__ static T st(float h, int q)
+    {
+        return __builtin_bit_cast(uint16_t, (__bf16)clampf(h, -wq(q), 10.0f - wq(q)));
+    }
+    __device__ static float pop(T v, int q) { return wq(q) + ld(v, q); }
+    __device__ static T from_pop(float f, int q) { return __builtin_bit_cast(uint16_t, (__bf16)(f - wq(q))); }
+};
+
+// Buffer-resource loads and stores of one population in storage format FMT.
+template <int FMT> struct Store {
+    using C = Conv<FMT>;
+    static constexpr unsigned BYTES = C::BYTES;
+    __device__ static float ld(rsrc_t r, unsigned voff, unsigned soff, int q)
+    {
+        if constexpr (BYTES == 4)
+            return C::ld(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0)), q);
+        else
+            return C::ld((uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0), q);
+    }
+    __device__ static void st(rsrc_t r, unsigned voff, unsigned soff, int q, float h)
+    {
+        if constexpr (BYTES == 4)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, C::st(h, q)), r, voff, soff, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b16(C::st(h, q), r, voff, soff, 0);
+    }
+};
+
+// FP32 populations in registers (pull kernel and the fused kernel's LDS): f -> h -> collide -> f'
 template <int MODEL>
 __device__ __forceinline__ void collide(const KPar &P, float f[9], bool inlet, float uin, float mac[3])
 {

@@ -84,79 +84,88 @@ __device__ __forceinline__ int nbr_index(const KPar &P, int x, int y, int q)
     return ny * P.W + nx;
 }
 
-template <int MODEL, int PARITY, bool PERIODIC, bool MACROS>
-__global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__restrict__ mask, float *__restrict__ f,
+template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__restrict__ mask, void *__restrict__ f,
                                                         const float *__restrict__ inlet_u, float *__restrict__ oux,
                                                         float *__restrict__ ouy, float *__restrict__ orho)
 {
+    using S = Store<FMT>;
+    constexpr unsigned B = S::BYTES;
     int bx, by;
     xcd_tile(bx, by);
     const int x = bx * TILE_X + threadIdx.x;
     const int y = by * TILE_Y + threadIdx.y;
     if (x >= P.W || y >= P.H) return;
-    const int W = P.W, plane = P.plane;
-    const int idx = y * W + x;
+    const int W = P.W;
+    const unsigned plane = (unsigned)P.plane;
+    const unsigned idx = (unsigned)(y * W + x);
+    const rsrc_t rf = make_rsrc(f, 9u * plane * B);
+    const rsrc_t rm = make_rsrc(mask, plane);
+    auto solid_at = [&](unsigned cell) { return __builtin_amdgcn_raw_buffer_load_b8(rm, cell, 0, 0) != 0; };
     bool inlet = false;
     if constexpr (!PERIODIC) {
         // wall rows and the outlet ghost are never updated (lbm_core4.cu:20-24)
         if (y == 0 || y == P.H - 1 || x == W - 1) return;
         inlet = x == 0;
-        if (!inlet && mask[idx]) return;
+        if (!inlet && solid_at(idx)) return;
     } else {
-        if (mask[idx]) return;
+        if (solid_at(idx)) return;
     }
     float g[9], mac[3];
-    int nb[9];
-    uint8_t solid[9];
+    unsigned nb[9];
+    bool solid[9];
     if constexpr (PARITY == 0) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) g[q] = f[q * plane + idx];
+        for (int q = 0; q < 9; ++q) g[q] = S::ld(rf, idx * B, (unsigned)q * plane * B, q);
     } else {
         nb[0] = idx;
-        solid[0] = 0;
+        solid[0] = false;
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
-            if (!PERIODIC && inlet && cx(q) < 0) { // x = -1 does not exist
-                nb[q] = idx;
-                solid[q] = 0;
+            if constexpr (PERIODIC) {
+                int nx = x + cx(q), ny = y + cy(q);
+                nx = nx < 0 ? nx + W : (nx >= W ? nx - W : nx);
+                ny = ny < 0 ? ny + P.H : (ny >= P.H ? ny - P.H : ny);
+                nb[q] = (unsigned)(ny * W + nx);
             } else {
-                nb[q] = nbr_index<PERIODIC>(P, x, y, q);
-                solid[q] = mask[nb[q]];
+                nb[q] = idx + (unsigned)(cx(q) + cy(q) * W); // x = -1 only at the inlet, never used there
             }
+            solid[q] = (!PERIODIC && inlet && cx(q) < 0) ? false : solid_at(nb[q]);
         }
-        g[0] = f[idx];
+        g[0] = S::ld(rf, idx * B, 0, 0);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
-            if (!PERIODIC && inlet && cx(q) > 0)
-                g[q] = 0.0f; // arrives from x = -1; rebuilt by Zou-He
-            else
-                g[q] = solid[o] ? f[q * plane + idx] : f[o * plane + nb[o]];
+            if (!PERIODIC && inlet && cx(q) > 0) {
+                g[q] = 0.0f - wq(q); // arrives from x = -1; rebuilt by Zou-He
+            } else {
+                // bounce-back from a solid source reads this cell's slot q, else the source's slot opp(q)
+                const unsigned e = solid[o] ? (unsigned)q * plane + idx : (unsigned)o * plane + nb[o];
+                g[q] = S::ld(rf, e * B, 0, q);
+            }
         }
     }
-    collide<MODEL>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
+    collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
     if constexpr (PARITY == 0) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) f[opp(q) * plane + idx] = g[q];
+        for (int q = 0; q < 9; ++q) S::st(rf, idx * B, (unsigned)opp(q) * plane * B, q, g[q]);
         if (!PERIODIC && x == W - 2) { // reversed copy into the outlet ghost (lbm_core4.cu:114-116)
 #pragma unroll
-            for (int q = 0; q < 9; ++q) f[opp(q) * plane + idx + 1] = g[q];
+            for (int q = 0; q < 9; ++q) S::st(rf, (idx + 1) * B, (unsigned)opp(q) * plane * B, q, g[q]);
         }
     } else {
-        f[idx] = g[0];
+        S::st(rf, idx * B, 0, 0, g[0]);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             if (!PERIODIC && inlet && cx(q) < 0) continue; // leaves through the open inlet
-            if (solid[q])
-                f[opp(q) * plane + idx] = g[q];
-            else
-                f[q * plane + nb[q]] = g[q];
+            const unsigned e = solid[q] ? (unsigned)opp(q) * plane + idx : (unsigned)q * plane + nb[q];
+            S::st(rf, e * B, 0, q, g[q]);
         }
         if (!PERIODIC && x == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-            f[3 * plane + idx] = g[3];
-            if (!solid[2]) f[6 * plane + nb[2]] = g[6];
-            if (!solid[4]) f[7 * plane + nb[4]] = g[7];
+            S::st(rf, idx * B, 3u * plane * B, 3, g[3]);
+            if (!solid[2]) S::st(rf, nb[2] * B, 6u * plane * B, 6, g[6]);
+            if (!solid[4]) S::st(rf, nb[4] * B, 7u * plane * B, 7, g[7]);
         }
     }
     if constexpr (MACROS) {
@@ -166,32 +175,49 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__
     }
 }
 
-template <int MODEL, int PARITY, bool PERIODIC, bool MACROS>
-static void launch_aa(const KPar &P, dim3 grid, hipStream_t s, const uint8_t *mask, float *f, const float *inlet_u,
+template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS>
+static void launch_aa(const KPar &P, dim3 grid, hipStream_t s, const uint8_t *mask, void *f, const float *inlet_u,
                       float *ux, float *uy, float *rho)
 {
-    hipLaunchKernelGGL((k_aa<MODEL, PARITY, PERIODIC, MACROS>), grid, dim3(TILE_X, TILE_Y), 0, s, P, mask, f, inlet_u, ux,
-                       uy, rho);
+    hipLaunchKernelGGL((k_aa<MODEL, FMT, PARITY, PERIODIC, MACROS>), grid, dim3(TILE_X, TILE_Y), 0, s, P, mask, f,
+                       inlet_u, ux, uy, rho);
 }
 
-template <int MODEL, int PARITY, bool PERIODIC>
-static void dispatch_aa_m(const KPar &P, dim3 grid, hipStream_t s, const uint8_t *mask, float *f, const float *inlet_u,
+template <int MODEL, int FMT, int PARITY, bool PERIODIC>
+static void dispatch_aa_m(const KPar &P, dim3 grid, hipStream_t s, const uint8_t *mask, void *f, const float *inlet_u,
                           float *ux, float *uy, float *rho)
 {
     if (rho)
-        launch_aa<MODEL, PARITY, PERIODIC, true>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+        launch_aa<MODEL, FMT, PARITY, PERIODIC, true>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
     else
-        launch_aa<MODEL, PARITY, PERIODIC, false>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+        launch_aa<MODEL, FMT, PARITY, PERIODIC, false>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
 }
 
-template <int MODEL, int PARITY>
-static void dispatch_aa_b(const KPar &P, int bc, dim3 grid, hipStream_t s, const uint8_t *mask, float *f,
-                          const float *inlet_u, float *ux, float *uy, float *rho)
+template <int MODEL, int FMT>
+static void dispatch_aa(const KPar &P, int bc, int parity, dim3 grid, hipStream_t s, const uint8_t *mask, void *f,
+                        const float *inlet_u, float *ux, float *uy, float *rho)
 {
-    if (bc == LBM_BC_PERIODIC)
-        dispatch_aa_m<MODEL, PARITY, true>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+    if (bc == LBM_BC_PERIODIC) {
+        if (parity)
+            dispatch_aa_m<MODEL, FMT, 1, true>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+        else
+            dispatch_aa_m<MODEL, FMT, 0, true>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+    } else {
+        if (parity)
+            dispatch_aa_m<MODEL, FMT, 1, false>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+        else
+            dispatch_aa_m<MODEL, FMT, 0, false>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+    }
+}
+
+template <int FMT>
+static void dispatch_aa_fmt(const KPar &P, int model, int bc, int parity, dim3 grid, hipStream_t s,
+                            const uint8_t *mask, void *f, const float *inlet_u, float *ux, float *uy, float *rho)
+{
+    if (model == LBM_BGK)
+        dispatch_aa<MODEL_BGK, FMT>(P, bc, parity, grid, s, mask, f, inlet_u, ux, uy, rho);
     else
-        dispatch_aa_m<MODEL, PARITY, false>(P, grid, s, mask, f, inlet_u, ux, uy, rho);
+        dispatch_aa<MODEL_MRT, FMT>(P, bc, parity, grid, s, mask, f, inlet_u, ux, uy, rho);
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -245,11 +271,12 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_pull(KPar P, const uint8_t *
 // macro extraction (lbm_core_smemaa_endfused.cu:377-422), both parities
 // ------------------------------------------------------------------------------------------------------
 
-template <bool PERIODIC>
+template <bool PERIODIC, int FMT>
 __global__ __launch_bounds__(TILE_X *TILE_Y) void k_macro(KPar P, const uint8_t *__restrict__ mask,
-                                                           const float *__restrict__ f, const float *__restrict__ inlet_u,
-                                                           int parity, float *__restrict__ oux,
-                                                           float *__restrict__ ouy, float *__restrict__ orho)
+                                                           const typename Conv<FMT>::T *__restrict__ f,
+                                                           const float *__restrict__ inlet_u, int parity,
+                                                           float *__restrict__ oux, float *__restrict__ ouy,
+                                                           float *__restrict__ orho)
 {
     const int x = blockIdx.x * TILE_X + threadIdx.x;
     const int y = blockIdx.y * TILE_Y + threadIdx.y;
@@ -265,7 +292,7 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_macro(KPar P, const uint8_t 
     }
     float g[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) g[q] = f[(parity ? opp(q) : q) * P.plane + idx];
+    for (int q = 0; q < 9; ++q) g[q] = Conv<FMT>::pop(f[(parity ? opp(q) : q) * P.plane + idx], q);
     float rho, ux, uy;
     if (inlet && parity == 0) {
         const float u = inlet_u[y];
@@ -311,6 +338,21 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_visualize(int W, int H, cons
     image[pid * 4 + 0] = (uint8_t)(r * 255.0f);
     image[pid * 4 + 1] = (uint8_t)(g * 255.0f);
     image[pid * 4 + 2] = (uint8_t)(b * 255.0f);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// format conversion of a whole state (populations f <-> deviations h = f - w)
+// ------------------------------------------------------------------------------------------------------
+
+template <int SRC, int DST>
+__global__ __launch_bounds__(256) void k_convert(unsigned plane, const typename Conv<SRC>::T *__restrict__ src,
+                                                  typename Conv<DST>::T *__restrict__ dst)
+{
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= plane) return;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        dst[(unsigned)q * plane + i] = Conv<DST>::from_pop(Conv<SRC>::pop(src[(unsigned)q * plane + i], q), q);
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -435,19 +477,7 @@ int lbm_step_aa(const lbm_params *p, const uint8_t *mask, float *f, const float 
     if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_step_aa: parity must be 0 or 1");
     if ((ux == nullptr) != (rho == nullptr) || (uy == nullptr) != (rho == nullptr))
         return fail(LBM_ERR_ARG, "lbm_step_aa: ux, uy, rho must be all NULL or all non-NULL");
-    const dim3 grid = step_grid(P);
-    hipStream_t s = to_hip(stream);
-    if (p->model == LBM_BGK) {
-        if (parity == 0)
-            dispatch_aa_b<MODEL_BGK, 0>(P, p->bc, grid, s, mask, f, inlet_u, ux, uy, rho);
-        else
-            dispatch_aa_b<MODEL_BGK, 1>(P, p->bc, grid, s, mask, f, inlet_u, ux, uy, rho);
-    } else {
-        if (parity == 0)
-            dispatch_aa_b<MODEL_MRT, 0>(P, p->bc, grid, s, mask, f, inlet_u, ux, uy, rho);
-        else
-            dispatch_aa_b<MODEL_MRT, 1>(P, p->bc, grid, s, mask, f, inlet_u, ux, uy, rho);
-    }
+    dispatch_aa_fmt<FMT_F32>(P, p->model, p->bc, parity, step_grid(P), to_hip(stream), mask, f, inlet_u, ux, uy, rho);
     return check_launch("lbm_step_aa");
 }
 
@@ -501,9 +531,9 @@ int lbm_macro(const lbm_params *p, const uint8_t *mask, const float *f, const fl
     const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);
     hipStream_t s = to_hip(stream);
     if (p->bc == LBM_BC_PERIODIC)
-        hipLaunchKernelGGL((k_macro<true>), grid, block, 0, s, P, mask, f, inlet_u, parity, ux, uy, rho);
+        hipLaunchKernelGGL((k_macro<true, FMT_F32>), grid, block, 0, s, P, mask, f, inlet_u, parity, ux, uy, rho);
     else
-        hipLaunchKernelGGL((k_macro<false>), grid, block, 0, s, P, mask, f, inlet_u, parity, ux, uy, rho);
+        hipLaunchKernelGGL((k_macro<false, FMT_F32>), grid, block, 0, s, P, mask, f, inlet_u, parity, ux, uy, rho);
     return check_launch("lbm_macro");
 }
 
@@ -517,6 +547,98 @@ int lbm_visualize(const lbm_params *p, const float *ux, const float *uy, const u
     const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);
     hipLaunchKernelGGL(k_visualize, grid, block, 0, to_hip(stream), P.W, P.H, ux, uy, mask, rgba, vort_scale);
     return check_launch("lbm_visualize");
+}
+
+int lbm_step_aa_h16(const lbm_params *p, const uint8_t *mask, uint16_t *h, int fmt, const float *inlet_u, int parity,
+                    float *ux, float *uy, float *rho, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P);
+    if (rc) return rc;
+    if (!mask || !h) return fail(LBM_ERR_ARG, "lbm_step_aa_h16: mask and h must be non-NULL");
+    if (fmt != LBM_FMT_F16 && fmt != LBM_FMT_BF16) return fail(LBM_ERR_ARG, "lbm_step_aa_h16: bad format %d", fmt);
+    if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_step_aa_h16: channel mode needs inlet_u");
+    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_step_aa_h16: parity must be 0 or 1");
+    if ((ux == nullptr) != (rho == nullptr) || (uy == nullptr) != (rho == nullptr))
+        return fail(LBM_ERR_ARG, "lbm_step_aa_h16: ux, uy, rho must be all NULL or all non-NULL");
+    if (fmt == LBM_FMT_F16)
+        dispatch_aa_fmt<FMT_F16>(P, p->model, p->bc, parity, step_grid(P), to_hip(stream), mask, h, inlet_u, ux, uy,
+                                 rho);
+    else
+        dispatch_aa_fmt<FMT_BF16>(P, p->model, p->bc, parity, step_grid(P), to_hip(stream), mask, h, inlet_u, ux, uy,
+                                  rho);
+    return check_launch("lbm_step_aa_h16");
+}
+
+int lbm_step_aa_h16_n(const lbm_params *p, const uint8_t *mask, uint16_t *h, int fmt, const float *inlet_u, int parity,
+                      int nsteps, lbm_stream_t stream)
+{
+    if (nsteps < 0) return fail(LBM_ERR_ARG, "lbm_step_aa_h16_n: nsteps must be >= 0");
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = lbm_step_aa_h16(p, mask, h, fmt, inlet_u, (parity + i) & 1, nullptr, nullptr, nullptr, stream);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
+int lbm_macro_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h, int fmt, const float *inlet_u,
+                  int parity, float *ux, float *uy, float *rho, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P);
+    if (rc) return rc;
+    if (!mask || !h || !ux || !uy || !rho) return fail(LBM_ERR_ARG, "lbm_macro_h16: NULL pointer");
+    if (fmt != LBM_FMT_F16 && fmt != LBM_FMT_BF16) return fail(LBM_ERR_ARG, "lbm_macro_h16: bad format %d", fmt);
+    if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_macro_h16: channel mode needs inlet_u");
+    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_macro_h16: parity must be 0 or 1");
+    const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);
+    hipStream_t s = to_hip(stream);
+    const bool per = p->bc == LBM_BC_PERIODIC;
+    if (fmt == LBM_FMT_F16) {
+        if (per)
+            hipLaunchKernelGGL((k_macro<true, FMT_F16>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy, rho);
+        else
+            hipLaunchKernelGGL((k_macro<false, FMT_F16>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy, rho);
+    } else {
+        if (per)
+            hipLaunchKernelGGL((k_macro<true, FMT_BF16>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy, rho);
+        else
+            hipLaunchKernelGGL((k_macro<false, FMT_BF16>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy, rho);
+    }
+    return check_launch("lbm_macro_h16");
+}
+
+int lbm_convert(const lbm_params *p, const void *src, int src_fmt, void *dst, int dst_fmt, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P);
+    if (rc) return rc;
+    if (!src || !dst) return fail(LBM_ERR_ARG, "lbm_convert: NULL pointer");
+    auto bytes = [](int f) { return f == LBM_FMT_F32 ? 4u : 2u; };
+    if (src_fmt < 0 || src_fmt > 2 || dst_fmt < 0 || dst_fmt > 2) return fail(LBM_ERR_ARG, "lbm_convert: bad format");
+    {
+        const char *a = static_cast<const char *>(src), *b = static_cast<const char *>(dst);
+        if (a < b + 9ull * P.plane * bytes(dst_fmt) && b < a + 9ull * P.plane * bytes(src_fmt))
+            return fail(LBM_ERR_ARG, "lbm_convert: src and dst overlap");
+    }
+    const dim3 grid((P.plane + 255) / 256), block(256);
+    hipStream_t s = to_hip(stream);
+    const unsigned plane = (unsigned)P.plane;
+#define LBM_CONV(S, D)                                                                                                  \
+    if (src_fmt == S && dst_fmt == D) {                                                                                 \
+        hipLaunchKernelGGL((k_convert<S, D>), grid, block, 0, s, plane,                                                 \
+                           static_cast<const typename Conv<S>::T *>(src), static_cast<typename Conv<D>::T *>(dst));     \
+        return check_launch("lbm_convert");                                                                             \
+    }
+    LBM_CONV(FMT_F32, FMT_F16)
+    LBM_CONV(FMT_F32, FMT_BF16)
+    LBM_CONV(FMT_F16, FMT_F32)
+    LBM_CONV(FMT_BF16, FMT_F32)
+    LBM_CONV(FMT_F16, FMT_BF16)
+    LBM_CONV(FMT_BF16, FMT_F16)
+    LBM_CONV(FMT_F32, FMT_F32)
+#undef LBM_CONV
+    return fail(LBM_ERR_UNSUPPORTED, "lbm_convert: %d -> %d not supported", src_fmt, dst_fmt);
 }
 
 size_t lbm_diagnostics_work_size(const lbm_params *) { return sizeof(double) * DIAG_BLOCKS * DIAG_N; }

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "liblbm.so")
 
 MRT_LES, BGK = 0, 1
 CHANNEL, PERIODIC = 0, 1
+FMT_F32, FMT_F16, FMT_BF16 = 0, 1, 2
 
 ABI_VERSION = 1
 
@@ -62,6 +63,11 @@ SIGNATURES = {
     "lbm_fused_default_cfg": (_i, [_P, _i, _vp]),
     "lbm_step_fused_cfg": (_i, [_P, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "lbm_macro": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "lbm_step_aa_h16": (_i, [_P, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp]),
+    "lbm_step_aa_h16_n": (_i, [_P, _vp, _vp, _i, _vp, _i, _i, _vp]),
+    "lbm_step_fused_h16": (_i, [_P, _vp, _vp, _vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "lbm_macro_h16": (_i, [_P, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp]),
+    "lbm_convert": (_i, [_P, _vp, _i, _vp, _i, _vp]),
     "lbm_visualize": (_i, [_P, _vp, _vp, _vp, _vp, _f, _vp]),
     "lbm_diagnostics_work_size": (ctypes.c_size_t, [_P]),
     "lbm_diagnostics": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp]),

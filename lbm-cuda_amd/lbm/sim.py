@@ -4,8 +4,9 @@ Mirrors the flow of the reference drivers (lbm_smemaa_main.py:21-174): build the
 mask on the host, move them to the GPU, then advance with single steps or k-step fused launches.  The
 state lives in torch tensors only for device memory and streams; all compute is liblbm's HIP kernels.
 
-Storage: ``f`` is float32 [9, H, W] (SoA planes, x fastest).  ``parity`` is the AA parity of ``f``
-(0 = canonical, 1 = reversed).  The fused kernel needs a second buffer, allocated on first use.
+Storage: ``f`` is float32 [9, H, W] (SoA planes, x fastest) for ``storage="f32"``; for "f16"/"bf16" it
+holds the 16-bit deviations h = f - w (int16 tensor of raw bits).  ``parity`` is the AA parity of the
+state (0 = canonical, 1 = reversed).  The fused kernel needs a second buffer, allocated on first use.
 """
 from __future__ import annotations
 
@@ -19,6 +20,7 @@ from . import geometry as geo
 
 _MODELS = {"mrt": _lib.MRT_LES, "mrt_les": _lib.MRT_LES, "bgk": _lib.BGK}
 _BCS = {"channel": _lib.CHANNEL, "periodic": _lib.PERIODIC}
+_FMTS = {"f32": _lib.FMT_F32, "f16": _lib.FMT_F16, "bf16": _lib.FMT_BF16}
 
 
 def _ptr(t):
@@ -42,7 +44,7 @@ class Simulation:
     """
 
     def __init__(self, W, H, tau_inv, *, cs=0.0, model="mrt", bc="channel", mask=None, inlet_u=None, f0=None,
-                 clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda"):
+                 clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda", storage="f32"):
         self.lib = _lib.load()
         self.W, self.H = int(W), int(H)
         self.model = _MODELS[model] if isinstance(model, str) else int(model)
@@ -71,7 +73,16 @@ class Simulation:
         f0 = np.asarray(f0)
         if f0.shape != (9, self.H, self.W):
             raise ValueError(f"f0 shape {f0.shape} != (9, {self.H}, {self.W})")
-        self.f = torch.from_numpy(np.ascontiguousarray(f0, dtype=np.float32)).to(self.device)
+        self.fmt = _FMTS[storage] if isinstance(storage, str) else int(storage)
+        f32 = torch.from_numpy(np.ascontiguousarray(f0, dtype=np.float32)).to(self.device)
+        if self.fmt == _lib.FMT_F32:
+            self.f = f32
+        else:
+            # deviations h = f - w rounded to nearest even on the device
+            self.f = torch.empty(f32.shape, dtype=torch.int16, device=self.device)
+            _lib.check(self.lib.lbm_convert(ctypes.byref(self.params), _ptr(f32), _lib.FMT_F32, _ptr(self.f), self.fmt,
+                                            _stream()), "lbm_convert")
+            del f32
         self.f_alt = None
         self.parity = int(parity)
         self.steps = 0
@@ -99,8 +110,13 @@ class Simulation:
 
     def _aa(self, macros=False):
         m = (self.ux, self.uy, self.rho) if macros else (None, None, None)
-        rc = self.lib.lbm_step_aa(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
-                                  self.parity, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]), _stream())
+        if self.fmt == _lib.FMT_F32:
+            rc = self.lib.lbm_step_aa(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
+                                      self.parity, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]), _stream())
+        else:
+            rc = self.lib.lbm_step_aa_h16(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), self.fmt,
+                                          _ptr(self.inlet_u), self.parity, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]),
+                                          _stream())
         _lib.check(rc, "lbm_step_aa")
         self.parity ^= 1
         self.steps += 1
@@ -110,9 +126,14 @@ class Simulation:
             self.f_alt = torch.empty_like(self.f)
         m = (self.ux, self.uy, self.rho) if macros else (None, None, None)
         c = None if cfg is None else ctypes.byref(_lib.LbmFusedCfg(int(cfg[0]), int(cfg[1])))
-        rc = self.lib.lbm_step_fused_cfg(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.f_alt),
-                                         _ptr(self.inlet_u), self.parity, k, c, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]),
-                                         _stream())
+        if self.fmt == _lib.FMT_F32:
+            rc = self.lib.lbm_step_fused_cfg(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f),
+                                             _ptr(self.f_alt), _ptr(self.inlet_u), self.parity, k, c, _ptr(m[0]),
+                                             _ptr(m[1]), _ptr(m[2]), _stream())
+        else:
+            rc = self.lib.lbm_step_fused_h16(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f),
+                                             _ptr(self.f_alt), self.fmt, _ptr(self.inlet_u), self.parity, k, c,
+                                             _ptr(m[0]), _ptr(m[1]), _ptr(m[2]), _stream())
         _lib.check(rc, "lbm_step_fused")
         self.f, self.f_alt = self.f_alt, self.f
         self.parity = (self.parity + k) & 1
@@ -121,10 +142,24 @@ class Simulation:
     # ------------------------------------------------------------------ observables
     def macro(self):
         """(rho, ux, uy) of the stored state (torch tensors on the device)."""
-        rc = self.lib.lbm_macro(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
-                                self.parity, _ptr(self.ux), _ptr(self.uy), _ptr(self.rho), _stream())
+        if self.fmt == _lib.FMT_F32:
+            rc = self.lib.lbm_macro(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
+                                    self.parity, _ptr(self.ux), _ptr(self.uy), _ptr(self.rho), _stream())
+        else:
+            rc = self.lib.lbm_macro_h16(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), self.fmt,
+                                        _ptr(self.inlet_u), self.parity, _ptr(self.ux), _ptr(self.uy), _ptr(self.rho),
+                                        _stream())
         _lib.check(rc, "lbm_macro")
         return self.rho, self.ux, self.uy
+
+    def populations(self):
+        """FP32 populations f of the stored state (a device tensor; converted for 16-bit storage)."""
+        if self.fmt == _lib.FMT_F32:
+            return self.f
+        out = torch.empty(self.f.shape, dtype=torch.float32, device=self.device)
+        _lib.check(self.lib.lbm_convert(ctypes.byref(self.params), _ptr(self.f), self.fmt, _ptr(out), _lib.FMT_F32,
+                                        _stream()), "lbm_convert")
+        return out
 
     def diagnostics(self):
         """FP64 (mass, jx, jy, Fx, Fy) reduced on the GPU."""
@@ -132,7 +167,8 @@ class Simulation:
             nbytes = self.lib.lbm_diagnostics_work_size(ctypes.byref(self.params))
             self._diag_work = torch.empty(nbytes // 8, dtype=torch.float64, device=self.device)
             self._diag_out = torch.empty(5, dtype=torch.float64, device=self.device)
-        rc = self.lib.lbm_diagnostics(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
+        f = self.populations()
+        rc = self.lib.lbm_diagnostics(ctypes.byref(self.params), _ptr(self.mask), _ptr(f), _ptr(self.inlet_u),
                                       self.parity, _ptr(self._diag_work), _ptr(self._diag_out), _stream())
         _lib.check(rc, "lbm_diagnostics")
         return self._diag_out.cpu().numpy().copy()
@@ -153,8 +189,9 @@ class Simulation:
         return img
 
     def state(self):
-        """(f as numpy float32 [9, H, W], parity)."""
-        return self.f.cpu().numpy(), self.parity
+        """(stored state as numpy [9, H, W] -- float32 f, or uint16 deviation bits -- , parity)."""
+        a = self.f.cpu().numpy()
+        return (a if self.fmt == _lib.FMT_F32 else a.view(np.uint16)), self.parity
 
     def synchronize(self):
         _lib.check(self.lib.lbm_sync(_stream()), "lbm_sync")
@@ -163,17 +200,19 @@ class Simulation:
     def save(self, path):
         """Write f, parity, step count and parameters to an .npz file."""
         p = self.params
-        np.savez(path, f=self.f.cpu().numpy(), parity=self.parity, steps=self.steps, mask=self.mask_host,
-                 inlet_u=self.inlet_u_host,
-                 params=np.array([p.W, p.H, p.model, p.bc, p.clamp], np.int64),
+        np.savez(path, f=self.populations().cpu().numpy(), parity=self.parity, steps=self.steps, mask=self.mask_host,
+                 inlet_u=self.inlet_u_host, raw=self.f.cpu().numpy(),
+                 params=np.array([p.W, p.H, p.model, p.bc, p.clamp, self.fmt], np.int64),
                  fparams=np.array([p.tau_inv, p.cs, p.s_e, p.s_eps, p.s_q], np.float32))
 
     @classmethod
     def load(cls, path, device="cuda"):
         z = np.load(path, allow_pickle=False)
-        W, H, model, bc, clamp = (int(v) for v in z["params"])
+        W, H, model, bc, clamp, fmt = (int(v) for v in z["params"])
         tau_inv, cs, s_e, s_eps, s_q = (float(v) for v in z["fparams"])
         sim = cls(W, H, tau_inv, cs=cs, model=model, bc=bc, mask=z["mask"], inlet_u=z["inlet_u"], f0=z["f"],
-                  clamp=bool(clamp), s_e=s_e, s_eps=s_eps, s_q=s_q, parity=int(z["parity"]), device=device)
+                  clamp=bool(clamp), s_e=s_e, s_eps=s_eps, s_q=s_q, parity=int(z["parity"]), device=device,
+                  storage=fmt)
+        sim.f.copy_(torch.from_numpy(z["raw"]).to(sim.device))  # exact stored bits
         sim.steps = int(z["steps"])
         return sim

@@ -77,6 +77,9 @@ def lib():
         L.oracle_mrt_forward_f64.argtypes = [vp, vp]
         L.oracle_mrt_inverse_f64.argtypes = [vp, vp]
         L.oracle_diagnostics_f32.argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
+        L.oracle_aa_step_h16.argtypes = [P, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp]
+        L.oracle_convert_f32_h16.argtypes = [vp, vp, ctypes.c_long, ctypes.c_int]
+        L.oracle_convert_h16_f32.argtypes = [vp, vp, ctypes.c_long, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -164,6 +167,46 @@ def diagnostics(p, mask, f32, inlet_u32, parity):
     out = np.zeros(5)
     lib().oracle_diagnostics_f32(ctypes.byref(p), _ptr(mask), _ptr(f32), _ptr(inlet_u32), int(parity), _ptr(out))
     return out
+
+
+F16, BF16 = 1, 2
+W_F32 = np.array([0.444444444] + [0.1111111111] * 4 + [0.0277777778] * 4, dtype=np.float32)
+
+
+def to_h16(x32, fmt):
+    """float32 -> IEEE half (fmt 1) or bfloat16 (fmt 2) bits, round to nearest even."""
+    x = np.ascontiguousarray(x32, dtype=np.float32)
+    y = np.empty(x.shape, np.uint16)
+    lib().oracle_convert_f32_h16(_ptr(x), _ptr(y), x.size, int(fmt))
+    return y
+
+
+def from_h16(bits, fmt):
+    b = np.ascontiguousarray(bits, dtype=np.uint16)
+    y = np.empty(b.shape, np.float32)
+    lib().oracle_convert_h16_f32(_ptr(b), _ptr(y), b.size, int(fmt))
+    return y
+
+
+def deviations(f64, fmt):
+    """16-bit deviation state h = f - w (w: the FP32 weight literals) from an FP64 population array."""
+    h = (np.asarray(f64, np.float64) - W_F32.astype(np.float64)[:, None, None]).astype(np.float32)
+    return to_h16(h, fmt)
+
+
+def populations(h16, fmt):
+    """FP64 populations w + h of a 16-bit deviation state."""
+    return from_h16(h16, fmt).astype(np.float64) + W_F32.astype(np.float64)[:, None, None]
+
+
+def aa_step_h16(p, mask, h16, fmt, inlet_u32, parity, macros=False):
+    """One AA step in place on a 16-bit deviation state (FP32 collision, RNE storage)."""
+    assert h16.dtype == np.uint16 and h16.shape == (9, p.H, p.W) and h16.flags.c_contiguous
+    prof = np.ascontiguousarray(inlet_u32, dtype=np.float32)
+    mac = [np.zeros((p.H, p.W), np.float32) for _ in range(3)] if macros else [None] * 3
+    lib().oracle_aa_step_h16(ctypes.byref(p), _ptr(mask), _ptr(h16), int(fmt), _ptr(prof), int(parity),
+                             _ptr(mac[1]), _ptr(mac[2]), _ptr(mac[0]))
+    return tuple(mac) if macros else None
 
 
 # --------------------------------------------------------------------------------------------------

@@ -12,6 +12,7 @@
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <string.h>
 
 /* D2Q9 velocity set and opposite table, lbm_core4.cu:1-3 */
 static const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
@@ -44,6 +45,7 @@ typedef struct {
 #define C_W0 (4.0 / 9.0)
 #define C_W1 (1.0 / 9.0)
 #define C_W5 (1.0 / 36.0)
+#define C_1_12 (1.0 / 12.0)
 #include "oracle_impl.h"
 #undef REAL
 #undef SUFFIX
@@ -56,6 +58,7 @@ typedef struct {
 #undef C_W0
 #undef C_W1
 #undef C_W5
+#undef C_1_12
 
 /* ---------------------------- FP32 mirror ---------------------------- */
 /* The literals are the reference's FP32 constants (lbm_core4.cu:15-16,55-57; lbm_core_smemaa.cu:4-7). */
@@ -70,6 +73,7 @@ typedef struct {
 #define C_W0 0.444444444f
 #define C_W1 0.1111111111f
 #define C_W5 0.0277777778f
+#define C_1_12 0.0833333333f
 #include "oracle_impl.h"
 
 /* ---------------------------- FP64 helpers ---------------------------- */
@@ -156,6 +160,153 @@ void oracle_diagnostics_f32(const oracle_params *pp, const uint8_t *mask, const 
     out[2] = jy;
     out[3] = fx;
     out[4] = fy;
+}
+
+/* ---------------------------- 16-bit deviation storage ---------------------------- */
+/* Memory holds h = f - w as IEEE half (fmt 1) or bfloat16 (fmt 2); the collision is the FP32 mirror on
+ * deviations; stores clamp h to [-w, 10 - w] (the population clamp [0, 10], lbm_core4.cu:203) and round
+ * to nearest even. */
+
+static float half_to_float(uint16_t b)
+{
+    uint32_t sign = (uint32_t)(b >> 15) << 31, e = (b >> 10) & 0x1f, m = b & 0x3ff, u;
+    if (e == 0) {
+        if (m == 0) {
+            u = sign;
+        } else { /* subnormal: m * 2^-24 */
+            float v = (float)m * 5.9604644775390625e-08f;
+            memcpy(&u, &v, 4);
+            u |= sign;
+        }
+    } else if (e == 31) {
+        u = sign | 0x7f800000u | (m << 13);
+    } else {
+        u = sign | ((e + 112) << 23) | (m << 13);
+    }
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+static uint16_t float_to_half(float f) /* round to nearest even */
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    uint16_t sign = (uint16_t)((u >> 16) & 0x8000);
+    uint32_t a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return sign | (a > 0x7f800000u ? 0x7e00 : 0x7c00);
+    if (a >= 0x477ff000u) return sign | 0x7c00; /* rounds to >= 65520: infinity */
+    if (a < 0x38800000u) {                      /* below 2^-14: subnormal half (or zero) */
+        float af;
+        memcpy(&af, &a, 4);
+        float scaled = af * 16777216.0f;       /* exact: units of 2^-24 */
+        float r = nearbyintf(scaled);          /* ties to even under the default rounding mode */
+        return sign | (uint16_t)r;
+    }
+    uint32_t mant = a & 0x7fffff, exp = (a >> 23) - 112;
+    uint32_t h = (exp << 10) | (mant >> 13);
+    uint32_t rem = mant & 0x1fff;
+    if (rem > 0x1000 || (rem == 0x1000 && (h & 1))) h += 1;
+    return sign | (uint16_t)h;
+}
+
+static float bf16_to_float(uint16_t b)
+{
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+static uint16_t float_to_bf16(float f) /* round to nearest even */
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+static inline float h16_load(int fmt, uint16_t b) { return fmt == 2 ? bf16_to_float(b) : half_to_float(b); }
+
+static inline uint16_t h16_store(int fmt, int q, float h)
+{
+    float w = wq_f32(q);
+    float lo = -w, hi = 10.0f - w;
+    float c = clampr_f32(h, lo, hi);
+    return fmt == 2 ? float_to_bf16(c) : float_to_half(c);
+}
+
+void oracle_convert_f32_h16(const float *x, uint16_t *y, long n, int fmt)
+{
+    for (long i = 0; i < n; ++i) y[i] = fmt == 2 ? float_to_bf16(x[i]) : float_to_half(x[i]);
+}
+
+void oracle_convert_h16_f32(const uint16_t *x, float *y, long n, int fmt)
+{
+    for (long i = 0; i < n; ++i) y[i] = h16_load(fmt, x[i]);
+}
+
+/* One AA step in place on 16-bit deviations; same streaming as oracle_aa_step_f32. */
+void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *f, int fmt, const float *inlet_u,
+                        int parity, float *oux, float *ouy, float *orho)
+{
+    kpar_t_f32 P;
+    make_kpar_f32(pp, &P);
+    const long plane = (long)P.W * P.H;
+    const int channel = P.bc == ORACLE_CHANNEL;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < P.H; ++y) {
+        for (int x = 0; x < P.W; ++x) {
+            int r = role_f32(&P, mask, x, y);
+            if (!r) continue;
+            const int inlet = r == 2;
+            const long idx = (long)y * P.W + x;
+            float g[9], mac[3];
+            if (parity == 0) {
+                for (int q = 0; q < 9; ++q) g[q] = h16_load(fmt, f[q * plane + idx]);
+            } else {
+                for (int q = 0; q < 9; ++q) {
+                    if (inlet && CX[q] == 1) {
+                        g[q] = -wq_f32(q);
+                        continue;
+                    }
+                    long s = nbr_f32(&P, x, y, -CX[q], -CY[q]);
+                    g[q] = h16_load(fmt, mask[s] ? f[q * plane + idx] : f[OPP[q] * plane + s]);
+                }
+            }
+            if (P.model == ORACLE_BGK)
+                collide_bgk_f32(&P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
+            else
+                collide_mrt_f32(&P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
+            if (parity == 0) {
+                for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx] = h16_store(fmt, q, g[q]);
+                if (channel && x == P.W - 2)
+                    for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx + 1] = h16_store(fmt, q, g[q]);
+            } else {
+                for (int q = 0; q < 9; ++q) {
+                    if (inlet && CX[q] == -1) continue;
+                    long d = nbr_f32(&P, x, y, CX[q], CY[q]);
+                    if (mask[d])
+                        f[OPP[q] * plane + idx] = h16_store(fmt, q, g[q]);
+                    else
+                        f[q * plane + d] = h16_store(fmt, q, g[q]);
+                }
+                if (channel && x == P.W - 2) {
+                    for (int q = 0; q < 9; ++q) {
+                        if (CX[q] != -1) continue;
+                        long d = (long)(y + CY[q]) * P.W + x;
+                        if (!mask[d]) f[q * plane + d] = h16_store(fmt, q, g[q]);
+                    }
+                }
+            }
+            if (orho) {
+                orho[idx] = mac[0];
+                oux[idx] = mac[1];
+                ouy[idx] = mac[2];
+            }
+        }
+    }
 }
 
 int oracle_abi_version(void) { return 1; }

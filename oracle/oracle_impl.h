@@ -164,28 +164,28 @@ static void FN(collide_mrt)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin
     m6 = FMA(-P->s_q, m6 + m5, m6);
     m7 = FMA(-w, n7, m7);
     m8 = FMA(-w, n8, m8);
-    /* inverse, lbm_core4.cu:187-199, of the deviation moments (rho - 1, m1d, m2d, m3, ..., m8) */
+    /* inverse, lbm_core4.cu:187-199, of the deviation moments (rho - 1, m1d, m2d, m3, ..., m8), with the
+       common 1/36 folded into the coefficients: 6/36 = 1/6, 3/36 = 1/12, 9/36 = 1/4 */
     REAL t0 = (drc - m1d) + m2d;
     REAL t1 = FMA((REAL)-2, m2d, FMA((REAL)4, drc, -m1d));
     REAL t2 = FMA((REAL)4, drc, FMA((REAL)2, m1d, m2d));
-    REAL A = (m3 - m4) * (REAL)6;
-    REAL B = (m5 - m6) * (REAL)6;
-    REAL Pp = m7 * (REAL)9;
-    REAL X = FMA((REAL)6, m3, m4 * (REAL)3);
-    REAL Y = FMA((REAL)6, m5, m6 * (REAL)3);
-    REAL Z = m8 * (REAL)9;
-    REAL t1p = t1 + Pp, t1m = t1 - Pp;
-    REAL u = t2 + Z, v = t2 - Z;
+    REAL q7 = m7 * (REAL)0.25;
+    REAL q8 = m8 * (REAL)0.25;
+    REAL t1p = FMA(t1, C_INV36, q7), t1m = FMA(t1, C_INV36, -q7);
+    REAL d34 = m3 - m4, d56 = m5 - m6;
+    REAL X = FMA(C_1_6, m3, m4 * C_1_12);
+    REAL Y = FMA(C_1_6, m5, m6 * C_1_12);
+    REAL u = FMA(t2, C_INV36, q8), v = FMA(t2, C_INV36, -q8);
     REAL XpY = X + Y, XmY = X - Y;
     h[0] = t0 * C_INV9;
-    h[1] = (t1p + A) * C_INV36;
-    h[2] = (t1m + B) * C_INV36;
-    h[3] = (t1p - A) * C_INV36;
-    h[4] = (t1m - B) * C_INV36;
-    h[5] = (u + XpY) * C_INV36;
-    h[6] = (v - XmY) * C_INV36;
-    h[7] = (u - XpY) * C_INV36;
-    h[8] = (v + XmY) * C_INV36;
+    h[1] = FMA(d34, C_1_6, t1p);
+    h[2] = FMA(d56, C_1_6, t1m);
+    h[3] = FMA(d34, -C_1_6, t1p);
+    h[4] = FMA(d56, -C_1_6, t1m);
+    h[5] = u + XpY;
+    h[6] = v - XmY;
+    h[7] = u - XpY;
+    h[8] = v + XmY;
 }
 
 /* BGK of lbm_core_smemaa.cu:194-212 on deviations: heq_q = w_q (drho + rho (3cu + 4.5cu^2 - 1.5u^2)) */
