@@ -29,7 +29,7 @@ def _cache_path():
 def _key(sim, k):
     p = sim.params
     dev = torch.cuda.get_device_name(sim.device)
-    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|k{k}"
+    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|fmt{sim.fmt}|k{k}"
 
 
 def load_cache():
@@ -62,11 +62,14 @@ def time_config(sim, k, cfg, reps=6, bufs=None):
     par = sim.parity
 
     def launch(src, dst, parity):
-        rc = L.lbm_step_fused_cfg(ctypes.byref(sim.params), ctypes.c_void_p(sim.mask.data_ptr()),
-                                  ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
-                                  ctypes.c_void_p(sim.inlet_u.data_ptr()), parity, k, ctypes.byref(c), None, None,
-                                  None, st)
-        _lib.check(rc, "lbm_step_fused_cfg")
+        ptrs = (ctypes.c_void_p(sim.mask.data_ptr()), ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()))
+        if sim.fmt == _lib.FMT_F32:
+            rc = L.lbm_step_fused_cfg(ctypes.byref(sim.params), *ptrs, ctypes.c_void_p(sim.inlet_u.data_ptr()), parity,
+                                      k, ctypes.byref(c), None, None, None, st)
+        else:
+            rc = L.lbm_step_fused_h16(ctypes.byref(sim.params), *ptrs, sim.fmt, ctypes.c_void_p(sim.inlet_u.data_ptr()),
+                                      parity, k, ctypes.byref(c), None, None, None, st)
+        _lib.check(rc, "lbm_step_fused")
 
     launch(a, b, par)  # warm-up
     launch(b, a, (par + k) & 1)
@@ -83,7 +86,7 @@ def time_config(sim, k, cfg, reps=6, bufs=None):
 def candidates(sim, k):
     H = sim.params.H
     out = []
-    for strip in STRIPS:
+    for strip in (STRIPS if sim.fmt == _lib.FMT_F32 else (128,)):
         if strip - 2 * (k + 1) < 16:
             continue
         for ch in CHUNKS:
