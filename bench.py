@@ -165,7 +165,7 @@ def main():
         n2 = args.l2_steps
         p = sim.params
         op = O.params(W, H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc, clamp=p.clamp)
-        f64 = np.ascontiguousarray(case.f0, dtype=np.float64)
+        f64 = np.array(case.f0, dtype=np.float64, copy=True)  # the oracle advances it in place
         prof64 = case.inlet_u.astype(np.float64)
         t1 = time.perf_counter()
         par = O.aa_steps(op, sim.mask_host, f64, prof64, 0, n2)

@@ -87,43 +87,52 @@ __global__ __launch_bounds__((Sched<K, P0, LW>::NT)) void k_fused(KPar P, const 
     const int yend = y0 + ch + HALO - 1;     // last row needed at level 0
     const int niter = ch + HALO + S::DOUT;
 
+    // ---- per-thread load and write-back slots: fixed for the whole sweep, so the divisions and the
+    //      periodic column wrap are done once here rather than per row -------------------------------
+    int lq[S::NLOAD], lc[S::NLOAD], lx[S::NLOAD]; // plane, strip column, global column (-1: outside)
+#pragma unroll
+    for (int n = 0; n < S::NLOAD; ++n) {
+        const int e = t + n * NT;
+        lq[n] = e / LW;
+        lc[n] = e - (e / LW) * LW;
+        int x = xs + lc[n];
+        if (PERIODIC) x = ((x % W) + W) % W;
+        lx[n] = (e < 9 * LW && x >= 0 && x < W) ? x : -1;
+    }
+    int fx = xs + t; // column whose flags this thread builds (t < LW)
+    if (PERIODIC) fx = ((fx % W) + W) % W;
+    constexpr int NOUT = (9 * CW + NT - 1) / NT;
+    int oq[NOUT], oc[NOUT];
+#pragma unroll
+    for (int n = 0; n < NOUT; ++n) {
+        const int e = t + n * NT;
+        oq[n] = e < 9 * CW ? e / CW : -1;
+        oc[n] = e - (e / CW) * CW;
+    }
+
     // ---- level-0 rows travel HBM -> registers (PF rows ahead) -> LDS ring ------------------------
     T pre[PF][S::NLOAD];
     uint8_t pfl[PF];
     auto issue_load = [&](int j, T(&buf)[S::NLOAD], uint8_t &preflag) { // global row ybeg + j
         const int y = ybeg + j;
-        bool rowin = PERIODIC ? true : (y >= 0 && y < H);
-        const int gy = PERIODIC ? ((y % H) + H) % H : y;
+        const bool rowin = PERIODIC ? true : (y >= 0 && y < H);
+        const int gy = PERIODIC ? ((y % H) + H) % H : y; // uniform
 #pragma unroll
         for (int n = 0; n < S::NLOAD; ++n) {
-            const int e = t + n * NT;
             buf[n] = T(0);
-            if (e < 9 * LW && rowin && j <= yend - ybeg) {
-                const int q = e / LW, c = e - q * LW;
-                int x = xs + c;
-                bool colin = true;
-                if (PERIODIC) {
-                    x = ((x % W) + W) % W;
-                } else {
-                    colin = x >= 0 && x < W;
-                }
-                if (colin) buf[n] = fin[q * plane + gy * W + x];
-            }
+            if (rowin && lx[n] >= 0) buf[n] = fin[lq[n] * plane + gy * W + lx[n]];
         }
         if (t < LW) {
             uint8_t fl = 0;
-            int x = xs + t;
             if (PERIODIC) {
-                x = ((x % W) + W) % W;
-                if (mask[gy * W + x]) fl = F_SOLID;
-                else fl = F_COMPUTE;
-            } else if (rowin && x >= 0 && x < W) {
-                const bool solid = mask[gy * W + x] != 0;
+                fl = mask[gy * W + fx] ? F_SOLID : F_COMPUTE;
+            } else if (rowin && fx >= 0 && fx < W) {
+                const bool solid = mask[gy * W + fx] != 0;
                 if (solid) fl |= F_SOLID;
-                if (gy >= 1 && gy <= H - 2 && x <= W - 2) {
-                    if (x == 0) fl |= F_COMPUTE | F_INLET;
+                if (gy >= 1 && gy <= H - 2 && fx <= W - 2) {
+                    if (fx == 0) fl |= F_COMPUTE | F_INLET;
                     else if (!solid) fl |= F_COMPUTE;
-                    if (x == W - 2) fl |= F_OUTADJ;
+                    if (fx == W - 2) fl |= F_OUTADJ;
                 }
             }
             preflag = fl;
@@ -132,13 +141,8 @@ __global__ __launch_bounds__((Sched<K, P0, LW>::NT)) void k_fused(KPar P, const 
     auto commit_load = [&](int j, const T(&buf)[S::NLOAD], uint8_t preflag) {
         const int slot = j % R;
 #pragma unroll
-        for (int n = 0; n < S::NLOAD; ++n) {
-            const int e = t + n * NT;
-            if (e < 9 * LW) {
-                const int q = e / LW, c = e - q * LW;
-                sf[q][slot][c] = buf[n];
-            }
-        }
+        for (int n = 0; n < S::NLOAD; ++n)
+            if (t + n * NT < 9 * LW) sf[lq[n]][slot][lc[n]] = buf[n];
         if (t < LW) sflag[slot][t] = preflag;
     };
 
@@ -159,7 +163,9 @@ __global__ __launch_bounds__((Sched<K, P0, LW>::NT)) void k_fused(KPar P, const 
         if (i + PF < nload) issue_load(i + PF, pre[u], pfl[u]);
 
         // (3) one cell of one step per thread
-        const int s = t / LW, c = t - (t / LW) * LW;
+        // s is the same for the 64 lanes of a wave (LW is a multiple of 64): keep the step's row, ring slots
+        // and margins in scalar registers
+        const int s = __builtin_amdgcn_readfirstlane(t / LW), c = t - (t / LW) * LW;
         if (s < K) {
             const int jrow = i - S::d(s);                   // ring index of the row this step works on
             const int m = S::margin(s);
@@ -240,8 +246,10 @@ __global__ __launch_bounds__((Sched<K, P0, LW>::NT)) void k_fused(KPar P, const 
         if (jout >= HALO && jout < HALO + ch) {
             const int sl = jout % R;
             const int y = ybeg + jout;
-            for (int e = t; e < 9 * CW; e += NT) {
-                const int q = e / CW, cc = e - q * CW;
+#pragma unroll
+            for (int n = 0; n < NOUT; ++n) {
+                const int q = oq[n], cc = oc[n];
+                if (q < 0) continue;
                 const int x = xs + HALO + cc;
                 if (x >= W) continue;
                 // The outlet ghost W-1 is part of cell W-2's state (its post-collision copy after an even
