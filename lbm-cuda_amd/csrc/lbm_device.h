@@ -3,8 +3,8 @@
 // One __device__ collide function is used by the single-step, pull and temporally blocked kernels, so
 // that k fused steps are bit-identical to k single steps.  The code is compiled with -ffp-contract=off:
 // every fused multiply-add is written out (__builtin_fmaf), divisions are the correctly rounded __fdiv_rn
-// and square roots sqrtf (which hipcc expands to a correctly rounded sequence on gfx950; __fsqrt_rn lowers
-// to the bare v_sqrt_f32, which is not), and nothing is reassociated.
+// and square roots are correctly rounded (sqrt_rn_normal; note that __fsqrt_rn lowers to the bare
+// v_sqrt_f32, which is not), and nothing is reassociated.
 //
 // Physics follows lbm_core4.cu:29-210 (MRT + Smagorinsky, Zou-He inlet, clamps) and
 // lbm_core_smemaa.cu:159-220 (BGK); the operation order is this project's own (fewer VALU operations:
@@ -45,6 +45,20 @@ struct KPar {
 #define LBM_C_1_12 0.0833333333f
 
 __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Correctly rounded square root for positive normal x (here x >= 1e-8): the hardware estimate (within 1 ulp)
+// is moved to the neighbour whose residual x - s*s has the right sign -- the same correction hipcc emits for
+// sqrtf, without the rescaling and special-value paths that only denormal, zero, infinite or NaN inputs need.
+__device__ __forceinline__ float sqrt_rn_normal(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sdn = __builtin_bit_cast(float, __builtin_bit_cast(int, s) - 1);
+    const float sup = __builtin_bit_cast(float, __builtin_bit_cast(int, s) + 1);
+    const float rdn = fma_(-sdn, s, x);
+    const float rup = fma_(-sup, s, x);
+    const float t = rdn <= 0.0f ? sdn : s;
+    return rup > 0.0f ? sup : t;
+}
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
 
 // The collision works on deviations h_q = f_q - w_q from the rest state (rho = 1, u = 0).  The rest
@@ -102,7 +116,9 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
         }
         rc = 1.0f + drc;
         inv_rc = inv_rho; // 1/rc == 1/rho bit for bit unless the density clamp engaged
-        if (drc != drho) inv_rc = __fdiv_rn(1.0f, rc);
+        if (__any(drc != drho)) { // a wave-uniform branch, so the division is not evaluated speculatively
+            if (drc != drho) inv_rc = __fdiv_rn(1.0f, rc);
+        }
         les = P.les != 0;
     }
     mac[0] = rc;
@@ -131,9 +147,9 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
     float w = P.tau_inv;
     if (les) {
         // Smagorinsky (lbm_core4.cu:172-177)
-        const float Q = sqrtf(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
+        const float Q = sqrt_rn_normal(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
         const float d = fma_(P.c18cs2, Q * inv_rc, P.tau0sq);
-        const float tau_eff = 0.5f * (P.tau0 + sqrtf(d));
+        const float tau_eff = 0.5f * (P.tau0 + sqrt_rn_normal(d));
         w = __fdiv_rn(1.0f, tau_eff);
     }
     // relaxation (lbm_core4.cu:180-185)

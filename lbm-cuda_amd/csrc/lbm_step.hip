@@ -138,10 +138,10 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__
             const int o = opp(q);
             if (!PERIODIC && inlet && cx(q) > 0) {
                 g[q] = 0.0f - wq(q); // arrives from x = -1; rebuilt by Zou-He
+            } else if (solid[o]) {
+                g[q] = S::ld(rf, idx * B, (unsigned)q * plane * B, q); // bounce-back: this cell's slot q
             } else {
-                // bounce-back from a solid source reads this cell's slot q, else the source's slot opp(q)
-                const unsigned e = solid[o] ? (unsigned)q * plane + idx : (unsigned)o * plane + nb[o];
-                g[q] = S::ld(rf, e * B, 0, q);
+                g[q] = S::ld(rf, nb[o] * B, (unsigned)o * plane * B, q); // the source's slot opp(q)
             }
         }
     }
@@ -158,8 +158,10 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             if (!PERIODIC && inlet && cx(q) < 0) continue; // leaves through the open inlet
-            const unsigned e = solid[q] ? (unsigned)opp(q) * plane + idx : (unsigned)q * plane + nb[q];
-            S::st(rf, e * B, 0, q, g[q]);
+            if (solid[q])
+                S::st(rf, idx * B, (unsigned)opp(q) * plane * B, q, g[q]); // bounce-back into this cell
+            else
+                S::st(rf, nb[q] * B, (unsigned)q * plane * B, q, g[q]);
         }
         if (!PERIODIC && x == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
