@@ -1,6 +1,7 @@
 """D2Q9 throughput on one MI355X: GLUPS at 4096 x 1024 (plain FP32 vs chosen form), % HBM roofline, L2 error.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--k 8] [--cpu-seconds 10] [--l2-steps 20]
+                    [--ensemble-steps 200]
 
 Workload: channel flow past a cylinder, 4096 x 1024, R = 64 at (1024, 512), Re = 1000, Cs = 0.15,
 parabolic inlet u_max = 0.05 (synthetic initial state: feq at rho = 1, u = inlet profile, seeded noise).
@@ -8,7 +9,8 @@ A "step" is one collide-stream update of all W*H cells with the state resident i
 steps bracketed by torch.cuda.synchronize(); HIP events on the launch stream give the per-launch kernel
 time used for the roofline (72 B/cell/step algorithmic traffic for FP32 single steps).
 The CPU baseline is the oracle's FP32 mirror (C, OpenMP over rows) on a bounded number of steps of the
-same lattice, timed on this host.
+same lattice, timed on this host.  The "ensemble" object reports the batched workload of 64 periodic
+lattices of uneven sizes (one persistent launch per step vs 64 separate launches per step).
 """
 from __future__ import annotations
 
@@ -35,6 +37,88 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def bench_ensemble(nsteps, graph_steps=20):
+    """64 periodic lattices of uneven sizes: one persistent launch per step per cache group, replayed from a
+    captured graph of `graph_steps` steps, vs the same steps as 64 separate launches per step (captured
+    step-major and lattice-major, and issued eagerly)."""
+    import torch
+
+    from lbm import configs
+    from lbm.ensemble import Ensemble, Graph, tune_ensemble
+
+    members = configs.ensemble()
+    t0 = time.perf_counter()
+    ens = Ensemble.from_members(members)
+    cells = ens.cells
+    log(f"ensemble: {len(members)} lattices, {cells / 1e6:.1f} M cells, built in {time.perf_counter() - t0:.1f} s")
+    blocks, group, tuned_us = tune_ensemble(ens, use_cache=False, steps=graph_steps, reps=2)
+    log(f"ensemble tuned: {blocks} workgroups, groups of <= {group >> 20} MB ({tuned_us:.1f} us/step)")
+    reps = max(1, nsteps // graph_steps)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        fn()  # warm-up replay
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        n = reps * graph_steps
+        return e0.elapsed_time(e1) * 1e3 / n, (time.perf_counter() - t) * 1e6 / n
+
+    def rate(us):
+        return round(cells / us / 1e3, 3)
+
+    g = ens.capture(graph_steps)
+    us_p, _ = timed(lambda: ens.replay(g))
+    ens.group_bytes = 0
+    g1 = ens.capture(graph_steps)
+    us_1, _ = timed(lambda: ens.replay(g1))
+    ens.group_bytes = group
+    gs = Graph()
+    with gs.capture():
+        for _ in range(graph_steps):
+            for m in ens.members:
+                m.step(1)
+    us_s, _ = timed(gs.replay)
+    gl = Graph()
+    with gl.capture():
+        for m in ens.members:
+            m.step(graph_steps)
+    us_l, _ = timed(gl.replay)
+    us_e, wall_e = timed(lambda: ens.step_separate(graph_steps))
+    for gr in (g, g1, gs, gl):
+        gr.close()
+    res = {
+        "workload": f"ensemble{len(members)}_periodic_mrt_les",
+        "members": len(members), "cells": cells, "steps_timed": reps * graph_steps,
+        "steps_per_graph": graph_steps,
+        "sizes": "W, H each from {256,384,512,768,1024,1536,2048}; nu log-uniform [1e-4, 1e-2]; seed 4",
+        "blocks": blocks, "group_mb": group >> 20,
+        "persistent_graph": {"glups": rate(us_p), "us_per_step": round(us_p, 2),
+                             "hbm_gbs_if_uncached": round(BYTES_PER_CELL_FP32 * cells / us_p / 1e3, 1)},
+        "persistent_graph_ungrouped": {"glups": rate(us_1), "us_per_step": round(us_1, 2),
+                                       "achieved_gbs": round(BYTES_PER_CELL_FP32 * cells / us_1 / 1e3, 1)},
+        "separate_graph_step_major": {"glups": rate(us_s), "us_per_step": round(us_s, 2),
+                                      "launches_per_step": len(members)},
+        "separate_graph_lattice_major": {"glups": rate(us_l), "us_per_step": round(us_l, 2),
+                                         "launches_per_step": len(members)},
+        "separate_eager": {"glups": rate(wall_e), "us_per_step_wall": round(wall_e, 2),
+                           "us_per_step_events": round(us_e, 2)},
+    }
+    best_sep = min(us_s, us_l)
+    res["speedup_vs_best_separate_graph"] = round(best_sep / us_p, 3)
+    res["speedup_vs_separate_eager"] = round(wall_e / us_p, 3)
+    log("ensemble GLUPS: " + ", ".join(f"{k} {v['glups']}" for k, v in res.items()
+                                       if isinstance(v, dict) and "glups" in v))
+    ens.close()
+    del ens
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,8 +128,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--l2-steps", type=int, default=200, help="steps for the L2 error against FP64 (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=("all", "aa", "fused"), default="all",
-                    help="run only the plain single step or only the fused form (for profiling)")
+    ap.add_argument("--only", choices=("all", "aa", "fused", "ensemble"), default="all",
+                    help="run only the plain single step, the fused form or the ensemble (for profiling)")
+    ap.add_argument("--ensemble-steps", type=int, default=200,
+                    help="timed steps of the 64-lattice batched ensemble (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy a device buffer of this many MiB (a known-size kernel for counter calibration)")
     args = ap.parse_args()
@@ -62,6 +148,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": bench_ensemble(max(args.ensemble_steps, 20))}), flush=True)
+        return
     if args.copy_probe:
         n = args.copy_probe * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
@@ -212,6 +301,8 @@ def main():
                "sample": f"oracle FP32 mirror (C, OpenMP), {n} AA steps of the full 4096x1024 lattice in {el:.1f} s"}
         log(f"CPU baseline: {cpu['value'] * 1e3:.1f} MLUPS on {cpu['cores']} threads")
 
+    ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 and args.only == "all" else None
+
     # the chosen form: the fastest one whose velocity L2 error against FP64 stays within 1e-3
     ok = {n: v for n, v in forms.items() if "glups" in v and v.get("l2_u", 0.0) <= 1e-3}
     chosen = max(ok, key=lambda n: ok[n]["glups"]) if ok else "aa_fp32"
@@ -240,6 +331,7 @@ def main():
         "forms": forms,
         "l2_err": l2,
         "drag_lift": drag,
+        "ensemble": ensemble,
     }
     print(json.dumps(out), flush=True)
 

@@ -18,7 +18,8 @@
  *   - AA parity: 0 = canonical (slot q at x holds the incoming f_q(x)); 1 = reversed (slot opp(q) at x
  *     holds the post-collision f*_q(x)).  This is the A/B convention of lbm_core_smemaa.cu:92-106,274-292
  *     applied to the HBM state.
- *   - Pointers are device addresses owned by the caller; nothing here allocates.  Calls are asynchronous
+ *   - Pointers are device addresses owned by the caller; only lbm_ensemble_create (its descriptor table)
+ *     and lbm_graph_end (the instantiated graph) allocate.  Calls are asynchronous
  *     on `stream` (NULL = default stream).  Every function returns 0 on success, otherwise a negative
  *     library code or a positive hipError_t, and lbm_last_error() describes the last failure of the
  *     calling thread.
@@ -190,6 +191,51 @@ int lbm_visualize(const lbm_params *p, const float *ux, const float *uy, const u
 size_t lbm_diagnostics_work_size(const lbm_params *p);
 int lbm_diagnostics(const lbm_params *p, const uint8_t *mask, const float *f, const float *inlet_u, int parity,
                     double *work, double *out, lbm_stream_t stream);
+
+/*
+ * Batched ensembles (no reference counterpart: the reference advances one lattice per process,
+ * lbm_smemaa_main.py:8-10).  An ensemble is a set of independent lattices of any sizes that share model,
+ * bc and storage format; lbm_ensemble_step advances every member by one in-place AA step (the update of
+ * lbm_step_aa / lbm_step_aa_h16, bit for bit) in ONE persistent launch whose workgroups walk a tile list
+ * spanning all members.  The instance table is copied at creation; the buffers it names must stay alive
+ * and in place while the ensemble exists.  All members share one parity: step them together.
+ * `blocks` <= 0 picks the default grid (as many workgroups as are resident at once); other values are
+ * rounded up to a multiple of 8.
+ */
+typedef struct lbm_instance {
+    lbm_params p;
+    const uint8_t *mask; /* W*H bytes */
+    void *f;             /* 9*W*H FP32 populations, or 16-bit deviations for LBM_FMT_F16 / LBM_FMT_BF16 */
+    const float *inlet_u; /* H floats (channel bc; may be NULL when periodic) */
+} lbm_instance;
+typedef struct lbm_ensemble lbm_ensemble;
+int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble **out);
+/* Tiles of 128 x 2 cells over all members, cells over all members, and the default grid. */
+int lbm_ensemble_info(const lbm_ensemble *e, int *total_tiles, long long *total_cells, int *default_blocks);
+int lbm_ensemble_step(lbm_ensemble *e, int parity, int blocks, lbm_stream_t stream);
+/*
+ * nsteps steps of every member, grouped for the last-level cache: members are taken in order in groups
+ * whose states total at most group_bytes (a larger member is a group of its own), and each group is
+ * advanced all nsteps steps (one launch per step) before the next group starts.  Members are independent,
+ * so every member ends bit-identical to nsteps calls of lbm_ensemble_step; a group that stays resident in
+ * the 256 MB Infinity Cache is read from HBM once instead of nsteps times.  group_bytes <= 0: one group.
+ */
+int lbm_ensemble_step_n(lbm_ensemble *e, int parity, int nsteps, long long group_bytes, int blocks,
+                        lbm_stream_t stream);
+int lbm_ensemble_destroy(lbm_ensemble *e);
+
+/*
+ * Stream-capture graphs: the C-ABI form of the reference driver's CuPy capture of two fused launches
+ * (lbm_smemaa_main.py:136-151, replayed at :173-174).  lbm_graph_begin starts capturing `stream` (not the default
+ * stream); every liblbm call issued on it until lbm_graph_end is recorded, not run.  lbm_graph_launch
+ * replays the recorded sequence on any stream.  No liblbm entry point records a memset, so the graph
+ * may be replayed any number of times.
+ */
+typedef struct lbm_graph lbm_graph;
+int lbm_graph_begin(lbm_stream_t stream);
+int lbm_graph_end(lbm_stream_t stream, lbm_graph **out);
+int lbm_graph_launch(lbm_graph *g, lbm_stream_t stream);
+int lbm_graph_destroy(lbm_graph *g);
 
 #ifdef __cplusplus
 }

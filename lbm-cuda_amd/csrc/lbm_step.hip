@@ -10,6 +10,7 @@
 // Odd (B) steps read and write the same 9 locations per cell (x - c_q / x + c_q pairs), which is what
 // makes the single-buffer update race free.
 #include "lbm_common.h"
+#include "lbm_aa_cell.h"
 
 #include <cstring>
 
@@ -89,92 +90,10 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__
                                                         const float *__restrict__ inlet_u, float *__restrict__ oux,
                                                         float *__restrict__ ouy, float *__restrict__ orho)
 {
-    using S = Store<FMT>;
-    constexpr unsigned B = S::BYTES;
     int bx, by;
     xcd_tile(bx, by);
-    const int x = bx * TILE_X + threadIdx.x;
-    const int y = by * TILE_Y + threadIdx.y;
-    if (x >= P.W || y >= P.H) return;
-    const int W = P.W;
-    const unsigned plane = (unsigned)P.plane;
-    const unsigned idx = (unsigned)(y * W + x);
-    const rsrc_t rf = make_rsrc(f, 9u * plane * B);
-    const rsrc_t rm = make_rsrc(mask, plane);
-    auto solid_at = [&](unsigned cell) { return __builtin_amdgcn_raw_buffer_load_b8(rm, cell, 0, 0) != 0; };
-    bool inlet = false;
-    if constexpr (!PERIODIC) {
-        // wall rows and the outlet ghost are never updated (lbm_core4.cu:20-24)
-        if (y == 0 || y == P.H - 1 || x == W - 1) return;
-        inlet = x == 0;
-        if (!inlet && solid_at(idx)) return;
-    } else {
-        if (solid_at(idx)) return;
-    }
-    float g[9], mac[3];
-    unsigned nb[9];
-    bool solid[9];
-    if constexpr (PARITY == 0) {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) g[q] = S::ld(rf, idx * B, (unsigned)q * plane * B, q);
-    } else {
-        nb[0] = idx;
-        solid[0] = false;
-#pragma unroll
-        for (int q = 1; q < 9; ++q) {
-            if constexpr (PERIODIC) {
-                int nx = x + cx(q), ny = y + cy(q);
-                nx = nx < 0 ? nx + W : (nx >= W ? nx - W : nx);
-                ny = ny < 0 ? ny + P.H : (ny >= P.H ? ny - P.H : ny);
-                nb[q] = (unsigned)(ny * W + nx);
-            } else {
-                nb[q] = idx + (unsigned)(cx(q) + cy(q) * W); // x = -1 only at the inlet, never used there
-            }
-            solid[q] = (!PERIODIC && inlet && cx(q) < 0) ? false : solid_at(nb[q]);
-        }
-        g[0] = S::ld(rf, idx * B, 0, 0);
-#pragma unroll
-        for (int q = 1; q < 9; ++q) {
-            const int o = opp(q);
-            if (!PERIODIC && inlet && cx(q) > 0) {
-                g[q] = 0.0f - wq(q); // arrives from x = -1; rebuilt by Zou-He
-            } else if (solid[o]) {
-                g[q] = S::ld(rf, idx * B, (unsigned)q * plane * B, q); // bounce-back: this cell's slot q
-            } else {
-                g[q] = S::ld(rf, nb[o] * B, (unsigned)o * plane * B, q); // the source's slot opp(q)
-            }
-        }
-    }
-    collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
-    if constexpr (PARITY == 0) {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) S::st(rf, idx * B, (unsigned)opp(q) * plane * B, q, g[q]);
-        if (!PERIODIC && x == W - 2) { // reversed copy into the outlet ghost (lbm_core4.cu:114-116)
-#pragma unroll
-            for (int q = 0; q < 9; ++q) S::st(rf, (idx + 1) * B, (unsigned)opp(q) * plane * B, q, g[q]);
-        }
-    } else {
-        S::st(rf, idx * B, 0, 0, g[0]);
-#pragma unroll
-        for (int q = 1; q < 9; ++q) {
-            if (!PERIODIC && inlet && cx(q) < 0) continue; // leaves through the open inlet
-            if (solid[q])
-                S::st(rf, idx * B, (unsigned)opp(q) * plane * B, q, g[q]); // bounce-back into this cell
-            else
-                S::st(rf, nb[q] * B, (unsigned)q * plane * B, q, g[q]);
-        }
-        if (!PERIODIC && x == W - 2) {
-            // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-            S::st(rf, idx * B, 3u * plane * B, 3, g[3]);
-            if (!solid[2]) S::st(rf, nb[2] * B, 6u * plane * B, 6, g[6]);
-            if (!solid[4]) S::st(rf, nb[4] * B, 7u * plane * B, 7, g[7]);
-        }
-    }
-    if constexpr (MACROS) {
-        orho[idx] = mac[0];
-        oux[idx] = mac[1];
-        ouy[idx] = mac[2];
-    }
+    aa_cell<MODEL, FMT, PARITY, PERIODIC, MACROS>(P, mask, f, inlet_u, oux, ouy, orho, bx * TILE_X + threadIdx.x,
+                                                  by * TILE_Y + threadIdx.y);
 }
 
 template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS>

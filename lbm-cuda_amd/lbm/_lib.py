@@ -71,6 +71,15 @@ SIGNATURES = {
     "lbm_visualize": (_i, [_P, _vp, _vp, _vp, _vp, _f, _vp]),
     "lbm_diagnostics_work_size": (ctypes.c_size_t, [_P]),
     "lbm_diagnostics": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "lbm_ensemble_create": (_i, [_vp, _i, _i, _vp]),
+    "lbm_ensemble_info": (_i, [_vp, _vp, _vp, _vp]),
+    "lbm_ensemble_step": (_i, [_vp, _i, _i, _vp]),
+    "lbm_ensemble_step_n": (_i, [_vp, _i, _i, ctypes.c_longlong, _i, _vp]),
+    "lbm_ensemble_destroy": (_i, [_vp]),
+    "lbm_graph_begin": (_i, [_vp]),
+    "lbm_graph_end": (_i, [_vp, _vp]),
+    "lbm_graph_launch": (_i, [_vp, _vp]),
+    "lbm_graph_destroy": (_i, [_vp]),
 }
 
 _lib = None

@@ -18,7 +18,8 @@ PKG_ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(PKG_ROOT, "csrc")
 LIB_PATH = os.path.join(HERE, "liblbm.so")
 
-SOURCES = ["lbm_step.hip", "lbm_fused.hip", "lbm_fused_f32.hip", "lbm_fused_f16.hip", "lbm_fused_bf16.hip"]
+SOURCES = ["lbm_step.hip", "lbm_fused.hip", "lbm_fused_f32.hip", "lbm_fused_f16.hip", "lbm_fused_bf16.hip",
+           "lbm_ensemble.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -7,6 +7,9 @@ here the disc is analytic):
 Both: wall rows, parabolic velocity inlet with u_max = 0.05, zero-gradient outlet, MRT + Smagorinsky,
 rho = 1 and ux = inlet profile everywhere at t = 0, uy ~ U(-1e-4, 1e-4); Re = u_max D / nu with D = 2R.
 Periodic double shear layer 8192 x 2048, nu = 1e-4, MRT without LES, seed 3 (precision study).
+Ensemble of 64 periodic lattices, W and H each drawn uniformly from {256, 384, 512, 768, 1024, 1536, 2048},
+nu log-uniform in [1e-4, 1e-2], random-phase solenoidal field E(k) ~ k^4 exp(-2 (k/8)^2) at u_rms = 0.04,
+MRT + Smagorinsky Cs = 0.1, rho = 1, seed 4 (batched-ensemble study).
 """
 from __future__ import annotations
 
@@ -66,3 +69,38 @@ def shear_layer(W=8192, H=2048, nu=1e-4, steps=20000, seed=3):
     tau_inv = float(np.float32(geo.nu_tau_inv(nu)))
     return Case(f"shear_layer_{W}x{H}", W, H, tau_inv, 0.0, "mrt", "periodic", geo.periodic_mask(W, H),
                 np.zeros(H), f0, steps, dict(nu=nu))
+
+
+ENSEMBLE_SIZES = (256, 384, 512, 768, 1024, 1536, 2048)
+
+
+@dataclass
+class Member:
+    """One lattice of an ensemble: size, rate, and a builder of its initial populations (built on demand,
+    so that a 64-member ensemble never holds every FP64 state on the host at once)."""
+    W: int
+    H: int
+    nu: float
+    tau_inv: float
+    cs: float
+    seed: int
+    k0: float = 8.0
+    u_rms: float = 0.04
+
+    def f0(self, dtype=np.float64):
+        """Equilibrium populations at rho = 1 and the member's velocity field, computed in FP64."""
+        ux, uy = geo.random_phase_velocity(self.W, self.H, k0=self.k0, u_rms=self.u_rms, seed=self.seed)
+        return geo.feq(np.ones((self.H, self.W)), ux, uy).astype(dtype, copy=False)
+
+
+def ensemble(n=64, sizes=ENSEMBLE_SIZES, nu_range=(1e-4, 1e-2), cs=0.1, seed=4):
+    """Members of the batched-ensemble workload (periodic, MRT + Smagorinsky)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        W = int(rng.choice(sizes))
+        H = int(rng.choice(sizes))
+        nu = float(np.exp(rng.uniform(np.log(nu_range[0]), np.log(nu_range[1]))))
+        out.append(Member(W, H, nu, float(np.float32(geo.nu_tau_inv(nu))), float(np.float32(cs)),
+                          seed=seed * 1000 + i))
+    return out

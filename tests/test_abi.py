@@ -57,3 +57,28 @@ def test_argument_errors_return_codes():
 
 def test_fused_max_k():
     assert lbm.load().lbm_fused_max_k() == 8
+
+
+def test_ensemble_and_graph_argument_errors():
+    """Ensemble / graph entry points reject bad arguments before touching the GPU."""
+    from lbm.ensemble import LbmInstance
+    L = lbm.load()
+    out = ctypes.c_void_p()
+    assert L.lbm_ensemble_create(None, 0, 0, ctypes.byref(out)) == -1
+    inst = (LbmInstance * 2)()
+    inst[0] = LbmInstance(_lib.params(64, 32, 1.5), 1, 1, 1)
+    inst[1] = LbmInstance(_lib.params(64, 32, 1.5), 0, 1, 1)  # NULL mask
+    assert L.lbm_ensemble_create(inst, 2, 0, ctypes.byref(out)) == -1
+    assert b"instance 1" in L.lbm_last_error()
+    inst[1] = LbmInstance(_lib.params(64, 32, 1.5, model=_lib.BGK), 1, 1, 1)
+    assert L.lbm_ensemble_create(inst, 2, 0, ctypes.byref(out)) == -1
+    assert b"differs" in L.lbm_last_error()
+    assert L.lbm_ensemble_create(inst, 1, 7, ctypes.byref(out)) == -1  # bad format
+    assert not out.value
+    assert L.lbm_ensemble_step(None, 0, 0, None) == -1
+    assert L.lbm_ensemble_info(None, None, None, None) == -1
+    assert L.lbm_ensemble_destroy(None) == 0
+    assert L.lbm_graph_begin(None) == -1  # the default stream cannot be captured
+    assert L.lbm_graph_launch(None, None) == -1
+    assert L.lbm_graph_end(None, None) == -1
+    assert L.lbm_graph_destroy(None) == 0

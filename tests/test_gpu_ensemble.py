@@ -1,0 +1,188 @@
+"""Batched ensembles and captured graphs on the GPU: every member of an ensemble stepped by the persistent
+kernel is bit-identical to the same lattice stepped on its own (lbm_step_aa / lbm_step_aa_h16), for
+uneven and ragged sizes, both boundary modes, both models, all storage formats and any grid; every replay
+of a captured graph equals the eager result."""
+import numpy as np
+import pytest
+import torch
+
+from lbm import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(64, 32), (70, 33), (129, 5), (256, 130), (384, 256), (128, 512), (3, 3), (200, 64), (512, 96)]
+
+
+def _periodic_pair(sizes, storage="f32", model="mrt", cs=0.1, seed=11):
+    from lbm.sim import Simulation
+    a, b = [], []
+    rng = np.random.default_rng(seed)
+    for i, (W, H) in enumerate(sizes):
+        ux, uy = G.random_phase_velocity(W, H, k0=min(8, max(1, min(W, H) // 4)), u_rms=0.04, seed=seed + i)
+        f0 = G.feq(np.ones((H, W)), ux, uy)
+        mask = G.periodic_mask(W, H)
+        if W > 16 and H > 16:
+            mask[H // 3:H // 3 + 5, W // 2:W // 2 + 4] = 1
+        tau_inv = float(np.float32(G.nu_tau_inv(float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-2)))))))
+        kw = dict(cs=cs if model == "mrt" else 0.0, model=model, bc="periodic", mask=mask, f0=f0, storage=storage)
+        a.append(Simulation(W, H, tau_inv, **kw))
+        b.append(Simulation(W, H, tau_inv, **kw))
+    return a, b
+
+
+def _channel_pair(sizes, storage="f32", model="mrt", seed=21):
+    from lbm.sim import Simulation
+    a, b = [], []
+    for i, (W, H) in enumerate(sizes):
+        W, H = max(W, 8), max(H, 6)
+        R = max(1, min(W, H) // 8)
+        mask = G.channel_mask(W, H, cylinder=(W // 4, H // 2, R))
+        prof = G.parabolic_profile(H, 0.05)
+        f0 = G.channel_initial_state(W, H, prof, noise=1e-3, seed=seed + i)
+        kw = dict(cs=0.12 if model == "mrt" else 0.0, model=model, bc="channel", mask=mask, inlet_u=prof, f0=f0,
+                  storage=storage)
+        a.append(Simulation(W, H, 1.7 - 0.01 * i, **kw))
+        b.append(Simulation(W, H, 1.7 - 0.01 * i, **kw))
+    return a, b
+
+
+def _assert_same(ens_sims, solo_sims, what):
+    torch.cuda.synchronize()
+    for i, (s, r) in enumerate(zip(ens_sims, solo_sims)):
+        assert s.parity == r.parity, f"{what}: member {i} parity"
+        x, y = s.f.cpu().numpy(), r.f.cpu().numpy()
+        bad = np.count_nonzero(x.view(np.uint8) != y.view(np.uint8))
+        assert bad == 0, f"{what}: member {i} ({s.W}x{s.H}) differs in {bad} bytes"
+
+
+def _solo_steps(sims, n):
+    for s in sims:
+        s.step(n)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_ensemble_periodic_bitwise(gpu, storage):
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES, storage=storage)
+    ens = Ensemble(a)
+    assert ens.cells == sum(W * H for W, H in SIZES)
+    for n in (1, 2, 3):
+        ens.step(n)
+        _solo_steps(b, n)
+        _assert_same(a, b, f"periodic {storage} after +{n}")
+    ens.step_single(3)  # one lbm_ensemble_step launch per step
+    _solo_steps(b, 3)
+    _assert_same(a, b, f"periodic {storage} single launches")
+
+
+@pytest.mark.parametrize("group_kb", [1, 300, 2000])
+def test_ensemble_grouped_bitwise(gpu, group_kb):
+    """Cache grouping (members advanced several steps per group) leaves every member bit-identical:
+    1 KB puts each member in a group of its own, 300 KB packs a few, 2000 KB packs all."""
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES, seed=13)
+    ens = Ensemble(a, group_bytes=group_kb << 10)
+    for n in (5, 2, 1):
+        ens.step(n)
+        _solo_steps(b, n)
+        _assert_same(a, b, f"group {group_kb} KB after +{n}")
+
+
+@pytest.mark.parametrize("model", ["mrt", "bgk"])
+def test_ensemble_channel_bitwise(gpu, model):
+    from lbm.ensemble import Ensemble
+    a, b = _channel_pair(SIZES, model=model)
+    ens = Ensemble(a)
+    ens.step(7)
+    _solo_steps(b, 7)
+    _assert_same(a, b, f"channel {model}")
+
+
+@pytest.mark.parametrize("blocks", [8, 24, 4096])
+def test_ensemble_any_grid(gpu, blocks):
+    """Small grids (many tiles per workgroup) and a grid larger than the tile list give the same state."""
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES[:5], seed=31)
+    ens = Ensemble(a, blocks=blocks)
+    ens.step(4)
+    _solo_steps(b, 4)
+    _assert_same(a, b, f"blocks {blocks}")
+
+
+def test_ensemble_single_member_and_solo_interleave(gpu):
+    """One member; and single steps of the members between ensemble steps keep everything consistent."""
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair([(256, 128), (128, 64)], seed=41)
+    ens = Ensemble(a)
+    ens.step(1)
+    _solo_steps(b, 1)
+    ens.members[0].step(1)  # a member advanced alone changes its parity: the ensemble must refuse to mix
+    with pytest.raises(ValueError):
+        Ensemble(a)
+    ens.members[1].step(1)
+    ens.parity ^= 1
+    _solo_steps(b, 1)
+    ens.step(2)
+    _solo_steps(b, 2)
+    _assert_same(a, b, "interleaved")
+    one, ref = _periodic_pair([(320, 200)], seed=43)
+    e1 = Ensemble(one)
+    e1.step(5)
+    _solo_steps(ref, 5)
+    _assert_same(one, ref, "single member")
+
+
+@pytest.mark.parametrize("group_kb", [0, 300])
+def test_ensemble_graph_replays(gpu, group_kb):
+    """A captured 4-step graph, replayed 3 times with eager ensemble steps and synchronisations between
+    the replays: every replay equals the eager result."""
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES, seed=51)
+    ens = Ensemble(a, group_bytes=group_kb << 10)
+    g = ens.capture(4)
+    _assert_same(a, b, "capture records, does not run")
+    for r in range(3):
+        ens.replay(g)
+        _solo_steps(b, 4)
+        _assert_same(a, b, f"replay {r}")
+        ens.step(2)  # eager work between replays
+        _solo_steps(b, 2)
+        torch.cuda.synchronize()
+        _assert_same(a, b, f"eager after replay {r}")
+    g.close()
+
+
+def test_graph_of_single_lattice_steps(gpu):
+    """The generic capture API records any liblbm launches: two single steps of one lattice + macros."""
+    from lbm.ensemble import Graph
+    a, b = _channel_pair([(300, 80)], seed=61)
+    s, r = a[0], b[0]
+    g = Graph()
+    with g.capture():
+        s.step(1, macros=True)
+        s.step(1)
+    s.parity, s.steps = r.parity, 0  # capture did not run anything
+    _assert_same(a, b, "capture")
+    for rep in range(3):
+        g.replay()
+        r.step(1, macros=True)
+        r.step(1)
+        _assert_same(a, b, f"replay {rep}")
+        assert torch.equal(s.ux, r.ux) and torch.equal(s.rho, r.rho)
+    g.close()
+
+
+def test_ensemble_tuner(gpu, tmp_path, monkeypatch):
+    from lbm.ensemble import Ensemble, tune_ensemble
+    monkeypatch.setenv("LBM_TUNE_CACHE", str(tmp_path / "fused.json"))
+    a, b = _periodic_pair(SIZES[:6], seed=71)
+    ens = Ensemble(a)
+    best = tune_ensemble(ens, steps=4, reps=1)
+    assert best[0] % 8 == 0 and 8 <= best[0] <= ens.default_blocks and best[2] > 0
+    assert (ens.blocks, ens.group_bytes) == (best[0], best[1])
+    again = tune_ensemble(ens, steps=4, reps=1)  # from the cache
+    assert tuple(again) == tuple(best)
+    n = ens.members[0].steps
+    _solo_steps(b, n)
+    _assert_same(a, b, "after tuning")
+
