@@ -29,8 +29,12 @@ for _p in (ROOT, os.path.join(ROOT, "lbm-cuda_amd")):
 METRIC = "D2Q9 GLUPS at 4096x1024 (plain fp32 vs chosen form), % HBM roofline, L2 err"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_FP32 = 72  # 9 populations read + 9 written, 4 B each
-# HBM bytes per plain FP32 step from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (see profiles/); None if unmeasured
-TRAFFIC_BYTES_PER_STEP = None
+# Fabric-side bytes per launch of the single-step kernels at 4096 x 1024, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script (--only aa --copy-probe 144,2048), scaled by what the counters read for copies
+# of known size (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); mean of the even and odd launches.  The summary with the
+# command lines is profiles/k_aa_cylinder_4096x1024.md.
+TRAFFIC_BYTES_PER_STEP = {"f32": 304.95e6, "f16": 154.75e6, "bf16": 154.75e6}
+TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-calibrated, even/odd mean"
 
 
 def log(*a):
@@ -119,6 +123,38 @@ def bench_ensemble(nsteps, graph_steps=20):
     return res
 
 
+def updated_cells(sim):
+    """Cells a step updates (fluid, not wall rows, not the outlet ghost column)."""
+    upd = sim.mask_host == 0
+    if sim.bc == 0:
+        upd[0, :] = upd[-1, :] = False
+        upd[:, -1] = False
+    return upd
+
+
+def copy_ceiling(nbytes, dev, reps=20):
+    """Measured ceilings for moving `nbytes` in and `nbytes` out: an out-of-place copy (src != dst, twice the
+    footprint) and an in-place read-modify-write of one buffer (the footprint of the in-place AA step)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {"bytes_each_way": nbytes}
+    for name, fn in (("copy", lambda: b.copy_(a)), ("inplace_rmw", lambda: a.mul_(1.0))):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        out[name + "_gbs"] = round(2 * nbytes / us / 1e3, 1)
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,8 +168,8 @@ def main():
                     help="run only the plain single step, the fused form or the ensemble (for profiling)")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="timed steps of the 64-lattice batched ensemble (0 = skip)")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="first copy a device buffer of this many MiB (a known-size kernel for counter calibration)")
+    ap.add_argument("--copy-probe", default="", metavar="MIB[,MIB...]",
+                    help="first copy device buffers of these many MiB (known-size kernels for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine runs one lattice per GPU and has no multi-GPU path "
@@ -151,13 +187,12 @@ def main():
     if args.only == "ensemble":
         print(json.dumps({"ensemble": bench_ensemble(max(args.ensemble_steps, 20))}), flush=True)
         return
-    if args.copy_probe:
-        n = args.copy_probe * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device=dev)
+    for mib in (int(v) for v in args.copy_probe.split(",") if v.strip()):
+        src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        log(f"copy probe: {mib} MiB read + {mib} MiB written")
         del src, dst
     case = configs.cylinder_large()
     W, H = case.W, case.H
@@ -192,6 +227,8 @@ def main():
     log(f"plain fp32 AA: {ms_per_step * 1e3:.1f} us/step wall, {kern_ms * 1e3:.1f} us/launch (events), "
         f"{glups:.2f} GLUPS, {achieved:.0f} GB/s")
     forms = {"aa_fp32": {"glups": glups, "us_per_step": kern_ms * 1e3, "achieved_gbs": achieved, "bytes_per_cell": 72}}
+    fluid = int(np.count_nonzero(updated_cells(sim)))
+    ceiling = copy_ceiling(BYTES_PER_CELL_FP32 // 2 * cells, dev) if args.only != "fused" else None
     cd, cl = sim.drag_lift(case.meta["u_ref"], case.meta["D"])
     drag = {"after_steps": sim.steps, "cd": cd, "cl": cl, "u_ref": case.meta["u_ref"], "D": case.meta["D"],
             "note": "momentum exchange on the cylinder, FP64 reduction on the GPU"}
@@ -222,7 +259,9 @@ def main():
             run16(args.warmup)
             us = time_steps(run16, args.steps)
             forms[f"aa_{storage}"] = {"glups": cells / us / 1e3, "us_per_step": us, "bytes_per_cell": 36,
-                                      "achieved_gbs": 36 * cells / us / 1e3}
+                                      "achieved_gbs": 36 * cells / us / 1e3,
+                                      "frac": 36 * cells / us / 1e3 / HBM_PEAK_GBS,
+                                      "traffic": TRAFFIC_BYTES_PER_STEP[storage]}
             runners[f"aa_{storage}"] = (storage, 1, None)
             log(f"{storage} AA: {us:.1f} us/step, {cells / us / 1e3:.2f} GLUPS")
             del s16
@@ -323,9 +362,13 @@ def main():
         "config": {"workload": case.name, "form": chosen, "W": W, "H": H, "re": 1000, "cs": 0.15,
                    "parallelism": "single_gpu"},
         "plain_fp32": {"glups": round(glups, 3), "us_per_step": round(kern_ms * 1e3, 2),
-                       "wall_ms_per_step": round(ms_per_step, 5)},
+                       "wall_ms_per_step": round(ms_per_step, 5), "fluid_glups": round(glups * fluid / cells, 3),
+                       "frac_of_copy": (round(achieved / ceiling["inplace_rmw_gbs"], 4) if ceiling else None)},
+        "copy_ceiling": ceiling,
+        "fluid_cells": fluid,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES_PER_STEP,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES_PER_STEP["f32"],
+                     "traffic_source": TRAFFIC_SOURCE, "algorithmic_bytes": BYTES_PER_CELL_FP32 * cells,
                      "kernel": "k_aa FP32 (plain form; even/odd average)", "bytes_per_cell": BYTES_PER_CELL_FP32},
         "cpu_baseline": cpu,
         "forms": forms,
