@@ -237,6 +237,57 @@ int lbm_graph_end(lbm_stream_t stream, lbm_graph **out);
 int lbm_graph_launch(lbm_graph *g, lbm_stream_t stream);
 int lbm_graph_destroy(lbm_graph *g);
 
+/*
+ * Learned subgrid closure (no reference counterpart: the reference's only closure is the Smagorinsky model
+ * of lbm_core4.cu:171-177).  An MLP 18 -> 64 -> 64 -> 1 with GELU, evaluated and trained on the matrix
+ * cores with BF16 operands and FP32 accumulation.  `params` holds the 5441 FP32 parameters in the order
+ *   W1[18][64] (input-major), b1[64], W2[64][64], b2[64], W3[64], b3;
+ * `packed` holds lbm_mlp_npack() BF16 values derived from them by lbm_mlp_pack (the kernels read both).
+ * Samples are rows of x[n][18] (FP32, rounded to BF16 on load).
+ */
+int lbm_mlp_nparam(void); /* 5441 */
+int lbm_mlp_npack(void);  /* 11264 */
+int lbm_mlp_pack(const float *params, uint16_t *packed, lbm_stream_t stream);
+int lbm_mlp_forward(const uint16_t *packed, const float *params, const float *x, int n, float *y, lbm_stream_t stream);
+/* Bytes of the `work` buffer lbm_mlp_grad / lbm_mlp_train_step need for n samples. */
+size_t lbm_mlp_work_size(int n);
+/* grad = dL/dparams of L = sum_s wgt[s] (y_s - t[s])^2; loss (may be NULL) = L.  Deterministic. */
+int lbm_mlp_grad(const uint16_t *packed, const float *params, const float *x, const float *t, const float *wgt,
+                 int n, float *work, float *grad, float *loss, lbm_stream_t stream);
+/* Adam: m = beta1 m + (1 - beta1) g; v = beta2 v + (1 - beta2) g^2; p -= lr (m / bc1) / (sqrt(v / bc2) + eps),
+ * with the bias corrections bc1 = 1 - beta1^t, bc2 = 1 - beta2^t supplied by the caller. */
+typedef struct lbm_adam {
+    float lr, beta1, beta2, eps, bc1, bc2;
+} lbm_adam;
+/* One training step: the gradient of lbm_mlp_grad, an Adam update of params / m / v, and a repack.
+ * grad and loss may be NULL. */
+int lbm_mlp_train_step(uint16_t *packed, float *params, float *m, float *v, const float *x, const float *t,
+                       const float *wgt, int n, const lbm_adam *adam, float *work, float *grad, float *loss,
+                       lbm_stream_t stream);
+
+/* m7neq and m8neq (lbm_core4.cu:166-173) of the pre-collision state of every cell of a periodic MRT lattice
+ * into neq[2][H][W] (solid cells: 0). */
+int lbm_neq(const lbm_params *p, const uint8_t *mask, const float *f, int parity, float *neq, lbm_stream_t stream);
+
+/* Feature scaling and trust region of the closure. */
+typedef struct lbm_closure {
+    float in_scale[18]; /* feature 2 s + c, s = 3 (dy + 1) + (dx + 1), c = 0: m7neq, 1: m8neq, is multiplied by
+                           in_scale[2 s + c] */
+    float nu_scale;     /* nu_t = nu_scale * network output (lattice units) */
+    float nu_max;       /* nu_t outside [0, nu_max] (or NaN) falls back to Smagorinsky */
+    float feat_max;     /* a scaled feature with |value| > feat_max falls back to Smagorinsky */
+} lbm_closure;
+/*
+ * One in-place AA step of a periodic MRT lattice (FP32 storage) whose stress moments relax at
+ * 1 / (tau_0 + 3 nu_t), nu_t from the network on the 3 x 3 stencil of (m7neq, m8neq) around each cell;
+ * fallback cells use Smagorinsky with p->cs (lbm_core4.cu:171-177; plain MRT when cs = 0).  `neq` is a work
+ * buffer of 2 W H floats (left holding the step's features); nut (may be NULL) receives nu_t per cell (-1 where
+ * the fallback was taken); fallbacks (may be NULL) is incremented by the number of fluid fallback cells.
+ */
+int lbm_step_closure(const lbm_params *p, const uint8_t *mask, float *f, int parity, const uint16_t *packed,
+                     const float *params, const lbm_closure *cl, float *neq, float *nut, unsigned *fallbacks,
+                     lbm_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,11 +6,13 @@
 
 namespace lbm {
 
-// One AA step of cell (x, y); cells outside the lattice return at once.
-template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS>
+// One AA step of cell (x, y); cells outside the lattice return at once.  EXT: the stress moments relax at
+// w_ext when w_ext >= 0 (a learned closure's rate), else by the model's own rule.
+template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS, bool EXT = false>
 __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict__ mask, void *__restrict__ f,
                                         const float *__restrict__ inlet_u, float *__restrict__ oux,
-                                        float *__restrict__ ouy, float *__restrict__ orho, int x, int y)
+                                        float *__restrict__ ouy, float *__restrict__ orho, int x, int y,
+                                        float w_ext = -1.0f)
 {
     using S = Store<FMT>;
     constexpr unsigned B = S::BYTES;
@@ -64,7 +66,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
             }
         }
     }
-    collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
+    collide_dev<MODEL, EXT>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac, w_ext);
     if constexpr (PARITY == 0) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) S::st(rf, idx * B, (unsigned)opp(q) * plane * B, q, g[q]);

@@ -85,7 +85,10 @@ __device__ __forceinline__ float zou_he(float h[9], float u, float &rho)
 // MRT + Smagorinsky collision (lbm_core4.cu:124-199) on deviations, h -> h' (unclamped).  `inlet`
 // selects Zou-He and the inlet collision of lbm_core4.cu:54-93 (no LES, no clamps, m3 = rho u, m5 = 0).
 // mac = (rho, ux, uy) as written by the reference (lbm_core4.cu:59-61,145-147).
-__device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3])
+// EXT: the stress moments relax at w_ext when w_ext >= 0 (a learned closure's rate), else as usual.
+template <bool EXT = false>
+__device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
+                                            float w_ext = -1.0f)
 {
     float rho = 0.f, ux = 0.f, uy = 0.f, drc = 0.f, rc = 0.f, inv_rc = 0.f;
     bool les = false;
@@ -145,7 +148,14 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
     const float n7 = m7 - m7eq;
     const float n8 = m8 - m8eq;
     float w = P.tau_inv;
-    if (les) {
+    bool smag = les;
+    if constexpr (EXT) {
+        if (w_ext >= 0.0f) {
+            w = w_ext;
+            smag = false;
+        }
+    }
+    if (smag) {
         // Smagorinsky (lbm_core4.cu:172-177)
         const float Q = sqrt_rn_normal(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
         const float d = fma_(P.c18cs2, Q * inv_rc, P.tau0sq);
@@ -214,13 +224,14 @@ __device__ __forceinline__ void collide_bgk(const KPar &P, float h[9], bool inle
     }
 }
 
-template <int MODEL>
-__device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inlet, float uin, float mac[3])
+template <int MODEL, bool EXT = false>
+__device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
+                                            float w_ext = -1.0f)
 {
     if constexpr (MODEL == MODEL_BGK)
         collide_bgk(P, h, inlet, uin, mac);
     else
-        collide_mrt(P, h, inlet, uin, mac);
+        collide_mrt<EXT>(P, h, inlet, uin, mac, w_ext);
 }
 
 // ------------------------------------------------------------------------------------------------------

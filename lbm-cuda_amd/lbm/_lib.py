@@ -41,6 +41,19 @@ class LbmFusedCfg(ctypes.Structure):
     _fields_ = [("strip", ctypes.c_int32), ("chunk", ctypes.c_int32)]
 
 
+class LbmAdam(ctypes.Structure):
+    """Mirror of ``lbm_adam`` (include/lbm.h)."""
+
+    _fields_ = [(n, ctypes.c_float) for n in ("lr", "beta1", "beta2", "eps", "bc1", "bc2")]
+
+
+class LbmClosure(ctypes.Structure):
+    """Mirror of ``lbm_closure`` (include/lbm.h)."""
+
+    _fields_ = [("in_scale", ctypes.c_float * 18), ("nu_scale", ctypes.c_float), ("nu_max", ctypes.c_float),
+                ("feat_max", ctypes.c_float)]
+
+
 class LbmError(RuntimeError):
     pass
 
@@ -80,6 +93,15 @@ SIGNATURES = {
     "lbm_graph_end": (_i, [_vp, _vp]),
     "lbm_graph_launch": (_i, [_vp, _vp]),
     "lbm_graph_destroy": (_i, [_vp]),
+    "lbm_mlp_nparam": (_i, []),
+    "lbm_mlp_npack": (_i, []),
+    "lbm_mlp_pack": (_i, [_vp, _vp, _vp]),
+    "lbm_mlp_forward": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
+    "lbm_mlp_work_size": (ctypes.c_size_t, [_i]),
+    "lbm_mlp_grad": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "lbm_mlp_train_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "lbm_neq": (_i, [_P, _vp, _vp, _i, _vp, _vp]),
+    "lbm_step_closure": (_i, [_P, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -74,6 +74,8 @@ def lib():
             getattr(L, "oracle_macro" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp, vp, vp]
             real = ctypes.c_double if suf == "_f64" else ctypes.c_float
             getattr(L, "oracle_collide" + suf).argtypes = [P, vp, ctypes.c_int, real, vp]
+            getattr(L, "oracle_aa_step_rate" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
+            getattr(L, "oracle_neq" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
         L.oracle_mrt_forward_f64.argtypes = [vp, vp]
         L.oracle_mrt_inverse_f64.argtypes = [vp, vp]
         L.oracle_diagnostics_f32.argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
@@ -118,6 +120,24 @@ def aa_steps(p, mask, f, inlet_u, parity, n):
         aa_step(p, mask, f, inlet_u, parity)
         parity ^= 1
     return parity
+
+
+def aa_step_rate(p, mask, f, inlet_u, parity, w_cell):
+    """One AA step in place whose MRT stress moments relax at w_cell[y, x] where that is >= 0 (the learned
+    closure's rate); other cells follow the model's own rule.  w_cell has f's precision."""
+    _check_state(p, mask, f, inlet_u)
+    w_cell = np.ascontiguousarray(w_cell, dtype=f.dtype)
+    getattr(lib(), "oracle_aa_step_rate" + _suffix(f.dtype))(
+        ctypes.byref(p), _ptr(mask), _ptr(f), _ptr(inlet_u), int(parity), _ptr(w_cell))
+
+
+def neq(p, mask, f, inlet_u, parity):
+    """m7neq and m8neq of the pre-collision state of every updated cell, [2, H, W] (0 elsewhere)."""
+    _check_state(p, mask, f, inlet_u)
+    out = np.zeros((2, p.H, p.W), dtype=f.dtype)
+    getattr(lib(), "oracle_neq" + _suffix(f.dtype))(ctypes.byref(p), _ptr(mask), _ptr(f), _ptr(inlet_u),
+                                                     int(parity), _ptr(out))
+    return out
 
 
 def pull_step(p, mask, f_now, f_out, inlet_u):

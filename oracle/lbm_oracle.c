@@ -278,7 +278,7 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
             if (P.model == ORACLE_BGK)
                 collide_bgk_f32(&P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
             else
-                collide_mrt_f32(&P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac);
+                collide_mrt_f32(&P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac, -1.0f);
             if (parity == 0) {
                 for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx] = h16_store(fmt, q, g[q]);
                 if (channel && x == P.W - 2)

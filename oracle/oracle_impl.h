@@ -90,7 +90,8 @@ static inline REAL FN(zou_he)(REAL *h, REAL uin, REAL *rho)
     return drho;
 }
 
-static void FN(collide_mrt)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin, REAL *mac)
+/* w_ext >= 0: the stress moments relax at w_ext (an external closure's rate) instead of the model's own */
+static void FN(collide_mrt)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin, REAL *mac, REAL w_ext)
 {
     REAL rho = 0, ux = 0, uy = 0, drc = 0, rc = 0;
     int les = 0;
@@ -149,6 +150,10 @@ static void FN(collide_mrt)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin
     REAL n7 = m7 - m7eq;
     REAL n8 = m8 - m8eq;
     REAL w = P->tau_inv;
+    if (w_ext >= (REAL)0) {
+        w = w_ext;
+        les = 0;
+    }
     if (les) {
         /* Smagorinsky, lbm_core4.cu:172-177 */
         REAL Q = SQRT(FMA(n7, n7, FMA(n8, n8, (REAL)1e-8)));
@@ -229,13 +234,13 @@ static void FN(collide_bgk)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin
 }
 
 /* Collision of FP32/FP64-stored populations: h = f - w, collide, f' = clamp(w + h', 0, 10). */
-static inline void FN(collide)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin, REAL *mac)
+static inline void FN(collide)(const FN(kpar_t) * P, REAL f[9], int inlet, REAL uin, REAL *mac, REAL w_ext)
 {
     for (int q = 0; q < 9; ++q) f[q] = f[q] - FN(wq)(q);
     if (P->model == ORACLE_BGK)
         FN(collide_bgk)(P, f, inlet, uin, mac);
     else
-        FN(collide_mrt)(P, f, inlet, uin, mac);
+        FN(collide_mrt)(P, f, inlet, uin, mac, w_ext);
     for (int q = 0; q < 9; ++q) f[q] = FN(clampr)(FN(wq)(q) + f[q], (REAL)0, (REAL)10);
 }
 
@@ -267,9 +272,86 @@ static inline int FN(role)(const FN(kpar_t) * P, const uint8_t *mask, int x, int
     return mask[(long)y * P->W + x] ? 0 : 1;
 }
 
+/* The incoming populations of cell (x, y) of an AA state of the given parity (see oracle_aa_step). */
+static void FN(gather)(const FN(kpar_t) * P, const uint8_t *mask, const REAL *f, int parity, int x, int y,
+                       int inlet, REAL g[9])
+{
+    const long plane = (long)P->W * P->H;
+    const long idx = (long)y * P->W + x;
+    if (parity == 0) {
+        for (int q = 0; q < 9; ++q) g[q] = f[q * plane + idx];
+        return;
+    }
+    for (int q = 0; q < 9; ++q) {
+        if (inlet && CX[q] == 1) {
+            g[q] = (REAL)0; /* from x = -1: rebuilt by Zou-He */
+            continue;
+        }
+        long s = FN(nbr)(P, x, y, -CX[q], -CY[q]);
+        g[q] = mask[s] ? f[q * plane + idx] : f[OPP[q] * plane + s];
+    }
+}
+
+static void FN(aa_step_impl)(const oracle_params *pp, const uint8_t *mask, REAL *f, const REAL *inlet_u,
+                             int parity, REAL *oux, REAL *ouy, REAL *orho, const REAL *w_cell);
+
 /* One AA step in place.  parity 0: canonical -> reversed (local).  parity 1: reversed -> canonical. */
 void FN(oracle_aa_step)(const oracle_params *pp, const uint8_t *mask, REAL *f, const REAL *inlet_u, int parity,
                         REAL *oux, REAL *ouy, REAL *orho)
+{
+    FN(aa_step_impl)(pp, mask, f, inlet_u, parity, oux, ouy, orho, NULL);
+}
+
+/* oracle_aa_step whose MRT stress moments relax at w_cell[cell] where that is >= 0 (the learned closure's
+   rate); cells with w_cell < 0 use the model's own rule. */
+void FN(oracle_aa_step_rate)(const oracle_params *pp, const uint8_t *mask, REAL *f, const REAL *inlet_u,
+                             int parity, const REAL *w_cell)
+{
+    FN(aa_step_impl)(pp, mask, f, inlet_u, parity, NULL, NULL, NULL, w_cell);
+}
+
+/* m7neq and m8neq of the pre-collision state of every updated cell (collide_mrt's n7, n8) into
+   neq[0][cell], neq[1][cell]; other cells get 0. */
+void FN(oracle_neq)(const oracle_params *pp, const uint8_t *mask, const REAL *f, const REAL *inlet_u, int parity,
+                    REAL *neq)
+{
+    FN(kpar_t) P;
+    FN(make_kpar)(pp, &P);
+    const long plane = (long)P.W * P.H;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < P.H; ++y) {
+        for (int x = 0; x < P.W; ++x) {
+            const long idx = (long)y * P.W + x;
+            neq[idx] = neq[plane + idx] = (REAL)0;
+            int r = FN(role)(&P, mask, x, y);
+            if (!r) continue;
+            REAL g[9];
+            FN(gather)(&P, mask, f, parity, x, y, r == 2, g);
+            REAL s13, s24, s57, s68, h[9];
+            for (int q = 0; q < 9; ++q) h[q] = g[q] - FN(wq)(q);
+            s13 = h[1] + h[3], s24 = h[2] + h[4], s57 = h[5] + h[7], s68 = h[6] + h[8];
+            REAL h14 = s13 + s24, h58 = s57 + s68;
+            REAL a13 = h[1] - h[3], a24 = h[2] - h[4], a57 = h[5] - h[7], a86 = h[8] - h[6];
+            REAL sx = a57 + a86, sy = a57 - a86;
+            REAL drho = (h[0] + h14) + h58;
+            REAL rho = (REAL)1 + drho;
+            REAL inv_rho = (REAL)1 / rho;
+            REAL ux = (a13 + sx) * inv_rho, uy = (a24 + sy) * inv_rho, drc = drho;
+            if (P.clamp) {
+                ux = FN(clampr)(ux, (REAL)-0.4, (REAL)0.4);
+                uy = FN(clampr)(uy, (REAL)-0.4, (REAL)0.4);
+                drc = FN(clampr)(drho, (REAL)-1, (REAL)0.5);
+            }
+            REAL rc = (REAL)1 + drc;
+            REAL m3 = rc * ux;
+            neq[idx] = (s13 - s24) - rc * (ux * ux - uy * uy);
+            neq[plane + idx] = (s57 - s68) - m3 * uy;
+        }
+    }
+}
+
+static void FN(aa_step_impl)(const oracle_params *pp, const uint8_t *mask, REAL *f, const REAL *inlet_u,
+                             int parity, REAL *oux, REAL *ouy, REAL *orho, const REAL *w_cell)
 {
     FN(kpar_t) P;
     FN(make_kpar)(pp, &P);
@@ -283,19 +365,8 @@ void FN(oracle_aa_step)(const oracle_params *pp, const uint8_t *mask, REAL *f, c
             const int inlet = r == 2;
             const long idx = (long)y * P.W + x;
             REAL g[9], mac[3];
-            if (parity == 0) {
-                for (int q = 0; q < 9; ++q) g[q] = f[q * plane + idx];
-            } else {
-                for (int q = 0; q < 9; ++q) {
-                    if (inlet && CX[q] == 1) {
-                        g[q] = (REAL)0; /* from x = -1: rebuilt by Zou-He */
-                        continue;
-                    }
-                    long s = FN(nbr)(&P, x, y, -CX[q], -CY[q]);
-                    g[q] = mask[s] ? f[q * plane + idx] : f[OPP[q] * plane + s];
-                }
-            }
-            FN(collide)(&P, g, inlet, inlet ? inlet_u[y] : (REAL)0, mac);
+            FN(gather)(&P, mask, f, parity, x, y, inlet, g);
+            FN(collide)(&P, g, inlet, inlet ? inlet_u[y] : (REAL)0, mac, w_cell ? w_cell[idx] : (REAL)-1);
             if (parity == 0) {
                 for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx] = g[q];
                 if (channel && x == P.W - 2)
@@ -351,7 +422,7 @@ void FN(oracle_pull_step)(const oracle_params *pp, const uint8_t *mask, const RE
                 long s = FN(nbr)(&P, x, y, -CX[q], -CY[q]);
                 g[q] = mask[s] ? f_now[OPP[q] * plane + idx] : f_now[q * plane + s];
             }
-            FN(collide)(&P, g, inlet, inlet ? inlet_u[y] : (REAL)0, mac);
+            FN(collide)(&P, g, inlet, inlet ? inlet_u[y] : (REAL)0, mac, (REAL)-1);
             for (int q = 0; q < 9; ++q) {
                 f_out[q * plane + idx] = g[q];
                 if (channel && x == P.W - 2) f_out[q * plane + idx + 1] = g[q];
@@ -406,7 +477,7 @@ void FN(oracle_collide)(const oracle_params *pp, REAL *f9, int inlet, REAL uin, 
 {
     FN(kpar_t) P;
     FN(make_kpar)(pp, &P);
-    FN(collide)(&P, f9, inlet, uin, mac3);
+    FN(collide)(&P, f9, inlet, uin, mac3, (REAL)-1);
 }
 
 #undef FN
