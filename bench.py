@@ -123,6 +123,14 @@ def bench_ensemble(nsteps, graph_steps=20):
     return res
 
 
+def bench_closure(n):
+    """The learned-closure study: fine n^2 run -> 8x box filter -> MLP trained on the GPU kernels -> coarse
+    runs with the learned closure and with plain Smagorinsky compared against the filtered fine run."""
+    from lbm.closure import StudyConfig, run_study
+    rep, _, _ = run_study(StudyConfig(N=n), log=log)
+    return rep
+
+
 def updated_cells(sim):
     """Cells a step updates (fluid, not wall rows, not the outlet ghost column)."""
     upd = sim.mask_host == 0
@@ -164,10 +172,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--l2-steps", type=int, default=200, help="steps for the L2 error against FP64 (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=("all", "aa", "fused", "ensemble"), default="all",
+    ap.add_argument("--only", choices=("all", "aa", "fused", "ensemble", "closure"), default="all",
                     help="run only the plain single step, the fused form or the ensemble (for profiling)")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="timed steps of the 64-lattice batched ensemble (0 = skip)")
+    ap.add_argument("--closure-n", type=int, default=4096,
+                    help="fine lattice size of the learned-closure study (coarse = n / 8; 0 = skip)")
     ap.add_argument("--copy-probe", default="", metavar="MIB[,MIB...]",
                     help="first copy device buffers of these many MiB (known-size kernels for counter calibration)")
     args = ap.parse_args()
@@ -186,6 +196,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.only == "ensemble":
         print(json.dumps({"ensemble": bench_ensemble(max(args.ensemble_steps, 20))}), flush=True)
+        return
+    if args.only == "closure":
+        print(json.dumps({"closure": bench_closure(args.closure_n or 4096)}), flush=True)
         return
     for mib in (int(v) for v in args.copy_probe.split(",") if v.strip()):
         src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device=dev)
@@ -341,6 +354,7 @@ def main():
         log(f"CPU baseline: {cpu['value'] * 1e3:.1f} MLUPS on {cpu['cores']} threads")
 
     ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 and args.only == "all" else None
+    closure = bench_closure(args.closure_n) if args.closure_n > 0 and args.only == "all" else None
 
     # the chosen form: the fastest one whose velocity L2 error against FP64 stays within 1e-3
     ok = {n: v for n, v in forms.items() if "glups" in v and v.get("l2_u", 0.0) <= 1e-3}
@@ -375,6 +389,7 @@ def main():
         "l2_err": l2,
         "drag_lift": drag,
         "ensemble": ensemble,
+        "closure": closure,
     }
     print(json.dumps(out), flush=True)
 

@@ -274,7 +274,7 @@ typedef struct lbm_closure {
     float in_scale[18]; /* feature 2 s + c, s = 3 (dy + 1) + (dx + 1), c = 0: m7neq, 1: m8neq, is multiplied by
                            in_scale[2 s + c] */
     float nu_scale;     /* nu_t = nu_scale * network output (lattice units) */
-    float nu_max;       /* nu_t outside [0, nu_max] (or NaN) falls back to Smagorinsky */
+    float nu_max;       /* nu_t above nu_max (or NaN) falls back to Smagorinsky; negative nu_t clamps to 0 */
     float feat_max;     /* a scaled feature with |value| > feat_max falls back to Smagorinsky */
 } lbm_closure;
 /*

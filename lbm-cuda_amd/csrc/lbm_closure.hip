@@ -1,8 +1,9 @@
 // The learned subgrid closure inside the collide-stream step (periodic lattices, FP32 state):
 //   lbm_neq           m7neq / m8neq of the pre-collision state at every cell (lbm_core4.cu:166-173)
 //   lbm_step_closure  one in-place AA step whose stress moments relax at 1 / (tau_0 + 3 nu_t), with nu_t
-//                     from the MLP of lbm_mlp.h evaluated on the 3 x 3 stencil of (m7neq, m8neq); cells whose
-//                     features or output leave the trusted range fall back to Smagorinsky (lbm_core4.cu:171-177)
+//                     from the MLP of lbm_mlp.h evaluated on the 3 x 3 stencil of (m7neq, m8neq) and clamped at
+//                     0; cells whose features or output leave the trusted range fall back to Smagorinsky
+//                     (lbm_core4.cu:171-177)
 //
 // No reference counterpart beyond the Smagorinsky fallback.  A step is two launches: k_neq writes the two
 // non-equilibrium moments of every cell (the features must be those of the current state, which for an odd
@@ -113,8 +114,11 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_closure(KPar P, const uint8_
     f4 z1[4][4], z2[4][4];
     wave_forward<false>(xs, h1s, h2s, wl, prm, ys[w], z1, z2);
     __syncthreads();
-    float nu = ys[w][lane] * cl.nu_scale;
-    const bool ok = trusted && nu >= 0.0f && nu <= cl.nu_max; // false for NaN
+    // negative outputs (backscatter) clamp to no eddy viscosity; NaN, nu > nu_max or untrusted features
+    // fall back to Smagorinsky
+    const float raw = ys[w][lane] * cl.nu_scale;
+    const bool ok = trusted && raw <= cl.nu_max; // false for NaN
+    const float nu = raw > 0.0f ? raw : 0.0f;
     float w_ext = -1.0f;
     if (ok) w_ext = __fdiv_rn(1.0f, P.tau0 + 3.0f * nu);
     if (nfall) {

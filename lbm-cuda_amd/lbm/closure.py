@@ -155,3 +155,268 @@ def stencil_features(neq_hw):
             for c in range(2):
                 cols.append(np.roll(neq_hw[c], (-dy, -dx), axis=(0, 1)).reshape(-1))
     return np.stack(cols, 1)
+
+
+# ------------------------------------------------------------------------------------------------ the study
+def box_filter(a, factor):
+    """Mean over factor x factor blocks (the coarse cell (i, j) covers fine cells i*factor .. i*factor+factor-1)."""
+    H, W = a.shape
+    return a.reshape(H // factor, factor, W // factor, factor).mean(axis=(1, 3))
+
+
+def filtered_fields(sim: Simulation, factor):
+    """Favre-filtered rho, u, v and the subgrid stress tau_ab = (rho u_a u_b)~ / rho~ - u~_a u~_b of a fine run
+    on the coarse grid (FP64, host)."""
+    rho, ux, uy = (t.cpu().numpy().astype(np.float64) for t in sim.macro())
+    rb = box_filter(rho, factor)
+    u = box_filter(rho * ux, factor) / rb
+    v = box_filter(rho * uy, factor) / rb
+    txx = box_filter(rho * ux * ux, factor) / rb - u * u
+    tyy = box_filter(rho * uy * uy, factor) / rb - v * v
+    txy = box_filter(rho * ux * uy, factor) / rb - u * v
+    return rb, u, v, (txx, tyy, txy)
+
+
+def strain(u, v):
+    """Deviatoric strain (s1 = (S_xx - S_yy) / 2, s2 = S_xy) by periodic central differences (lattice units)."""
+    dudx = 0.5 * (np.roll(u, -1, 1) - np.roll(u, 1, 1))
+    dudy = 0.5 * (np.roll(u, -1, 0) - np.roll(u, 1, 0))
+    dvdx = 0.5 * (np.roll(v, -1, 1) - np.roll(v, 1, 1))
+    dvdy = 0.5 * (np.roll(v, -1, 0) - np.roll(v, 1, 0))
+    return 0.5 * (dudx - dvdy), 0.5 * (dudy + dvdx)
+
+
+def window_sum(a, w):
+    """Periodic sum over the w x w neighbourhood centred on each cell (w odd)."""
+    out = np.zeros_like(a)
+    r = w // 2
+    for dy in range(-r, r + 1):
+        for dx in range(-r, r + 1):
+            out += np.roll(a, (dy, dx), axis=(0, 1))
+    return out
+
+
+def sgs_target(u, v, tau, window=1):
+    """Eddy viscosity that best matches the deviatoric subgrid stress in the least-squares sense over the
+    window x window neighbourhood of each cell, nu* = -sum(d : S) / (2 sum(S : S)), and the weight sum(S : S)
+    of that fit (sum |tau_d + 2 nu S|^2 = 4 sum|S|^2 (nu - nu*)^2 + c).  window = 1 is the point-wise fit."""
+    s1, s2 = strain(u, v)
+    txx, tyy, txy = tau
+    d1, d2 = 0.5 * (txx - tyy), txy
+    num = -(d1 * s1 + d2 * s2)
+    ss = s1 * s1 + s2 * s2
+    if window > 1:
+        num, ss = window_sum(num, window), window_sum(ss, window)
+    return num / (2.0 * ss + 1e-30), ss
+
+
+def smagorinsky_nu(neq_hw, rho, tau0, cs):
+    """nu_t = (tau_eff - tau_0) / 3 of the Smagorinsky rule of lbm_core4.cu:174-176 for given moments."""
+    Q = np.sqrt(neq_hw[0].astype(np.float64) ** 2 + neq_hw[1].astype(np.float64) ** 2 + 1e-8)
+    tau_eff = 0.5 * (tau0 + np.sqrt(tau0 * tau0 + 18.0 * cs * cs * Q / rho))
+    return (tau_eff - tau0) / 3.0
+
+
+def spectrum(u, v):
+    """Shell-averaged kinetic energy spectrum E(k), k = 0 .. N/2 (integer shells), of a periodic field."""
+    H, W = u.shape
+    uk = np.fft.fft2(u) / (H * W)
+    vk = np.fft.fft2(v) / (H * W)
+    e = 0.5 * (np.abs(uk) ** 2 + np.abs(vk) ** 2)
+    kx = np.fft.fftfreq(W) * W
+    ky = np.fft.fftfreq(H) * H
+    k = np.rint(np.hypot(*np.meshgrid(kx, ky))).astype(int)
+    kmax = min(H, W) // 2
+    return np.bincount(k.ravel(), weights=e.ravel(), minlength=kmax + 1)[:kmax + 1]
+
+
+def enstrophy(u, v):
+    """Z = <omega^2> / 2 with omega = dv/dx - du/dy (spectral derivatives)."""
+    H, W = u.shape
+    kx = 2j * np.pi * np.fft.fftfreq(W)[None, :]
+    ky = 2j * np.pi * np.fft.fftfreq(H)[:, None]
+    w = np.real(np.fft.ifft2(kx * np.fft.fft2(v) - ky * np.fft.fft2(u)))
+    return 0.5 * float(np.mean(w * w))
+
+
+@dataclass
+class StudyConfig:
+    """The learned-closure workload: fine N x N periodic run (MRT + Smagorinsky cs_fine, random-phase field
+    with E(k) ~ k^4 exp(-2 (k / k0)^2) at u_rms), box filter by `factor`, coarse runs with nu / factor
+    (acoustic scaling: one coarse step = `factor` fine steps)."""
+    N: int = 4096
+    factor: int = 8
+    nu_fine: float = 1e-4
+    cs_fine: float = 0.05
+    cs_coarse: float = 0.1     # the Smagorinsky baseline and the closure's fallback
+    k0: float = 16.0
+    u_rms: float = 0.05
+    seed: int = 5
+    spinup: int = 2000         # fine steps before the first snapshot
+    snapshots: int = 8         # training snapshots (the last one is held out for validation)
+    interval: int = 400        # fine steps between snapshots
+    warm: int = 10             # coarse Smagorinsky steps before the features are taken
+    window: int = 5            # neighbourhood of the least-squares eddy-viscosity target
+    eval_steps: int = 400      # coarse steps of the a-posteriori comparison
+    eval_every: int = 100
+    epochs: int = 6
+    batch: int = 8192
+    lr: float = 2e-3
+
+
+def _coarse_sim(cfg: StudyConfig, rho, u, v, cs, device):
+    n = rho.shape[0]
+    nu = cfg.nu_fine / cfg.factor
+    tau_inv = float(np.float32(1.0 / (3.0 * nu + 0.5)))
+    return Simulation(n, n, tau_inv, cs=cs, model="mrt", bc="periodic", f0=regularized_state(rho, u, v, nu, cs),
+                      device=device)
+
+
+def regularized_state(rho, u, v, nu, cs):
+    """Populations with the Chapman-Enskog non-equilibrium part of the given velocity field:
+    f_q = feq_q - 3 w_q rho tau (c_q c_q - I / 3) : S, tau = 3 (nu + cs^2 |S|) + 1/2, |S| = sqrt(2 S : S).
+    Starting a lattice near tau = 1/2 from feq alone leaves a non-equilibrium transient that changes sign every
+    step and decays only as |1 - 1/tau|^n."""
+    from .geometry import CX, CY, WEIGHTS, feq
+    f = feq(rho, u, v)
+    dudx = 0.5 * (np.roll(u, -1, 1) - np.roll(u, 1, 1))
+    dudy = 0.5 * (np.roll(u, -1, 0) - np.roll(u, 1, 0))
+    dvdx = 0.5 * (np.roll(v, -1, 1) - np.roll(v, 1, 1))
+    dvdy = 0.5 * (np.roll(v, -1, 0) - np.roll(v, 1, 0))
+    sxx, syy, sxy = dudx, dvdy, 0.5 * (dudy + dvdx)
+    tau = 3.0 * (nu + cs * cs * np.sqrt(2.0 * (sxx * sxx + syy * syy + 2.0 * sxy * sxy))) + 0.5
+    tr = (sxx + syy) / 3.0
+    for q in range(9):
+        cs_q = CX[q] * CX[q] * sxx + 2.0 * CX[q] * CY[q] * sxy + CY[q] * CY[q] * syy - tr
+        f[q] -= 3.0 * WEIGHTS[q] * rho * tau * cs_q
+    return f
+
+
+def _coarse_uv(sim):
+    rho, ux, uy = (t.cpu().numpy().astype(np.float64) for t in sim.macro())
+    return rho, ux, uy
+
+
+def run_study(cfg: StudyConfig = StudyConfig(), device="cuda", log=print):
+    """Build the training set from the fine run, train the MLP on the GPU, and compare the closure with
+    plain Smagorinsky a posteriori.  Returns a report dict."""
+    import time
+    from . import geometry as geo
+    t_start = time.perf_counter()
+    N, fct = cfg.N, cfg.factor
+    n = N // fct
+    ux, uy = geo.random_phase_velocity(N, N, k0=cfg.k0, u_rms=cfg.u_rms, seed=cfg.seed)
+    fine = Simulation(N, N, float(np.float32(geo.nu_tau_inv(cfg.nu_fine))), cs=cfg.cs_fine, model="mrt",
+                      bc="periodic", f0=geo.feq(np.ones((N, N)), ux, uy), device=device)
+    del ux, uy
+    log(f"closure study: fine {N}^2 built in {time.perf_counter() - t_start:.1f} s")
+    fine.step(cfg.spinup)
+    X, T, Wt, smag_nu = [], [], [], []
+    for s in range(cfg.snapshots):
+        rb, u, v, _ = filtered_fields(fine, fct)
+        coarse = _coarse_sim(cfg, rb, u, v, cfg.cs_coarse, device)
+        coarse.step(cfg.warm)
+        nq = neq(coarse).cpu().numpy()
+        fine.step(cfg.warm * fct)
+        rb2, u2, v2, tau2 = filtered_fields(fine, fct)
+        nu_star, ss = sgs_target(u2, v2, tau2, cfg.window)
+        X.append(stencil_features(nq).astype(np.float32))
+        T.append(nu_star.reshape(-1))
+        Wt.append(ss.reshape(-1))
+        rho_c = coarse.macro()[0].cpu().numpy().astype(np.float64)
+        smag_nu.append(smagorinsky_nu(nq, rho_c, 1.0 / coarse.params.tau_inv, cfg.cs_coarse).reshape(-1))
+        del coarse
+        fine.step(cfg.interval)
+    log(f"closure study: {cfg.snapshots} snapshots in {time.perf_counter() - t_start:.1f} s")
+    # normalisation from the training snapshots
+    Xtr = np.concatenate(X[:-1])
+    s7 = 1.0 / Xtr[:, 0::2].std()
+    s8 = 1.0 / Xtr[:, 1::2].std()
+    in_scale = np.tile(np.array([s7, s8], np.float32), 9)
+    Ttr = np.concatenate(T[:-1])
+    Wtr = np.concatenate(Wt[:-1])
+    nu_scale = float(np.sqrt(np.sum(Wtr * Ttr ** 2) / np.sum(Wtr)))
+    ccfg = ClosureConfig(in_scale=in_scale, nu_scale=nu_scale, nu_max=0.05, feat_max=20.0)
+
+    def prep(Xs, Ts, Ws):
+        x = np.concatenate(Xs) * in_scale[None, :]
+        t = np.concatenate(Ts) / nu_scale
+        w = np.concatenate(Ws)
+        return x.astype(np.float32), t.astype(np.float32), (w / w.mean()).astype(np.float32)
+
+    xtr, ttr, wtr = prep(X[:-1], T[:-1], Wt[:-1])
+    xva, tva, wva = prep(X[-1:], T[-1:], Wt[-1:])
+    mlp = MLP(seed=cfg.seed, device=device)
+    rng = np.random.default_rng(cfg.seed)
+    nb = len(xtr) // cfg.batch
+    loss = torch.empty(1, dtype=torch.float32, device=device)
+    t_train = time.perf_counter()
+    steps = 0
+    for ep in range(cfg.epochs):
+        perm = rng.permutation(len(xtr))[:nb * cfg.batch]
+        xe = torch.from_numpy(xtr[perm]).to(device)
+        te = torch.from_numpy(ttr[perm]).to(device)
+        we = torch.from_numpy(wtr[perm] / cfg.batch).to(device)
+        lr = cfg.lr * 0.5 * (1 + math.cos(math.pi * ep / cfg.epochs))
+        for b in range(nb):
+            sl = slice(b * cfg.batch, (b + 1) * cfg.batch)
+            mlp.train_step(xe[sl], te[sl], we[sl], lr=lr, loss=loss)
+            steps += 1
+        log(f"  epoch {ep}: lr {lr:.2e}, last batch loss {loss.item():.4f}")
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t_train
+
+    def wmse(pred, t, w):
+        return float(np.sum(w * (pred - t) ** 2) / np.sum(w))
+
+    y_va = mlp.forward(torch.from_numpy(xva).to(device)).cpu().numpy()
+    smag_va = smag_nu[-1] / nu_scale
+    apriori = {"val_wmse_mlp": wmse(y_va, tva, wva), "val_wmse_smagorinsky": wmse(smag_va, tva, wva),
+               "val_wmse_zero": wmse(np.zeros_like(tva), tva, wva),
+               "corr_mlp": float(np.corrcoef(y_va, tva)[0, 1]), "corr_smagorinsky": float(np.corrcoef(smag_va, tva)[0, 1])}
+    log(f"closure study: trained {steps} steps in {train_s:.1f} s; a-priori {apriori}")
+
+    # a posteriori: coarse Smagorinsky vs coarse learned closure from the same filtered state
+    rb, u, v, _ = filtered_fields(fine, fct)
+    smag = _coarse_sim(cfg, rb, u, v, cfg.cs_coarse, device)
+    learned = ClosureSim(_coarse_sim(cfg, rb, u, v, cfg.cs_coarse, device), mlp, ccfg)
+    rows = []
+    kres = np.arange(1, n // 8 + 1)  # shells well inside the filter (k <= n / 8)
+    for k in range(0, cfg.eval_steps + 1, cfg.eval_every):
+        if k:
+            fine.step(cfg.eval_every * fct)
+            smag.step(cfg.eval_every)
+            learned.step(cfg.eval_every)
+        _, uf, vf, _ = filtered_fields(fine, fct)
+        ref_e, ref_z = spectrum(uf, vf), enstrophy(uf, vf)
+        row = {"coarse_step": k, "enstrophy_ref": ref_z}
+        for name, sim in (("smagorinsky", smag), ("learned", learned.sim)):
+            _, uc, vc = _coarse_uv(sim)
+            e = spectrum(uc, vc)
+            row[f"spectrum_err_{name}"] = float(np.mean(np.abs(np.log(e[kres] / ref_e[kres]))))
+            row[f"energy_err_{name}"] = abs(float(e[1:].sum() / ref_e[1:].sum()) - 1.0)
+            row[f"enstrophy_{name}"] = enstrophy(uc, vc)
+            row[f"enstrophy_err_{name}"] = abs(row[f"enstrophy_{name}"] - ref_z) / ref_z
+        rows.append(row)
+        log(f"  coarse step {k}: spectrum err smag {row['spectrum_err_smagorinsky']:.4f} learned "
+            f"{row['spectrum_err_learned']:.4f}; enstrophy err smag {row['enstrophy_err_smagorinsky']:.4f} "
+            f"learned {row['enstrophy_err_learned']:.4f}")
+    fb = int(learned.fallbacks.item())
+    report = {
+        "config": {k: getattr(cfg, k) for k in cfg.__dataclass_fields__},
+        "coarse": n, "samples_train": int(len(xtr)), "samples_val": int(len(xva)), "train_steps": steps,
+        "train_seconds": round(train_s, 2), "nu_scale": nu_scale, "apriori": apriori, "aposteriori": rows,
+        "fallback_fraction": fb / float(cfg.eval_steps * n * n),
+        "seconds": round(time.perf_counter() - t_start, 1),
+    }
+    final = rows[-1]
+    report["summary"] = {
+        "spectrum_shells": f"1..{n // 8}",
+        "spectrum_err_mean_smagorinsky": float(np.mean([r["spectrum_err_smagorinsky"] for r in rows[1:]])),
+        "spectrum_err_mean_learned": float(np.mean([r["spectrum_err_learned"] for r in rows[1:]])),
+        "energy_err_final_smagorinsky": final["energy_err_smagorinsky"],
+        "energy_err_final_learned": final["energy_err_learned"],
+        "enstrophy_err_final_smagorinsky": final["enstrophy_err_smagorinsky"],
+        "enstrophy_err_final_learned": final["enstrophy_err_learned"],
+    }
+    return report, mlp, ccfg

@@ -56,7 +56,7 @@ def test_neq_bitwise(gpu, parity):
 def test_closure_step_bitwise_given_nut(gpu, parity):
     from lbm.closure import ClosureSim, stencil_features
     sim = _sim(seed=5)
-    sim.step(2 + parity)
+    sim.step(20 + parity)
     mlp, cfg, p = _model(sim)
     cl = ClosureSim(sim, mlp, cfg)
     op = _op(sim)
@@ -70,7 +70,7 @@ def test_closure_step_bitwise_given_nut(gpu, parity):
         assert np.array_equal(feats, O.neq(op, sim.mask_host, f0, zero, par)), f"step {n}: features"
         fluid = sim.mask_host == 0
         ok = nut >= 0
-        assert ok[fluid].mean() > 0.9, "most cells should take the learned rate"
+        assert ok[fluid].mean() > 0.5, f"most cells should take the learned rate ({ok[fluid].mean():.3f})"
         # populations: oracle AA step with the GPU's per-cell rate
         tau0 = np.float32(1) / np.float32(sim.params.tau_inv)
         w_cell = np.where(ok, np.float32(1) / (tau0 + np.float32(3) * nut.astype(np.float32)), np.float32(-1))
@@ -83,10 +83,17 @@ def test_closure_step_bitwise_given_nut(gpu, parity):
         # nu_t against the CPU FP64 MLP on the same scaled features
         x = stencil_features(feats) * cfg.in_scale[None, :]
         y, _ = M.forward(p, x.astype(np.float32), mirror=True)
-        nu_cpu = (y * cfg.nu_scale).reshape(sim.H, sim.W)
-        both = ok & fluid & (nu_cpu >= 0) & (nu_cpu <= cfg.nu_max)
+        raw_cpu = (y * cfg.nu_scale).reshape(sim.H, sim.W)
+        nu_cpu = np.maximum(raw_cpu, 0.0)  # negative outputs clamp to 0
+        both = ok & fluid & (raw_cpu <= cfg.nu_max)
         err = np.abs(nut[both] - nu_cpu[both])
         assert err.max() <= 2e-3 * np.abs(nu_cpu[both]).max() + 1e-9, err.max()
+        # the trust region: a cell takes the learned rate iff its features are in range and nu_t <= nu_max
+        trusted = np.all(np.abs(x) <= cfg.feat_max, axis=1).reshape(sim.H, sim.W)
+        margin = 1e-3 * cfg.nu_max
+        clear = fluid & (np.abs(raw_cpu - cfg.nu_max) > margin)
+        want = trusted & (raw_cpu <= cfg.nu_max)
+        assert np.array_equal(ok[clear], want[clear]), f"step {n}: trust region"
 
 
 def test_closure_fallback_is_smagorinsky(gpu):
@@ -117,3 +124,19 @@ def test_closure_argument_errors(gpu):
                             ctypes.c_void_p(sim.f.data_ptr()), 0, ctypes.c_void_p(1), ctypes.c_void_p(1),
                             ctypes.byref(c), ctypes.c_void_p(1), None, None, None)
     assert rc == -1
+
+
+def test_study_pipeline_small(gpu):
+    """The whole learned-closure study at 512^2 -> 64^2: data, training on the GPU kernels, a-priori and
+    a-posteriori comparisons all produce finite numbers, and the network beats the zero predictor a priori."""
+    from lbm.closure import StudyConfig, run_study
+    cfg = StudyConfig(N=512, factor=8, spinup=200, snapshots=3, interval=100, eval_steps=40, eval_every=20,
+                      epochs=3, batch=1024, lr=3e-3)
+    rep, mlp, ccfg = run_study(cfg, log=lambda *a: None)
+    assert rep["train_steps"] > 0 and rep["samples_val"] == 64 * 64
+    ap = rep["apriori"]
+    assert all(np.isfinite(v) for v in ap.values())
+    assert ap["val_wmse_mlp"] < ap["val_wmse_zero"]
+    for row in rep["aposteriori"]:
+        assert all(np.isfinite(v) for v in row.values())
+    assert 0.0 <= rep["fallback_fraction"] <= 1.0
