@@ -131,6 +131,104 @@ def bench_closure(n):
     return rep
 
 
+def _l2_fields(m, ref):
+    """L2(rho) = |rho - rho_ref| / |rho_ref| and L2(u) = |u - u_ref| / |u_ref| of macro fields (rho, ux, uy)."""
+    import numpy as np
+    er = float(np.sqrt(np.sum((m[0] - ref[0]) ** 2) / np.sum(ref[0] ** 2)))
+    eu = float(np.sqrt(np.sum((m[1] - ref[1]) ** 2 + (m[2] - ref[2]) ** 2) / np.sum(ref[1] ** 2 + ref[2] ** 2)))
+    return {"rho": er, "u": eu}
+
+
+def bench_precision(steps_full, steps_scaled=1000):
+    """16-bit deviation storage vs FP32 / FP64 on the periodic double shear layer (8192 x 2048, nu = 1e-4,
+    MRT without LES) and the Taylor-Green vortex on the same grid.  FP64 at full size is out of reach on the
+    host, so full-size errors are against the GPU's FP32 run, sampled along the run (the shear layer rolls up
+    and any two runs decorrelate in time, whatever the precision), and the FP32-vs-FP64 gap that this proxy
+    carries is measured on the same flow at 2048 x 512 with the FP64 oracle, in the same run."""
+    import numpy as np
+    import torch
+
+    from lbm import configs
+    from lbm import geometry as geo
+    from lbm.sim import Simulation
+
+    def macros(s):
+        return tuple(t.cpu().numpy().astype(np.float64) for t in s.macro())
+
+    def sims(case):
+        return {st: Simulation(case.W, case.H, case.tau_inv, **case.sim_kwargs(), storage=st)
+                for st in ("f32", "f16", "bf16")}
+
+    out = {}
+    t0 = time.perf_counter()
+    full = configs.shear_layer(steps=steps_full)
+    ss = sims(full)
+    del full
+    marks = sorted({m for m in (1000, 2000, 5000, 10000, steps_full) if m <= steps_full})
+    timeline, done = [], 0
+    for m in marks:
+        for s in ss.values():
+            s.step(m - done)
+        done = m
+        ms = {st: macros(s) for st, s in ss.items()}
+        timeline.append({"steps": m, "f16": _l2_fields(ms["f16"], ms["f32"]), "bf16": _l2_fields(ms["bf16"], ms["f32"])})
+    del ss, ms
+    torch.cuda.empty_cache()
+    out["shear_layer"] = {"W": 8192, "H": 2048, "nu": 1e-4, "vs": "FP32 (GPU)", "timeline": timeline}
+    log(f"precision: shear layer 8192x2048 in {time.perf_counter() - t0:.1f} s: {timeline}")
+    # Taylor-Green vortex (consistent pressure at t = 0): amplitude against exp(-nu (kx^2 + ky^2) t)
+    W, H, u0, nu = 8192, 2048, 0.04, 1e-4
+    ux, uy = geo.taylor_green_velocity(W, H, u0)
+    rho0 = geo.taylor_green_density(W, H, u0)
+    tg = configs.Case("taylor_green", W, H, float(np.float32(geo.nu_tau_inv(nu))), 0.0, "mrt", "periodic",
+                      geo.periodic_mask(W, H), np.zeros(H), geo.feq(rho0, ux, uy), steps_full)
+    kx, ky = 2 * np.pi / W, 2 * np.pi / H
+    nu_eff = (1.0 / tg.tau_inv - 0.5) / 3.0  # the FP32 rate the kernels use
+    want = np.exp(-nu_eff * (kx * kx + ky * ky) * steps_full)
+    norm = np.sum(ux * ux + uy * uy)
+    tgv = {"W": W, "H": H, "u0": u0, "nu": nu, "steps": steps_full, "analytic_decay": float(want)}
+    ss = sims(tg)
+    del tg
+    amps = {}
+    for st, s in ss.items():
+        s.step(steps_full)
+        m = macros(s)
+        amps[st] = float(np.sum(m[1] * ux + m[2] * uy) / norm)
+        tgv[st] = {"amplitude_ratio": amps[st], "rel_err_vs_analytic": abs(amps[st] / want - 1.0)}
+        if st != "f32":
+            tgv[st]["rel_err_vs_fp32"] = abs(amps[st] / amps["f32"] - 1.0)
+            tgv[st]["l2_u_vs_fp32"] = _l2_fields(m, mf32)["u"]
+        else:
+            mf32 = m
+    del ss, ux, uy, rho0, mf32
+    torch.cuda.empty_cache()
+    out["taylor_green"] = tgv
+    log(f"precision: Taylor-Green {tgv}")
+    # scaled shear layer against the FP64 oracle
+    import oracle as O
+    sc = configs.shear_layer(W=2048, H=512, steps=steps_scaled)
+    p = _lib_params(sc)
+    op = O.params(sc.W, sc.H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc, clamp=p.clamp)
+    f64 = np.array(sc.f0, dtype=np.float64, copy=True)
+    t1 = time.perf_counter()
+    par = O.aa_steps(op, sc.mask, f64, np.zeros(sc.H), 0, steps_scaled)
+    m64 = O.macro(op, sc.mask, f64, np.zeros(sc.H), par)
+    scaled = {"W": sc.W, "H": sc.H, "steps": steps_scaled, "vs": "FP64 oracle",
+              "oracle_seconds": round(time.perf_counter() - t1, 1)}
+    for st, s in sims(sc).items():
+        s.step(steps_scaled)
+        scaled[st] = _l2_fields(macros(s), m64)
+    out["shear_layer_scaled"] = scaled
+    log(f"precision: scaled shear layer vs FP64 {scaled}")
+    return out
+
+
+def _lib_params(case):
+    from lbm import _lib
+    return _lib.params(case.W, case.H, case.tau_inv, case.cs, _lib.MRT_LES if case.model == "mrt" else _lib.BGK,
+                       _lib.PERIODIC if case.bc == "periodic" else _lib.CHANNEL)
+
+
 def updated_cells(sim):
     """Cells a step updates (fluid, not wall rows, not the outlet ghost column)."""
     upd = sim.mask_host == 0
@@ -172,10 +270,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--l2-steps", type=int, default=200, help="steps for the L2 error against FP64 (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=("all", "aa", "fused", "ensemble", "closure"), default="all",
+    ap.add_argument("--only", choices=("all", "aa", "fused", "ensemble", "closure", "precision"), default="all",
                     help="run only the plain single step, the fused form or the ensemble (for profiling)")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="timed steps of the 64-lattice batched ensemble (0 = skip)")
+    ap.add_argument("--precision-steps", type=int, default=20000,
+                    help="steps of the 8192x2048 shear-layer / Taylor-Green precision study (0 = skip)")
     ap.add_argument("--closure-n", type=int, default=4096,
                     help="fine lattice size of the learned-closure study (coarse = n / 8; 0 = skip)")
     ap.add_argument("--copy-probe", default="", metavar="MIB[,MIB...]",
@@ -199,6 +299,9 @@ def main():
         return
     if args.only == "closure":
         print(json.dumps({"closure": bench_closure(args.closure_n or 4096)}), flush=True)
+        return
+    if args.only == "precision":
+        print(json.dumps({"precision": bench_precision(args.precision_steps or 20000)}), flush=True)
         return
     for mib in (int(v) for v in args.copy_probe.split(",") if v.strip()):
         src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device=dev)
@@ -355,6 +458,7 @@ def main():
 
     ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 and args.only == "all" else None
     closure = bench_closure(args.closure_n) if args.closure_n > 0 and args.only == "all" else None
+    precision = bench_precision(args.precision_steps) if args.precision_steps > 0 and args.only == "all" else None
 
     # the chosen form: the fastest one whose velocity L2 error against FP64 stays within 1e-3
     ok = {n: v for n, v in forms.items() if "glups" in v and v.get("l2_u", 0.0) <= 1e-3}
@@ -390,6 +494,7 @@ def main():
         "drag_lift": drag,
         "ensemble": ensemble,
         "closure": closure,
+        "precision": precision,
     }
     print(json.dumps(out), flush=True)
 

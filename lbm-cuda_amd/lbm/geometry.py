@@ -135,6 +135,18 @@ def taylor_green_velocity(W, H, u0, t=0.0, nu=0.0):
     return np.broadcast_to(ux, (H, W)).copy(), np.broadcast_to(uy, (H, W)).copy()
 
 
+def taylor_green_density(W, H, u0):
+    """rho = 1 + 3 p of the Taylor-Green pressure p = -(A^2 cos(2 kx x) + B^2 cos(2 ky y)) / 4, A = u0 ky / k,
+    B = u0 kx / k (the velocity of taylor_green_velocity); starting from it avoids the acoustic transient of a
+    uniform density."""
+    kx, ky = 2.0 * math.pi / W, 2.0 * math.pi / H
+    k = math.hypot(kx, ky)
+    a, b = u0 * ky / k, u0 * kx / k
+    x = np.arange(W, dtype=np.float64)[None, :]
+    y = np.arange(H, dtype=np.float64)[:, None]
+    return 1.0 - 0.75 * (a * a * np.cos(2 * kx * x) + b * b * np.cos(2 * ky * y))
+
+
 def random_phase_velocity(W, H, k0, u_rms, seed, power=4.0):
     """Solenoidal random-phase field from a stream function with E(k) ~ k^power exp(-2 (k/k0)^2),
     rescaled to the given u_rms (mean of ux^2 + uy^2 over the box, square-rooted)."""
