@@ -49,21 +49,25 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
                 ny = ny < 0 ? ny + P.H : (ny >= P.H ? ny - P.H : ny);
                 nb[q] = (unsigned)(ny * W + nx);
             } else {
-                nb[q] = idx + (unsigned)(cx(q) + cy(q) * W); // x = -1 only at the inlet, never used there
+                // x = -1 only at the inlet: the previous row's last cell, or past the end of the mask (reads
+                // 0); either way the result is discarded below
+                nb[q] = idx + (unsigned)(cx(q) + cy(q) * W);
             }
-            solid[q] = (!PERIODIC && inlet && cx(q) < 0) ? false : solid_at(nb[q]);
+            // all eight mask loads are issued unconditionally, so that they share one wait
+            const bool s_nb = solid_at(nb[q]);
+            solid[q] = s_nb && !(!PERIODIC && inlet && cx(q) < 0);
         }
+        // Bounce-back reads this cell's slot q, otherwise the source's slot opp(q).  The plane offset is part
+        // of the per-lane offset, so the choice is a select: a per-lane scalar offset would force a loop over
+        // its distinct values around every load.  At the inlet the q with c_qx > 0 arrive from x = -1; the
+        // value read for them (from an in-range or past-the-end offset, both harmless) is replaced by Zou-He,
+        // which never reads it.
         g[0] = S::ld(rf, idx * B, 0, 0);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
-            if (!PERIODIC && inlet && cx(q) > 0) {
-                g[q] = 0.0f - wq(q); // arrives from x = -1; rebuilt by Zou-He
-            } else if (solid[o]) {
-                g[q] = S::ld(rf, idx * B, (unsigned)q * plane * B, q); // bounce-back: this cell's slot q
-            } else {
-                g[q] = S::ld(rf, nb[o] * B, (unsigned)o * plane * B, q); // the source's slot opp(q)
-            }
+            const unsigned off = solid[o] ? (idx + (unsigned)q * plane) * B : (nb[o] + (unsigned)o * plane) * B;
+            g[q] = S::ld(rf, off, 0, q);
         }
     }
     collide_dev<MODEL, EXT>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac, w_ext);
@@ -75,14 +79,16 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
             for (int q = 0; q < 9; ++q) S::st(rf, (idx + 1) * B, (unsigned)opp(q) * plane * B, q, g[q]);
         }
     } else {
+        // Bounce-back writes this cell's slot opp(q), otherwise the target's slot q (again a select).  The
+        // populations leaving through the open inlet (c_qx < 0) are parked in this cell's slot opp(q): those
+        // are the canonical inlet slots q = 1, 5, 8, which no other cell reads or writes in this step and which
+        // the next step's Zou-He rebuilds without reading.
         S::st(rf, idx * B, 0, 0, g[0]);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
-            if (!PERIODIC && inlet && cx(q) < 0) continue; // leaves through the open inlet
-            if (solid[q])
-                S::st(rf, idx * B, (unsigned)opp(q) * plane * B, q, g[q]); // bounce-back into this cell
-            else
-                S::st(rf, nb[q] * B, (unsigned)q * plane * B, q, g[q]);
+            const bool back = solid[q] || (!PERIODIC && inlet && cx(q) < 0);
+            const unsigned off = back ? (idx + (unsigned)opp(q) * plane) * B : (nb[q] + (unsigned)q * plane) * B;
+            S::st(rf, off, 0, q, g[q]);
         }
         if (!PERIODIC && x == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
