@@ -33,7 +33,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         if (solid_at(idx)) return;
     }
     float g[9], mac[3];
-    unsigned nb[9];
+    unsigned nb[9], nbB[9], own[9];
     bool solid[9];
     if constexpr (PARITY == 0) {
 #pragma unroll
@@ -62,11 +62,23 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         // its distinct values around every load.  At the inlet the q with c_qx > 0 arrive from x = -1; the
         // value read for them (from an in-range or past-the-end offset, both harmless) is replaced by Zou-He,
         // which never reads it.
-        g[0] = S::ld(rf, idx * B, 0, 0);
+        // Every offset is this cell's byte offset plus a wave-uniform constant (kept in scalar registers); the
+        // bounce-back offsets own[k] serve the gather and the scatter alike.
+        const unsigned idxB = idx * B;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) own[k] = idxB + (unsigned)k * plane * B;
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            if constexpr (PERIODIC)
+                nbB[q] = nb[q] * B;
+            else
+                nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
+        }
+        g[0] = S::ld(rf, own[0], 0, 0);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
-            const unsigned off = solid[o] ? (idx + (unsigned)q * plane) * B : (nb[o] + (unsigned)o * plane) * B;
+            const unsigned off = solid[o] ? own[q] : nbB[o] + (unsigned)o * plane * B;
             g[q] = S::ld(rf, off, 0, q);
         }
     }
@@ -83,11 +95,11 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         // populations leaving through the open inlet (c_qx < 0) are parked in this cell's slot opp(q): those
         // are the canonical inlet slots q = 1, 5, 8, which no other cell reads or writes in this step and which
         // the next step's Zou-He rebuilds without reading.
-        S::st(rf, idx * B, 0, 0, g[0]);
+        S::st(rf, own[0], 0, 0, g[0]);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const bool back = solid[q] || (!PERIODIC && inlet && cx(q) < 0);
-            const unsigned off = back ? (idx + (unsigned)opp(q) * plane) * B : (nb[q] + (unsigned)q * plane) * B;
+            const unsigned off = back ? own[opp(q)] : nbB[q] + (unsigned)q * plane * B;
             S::st(rf, off, 0, q, g[q]);
         }
         if (!PERIODIC && x == W - 2) {

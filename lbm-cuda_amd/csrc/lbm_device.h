@@ -59,7 +59,8 @@ __device__ __forceinline__ float sqrt_rn_normal(float x)
     const float t = rdn <= 0.0f ? sdn : s;
     return rup > 0.0f ? sup : t;
 }
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+// clamp to [lo, hi] (lo <= hi) in one v_med3_f32; equal to min(max(v, lo), hi) for every non-NaN v
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
 
 // The collision works on deviations h_q = f_q - w_q from the rest state (rho = 1, u = 0).  The rest
 // state's moments (1, -2, 1, 0, ..., 0) are carried exactly and only the small remainders are rounded,
@@ -118,10 +119,9 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
             drc = clampf(drho, -1.0f, 0.5f); // rho in [0, 1.5]
         }
         rc = 1.0f + drc;
-        inv_rc = inv_rho; // 1/rc == 1/rho bit for bit unless the density clamp engaged
-        if (__any(drc != drho)) { // a wave-uniform branch, so the division is not evaluated speculatively
-            if (drc != drho) inv_rc = __fdiv_rn(1.0f, rc);
-        }
+        // 1/rc == 1/rho bit for bit unless the density clamp engaged, and then rc is 1.5 or 0 exactly:
+        // 1/1.5 rounds to 0x3F2AAAAB and 1/0 is +inf, so no second division is needed
+        inv_rc = drc == drho ? inv_rho : (drc > 0.0f ? __builtin_bit_cast(float, 0x3F2AAAABu) : __builtin_inff());
         les = P.les != 0;
     }
     mac[0] = rc;

@@ -347,6 +347,7 @@ def run_study(cfg: StudyConfig = StudyConfig(), device="cuda", log=print):
     xtr, ttr, wtr = prep(X[:-1], T[:-1], Wt[:-1])
     xva, tva, wva = prep(X[-1:], T[-1:], Wt[-1:])
     mlp = MLP(seed=cfg.seed, device=device)
+    y_init = mlp.forward(torch.from_numpy(xva).to(device)).cpu().numpy()
     rng = np.random.default_rng(cfg.seed)
     nb = len(xtr) // cfg.batch
     loss = torch.empty(1, dtype=torch.float32, device=device)
@@ -371,7 +372,8 @@ def run_study(cfg: StudyConfig = StudyConfig(), device="cuda", log=print):
 
     y_va = mlp.forward(torch.from_numpy(xva).to(device)).cpu().numpy()
     smag_va = smag_nu[-1] / nu_scale
-    apriori = {"val_wmse_mlp": wmse(y_va, tva, wva), "val_wmse_smagorinsky": wmse(smag_va, tva, wva),
+    apriori = {"val_wmse_mlp": wmse(y_va, tva, wva), "val_wmse_untrained": wmse(y_init, tva, wva),
+               "val_wmse_smagorinsky": wmse(smag_va, tva, wva),
                "val_wmse_zero": wmse(np.zeros_like(tva), tva, wva),
                "corr_mlp": float(np.corrcoef(y_va, tva)[0, 1]), "corr_smagorinsky": float(np.corrcoef(smag_va, tva)[0, 1])}
     log(f"closure study: trained {steps} steps in {train_s:.1f} s; a-priori {apriori}")

@@ -137,6 +137,9 @@ def test_study_pipeline_small(gpu):
     ap = rep["apriori"]
     assert all(np.isfinite(v) for v in ap.values())
     assert ap["val_wmse_mlp"] < ap["val_wmse_zero"]
+    print(f"\nheld-out weighted MSE: untrained {ap['val_wmse_untrained']:.4f} -> trained {ap['val_wmse_mlp']:.4f}; "
+          f"Smagorinsky {ap['val_wmse_smagorinsky']:.4f}, zero {ap['val_wmse_zero']:.4f}")
+    assert ap["val_wmse_mlp"] < ap["val_wmse_untrained"]  # what training on the GPU kernels gains
     for row in rep["aposteriori"]:
         assert all(np.isfinite(v) for v in row.values())
     assert 0.0 <= rep["fallback_fraction"] <= 1.0
