@@ -433,28 +433,41 @@ def main():
             log(f"L2 after {n2} steps, {name}: rho {er:.2e}, u {eu:.2e}")
             del s2
 
-    # ---------------- CPU baseline: the oracle's FP32 mirror on this host
+    # ---------------- CPU baseline: the oracle (FP32 mirror and FP64) on this host's cores, and on one core
     cpu = None
     if not args.no_cpu:
         import oracle as O
         p = sim.params
         op = O.params(W, H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc, clamp=p.clamp)
-        fc = np.ascontiguousarray(case.f0, dtype=np.float32)
-        prof = case.inlet_u.astype(np.float32)
-        cores = len(os.sched_getaffinity(0))
-        n, par = 0, 0
-        t1 = time.perf_counter()
-        while True:
-            O.aa_step(op, sim.mask_host, fc, prof, par)
-            par ^= 1
-            n += 1
-            el = time.perf_counter() - t1
-            if el >= args.cpu_seconds or n >= 400:
-                break
-        cpu = {"value": cells * n / el / 1e9, "unit": "GLUPS", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
-               "kind": "port",
-               "sample": f"oracle FP32 mirror (C, OpenMP), {n} AA steps of the full 4096x1024 lattice in {el:.1f} s"}
-        log(f"CPU baseline: {cpu['value'] * 1e3:.1f} MLUPS on {cpu['cores']} threads")
+
+        def cpu_rate(dtype, seconds, max_steps):
+            fc = np.ascontiguousarray(case.f0, dtype=dtype)
+            prof = case.inlet_u.astype(dtype)
+            n, par = 0, 0
+            t1 = time.perf_counter()
+            while True:
+                O.aa_step(op, sim.mask_host, fc, prof, par)
+                par ^= 1
+                n += 1
+                el = time.perf_counter() - t1
+                if el >= seconds or n >= max_steps:
+                    break
+            return cells * n / el / 1e9, n, el
+
+        threads = O.set_threads(0)
+        r32, n32, e32 = cpu_rate(np.float32, args.cpu_seconds, 400)
+        r64, n64, e64 = cpu_rate(np.float64, args.cpu_seconds / 2, 200)
+        O.set_threads(1)
+        r1, n1, e1s = cpu_rate(np.float32, args.cpu_seconds / 4, 50)
+        O.set_threads(0)
+        cpu = {"value": r32, "unit": "GLUPS", "cores": threads, "kind": "port",
+               "sample": f"oracle FP32 mirror (C, OpenMP over rows, {threads} threads), {n32} AA steps of the full "
+                         f"4096x1024 lattice in {e32:.1f} s",
+               "fp64": {"value": r64, "cores": threads, "sample": f"{n64} FP64 steps in {e64:.1f} s"},
+               "fp32_one_core": {"value": r1, "cores": 1, "sample": f"{n1} FP32 steps in {e1s:.1f} s"},
+               "host_cpus": len(os.sched_getaffinity(0))}
+        log(f"CPU baseline: FP32 {r32 * 1e3:.1f} MLUPS on {threads} threads, FP64 {r64 * 1e3:.1f} MLUPS, "
+            f"FP32 one core {r1 * 1e3:.1f} MLUPS")
 
     ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 and args.only == "all" else None
     closure = bench_closure(args.closure_n) if args.closure_n > 0 and args.only == "all" else None

@@ -76,6 +76,7 @@ def lib():
             getattr(L, "oracle_collide" + suf).argtypes = [P, vp, ctypes.c_int, real, vp]
             getattr(L, "oracle_aa_step_rate" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
             getattr(L, "oracle_neq" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_mrt_forward_f64.argtypes = [vp, vp]
         L.oracle_mrt_inverse_f64.argtypes = [vp, vp]
         L.oracle_diagnostics_f32.argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
@@ -120,6 +121,11 @@ def aa_steps(p, mask, f, inlet_u, parity, n):
         aa_step(p, mask, f, inlet_u, parity)
         parity ^= 1
     return parity
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops (n <= 0: default); returns the count in effect."""
+    return int(lib().oracle_set_threads(int(n)))
 
 
 def aa_step_rate(p, mask, f, inlet_u, parity, w_cell):

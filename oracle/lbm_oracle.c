@@ -310,3 +310,13 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
 }
 
 int oracle_abi_version(void) { return 1; }
+
+/* Threads of the OpenMP loops (<= 0: the runtime's default); returns the count now in effect. */
+#include <omp.h>
+int oracle_set_threads(int n)
+{
+    static int dflt = 0;
+    if (!dflt) dflt = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : dflt);
+    return omp_get_max_threads();
+}
