@@ -267,6 +267,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--k", type=int, default=8, help="steps per launch of the fused form")
+    ap.add_argument("--fused-cfg", default="", metavar="K,STRIP,CHUNK",
+                    help="time the fused form at this configuration instead of autotuning it")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--l2-steps", type=int, default=200, help="steps for the L2 error against FP64 (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -386,8 +388,12 @@ def main():
         ks = tuple(k for k in autotune.KS if k <= args.k)
         for storage in ("f32", "f16", "bf16"):
             sf = sim if storage == "f32" else make(storage)
-            best, table = autotune.tune_fused(sf, ks=ks, use_cache=False)
-            k, strip, chunk, _ = best
+            if args.fused_cfg:
+                k, strip, chunk = (int(v) for v in args.fused_cfg.split(","))
+                table = []
+            else:
+                best, table = autotune.tune_fused(sf, ks=ks, use_cache=False)
+                k, strip, chunk, _ = best
             nlaunch = max(1, args.steps // k)
             sf.step(2 * k, k=k, cfg=(strip, chunk))
             us = time_steps(lambda n, sf=sf, k=k, cfg=(strip, chunk): sf.step(n, k=k, cfg=cfg), nlaunch * k)

@@ -166,3 +166,64 @@ def random_phase_velocity(W, H, k0, u_rms, seed, power=4.0):
     uy = np.real(np.fft.ifft2(-1j * 2 * math.pi * KX / W * psi_k))
     scale = u_rms / math.sqrt(float(np.mean(ux * ux + uy * uy)))
     return ux * scale, uy * scale
+
+
+# ------------------------------------------------------------------------------------------------ image I/O
+def read_bmp(path):
+    """Pixels of an uncompressed 24- or 32-bit BMP as uint8 [rows, cols, 3] in B, G, R order, top row first
+    (the channel order cv2.imread returns)."""
+    import struct
+    data = open(path, "rb").read()
+    if data[:2] != b"BM":
+        raise ValueError(f"{path}: not a BMP file")
+    off = struct.unpack_from("<I", data, 10)[0]
+    width, height = struct.unpack_from("<ii", data, 18)
+    bpp, comp = struct.unpack_from("<HI", data, 28)
+    if bpp not in (24, 32) or comp not in (0, 3):
+        raise ValueError(f"{path}: only uncompressed 24/32-bit BMP is supported (bpp {bpp}, compression {comp})")
+    nb = bpp // 8
+    stride = (width * nb + 3) & ~3
+    rows = abs(height)
+    px = np.frombuffer(data, np.uint8, count=stride * rows, offset=off).reshape(rows, stride)
+    img = px[:, :width * nb].reshape(rows, width, nb)[:, :, :3]
+    return np.ascontiguousarray(img[::-1] if height > 0 else img)  # positive height: stored bottom-up
+
+
+def resize_linear(img, W, H):
+    """Bilinear resize to H x W with half-pixel centres (src = (dst + 1/2) scale - 1/2, edges clamped), rounded
+    to uint8 -- the geometry of cv2.resize's default INTER_LINEAR (its fixed-point weights are not reproduced)."""
+    h, w = img.shape[:2]
+    if (h, w) == (H, W):
+        return img.copy()
+
+    def axis(n_out, n_in):
+        s = (np.arange(n_out) + 0.5) * (n_in / n_out) - 0.5
+        s = np.clip(s, 0, n_in - 1)
+        i0 = np.floor(s).astype(int)
+        i1 = np.minimum(i0 + 1, n_in - 1)
+        return i0, i1, s - i0
+
+    y0, y1, fy = axis(H, h)
+    x0, x1, fx = axis(W, w)
+    a = img.astype(np.float64)
+    top = a[y0][:, x0] * (1 - fx)[None, :, None] + a[y0][:, x1] * fx[None, :, None]
+    bot = a[y1][:, x0] * (1 - fx)[None, :, None] + a[y1][:, x1] * fx[None, :, None]
+    out = top * (1 - fy)[:, None, None] + bot * fy[:, None, None]
+    return np.clip(np.rint(out), 0, 255).astype(np.uint8)
+
+
+def mask_from_image(path, W, H):
+    """Channel mask from an image, as the reference's load_mask_from_image (lbm_smemaa_main.py:53-68): resize
+    to W x H, solid where the green channel is 0, rows 0 and H-1 solid, columns 0 and W-1 open on rows 1..H-2."""
+    img = resize_linear(read_bmp(path), W, H)
+    mask = (img[:, :, 1] == 0).astype(np.uint8)
+    return finalize_channel_mask(mask)
+
+
+def write_ppm(path, rgba):
+    """Binary PPM (P6) of an [H, W, 4] or [H, W, 3] uint8 image (the lbm_visualize output), row 0 first."""
+    img = np.ascontiguousarray(np.asarray(rgba, np.uint8)[:, :, :3])
+    H, W = img.shape[:2]
+    with open(path, "wb") as fh:
+        fh.write(f"P6\n{W} {H}\n255\n".encode())
+        fh.write(img.tobytes())
