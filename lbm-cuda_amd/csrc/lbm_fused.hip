@@ -10,6 +10,23 @@ int launch_fused_fmt(const KPar &P, int model, int bc, int k, int strip, int par
                      const uint8_t *mask, const void *fin, void *fout, const float *inlet_u, int CH, float *ux,
                      float *uy, float *rho);
 
+// defined in lbm_pair.hip
+template <int FMT>
+int launch_pair_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, const uint8_t *mask, const void *fin,
+                    void *fout, const float *inlet_u, float *ux, float *uy, float *rho);
+
+// strip = 0 selects the two-step tile kernel (k = 2 from an even state, chunk = 8 or 16 tile rows); otherwise
+// the strip-sweep kernel
+template <int FMT>
+static int launch_any(const KPar &P, const lbm_params *p, int k, const lbm_fused_cfg &c, int parity, hipStream_t st,
+                      const uint8_t *mask, const void *fin, void *fout, const float *inlet_u, float *ux, float *uy,
+                      float *rho)
+{
+    if (c.strip == 0) return launch_pair_fmt<FMT>(P, p->model, p->bc, c.chunk, st, mask, fin, fout, inlet_u, ux, uy, rho);
+    return launch_fused_fmt<FMT>(P, p->model, p->bc, k, c.strip, parity, st, mask, fin, fout, inlet_u, c.chunk, ux,
+                                 uy, rho);
+}
+
 // Default launch configuration: 128-column strips, and the tallest chunk that still gives at least 512
 // workgroups (two per CU on 256 CUs) so the k-row warm-up at each chunk end stays small.
 static void default_cfg(int W, int H, int k, lbm_fused_cfg *c)
@@ -44,7 +61,13 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
         c = *cfg;
     else
         default_cfg(P.W, P.H, k, &c);
-    if (c.strip != 64 && c.strip != 128) return fail(LBM_ERR_ARG, "lbm_step_fused: strip must be 64 or 128");
+    if (c.strip == 0) {
+        if (k != 2 || parity != 0)
+            return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: the two-step tile kernel (strip 0) needs k = 2 and parity 0");
+        if (c.chunk != 8 && c.chunk != 16) return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 needs chunk 8 or 16");
+        return LBM_OK;
+    }
+    if (c.strip != 64 && c.strip != 128) return fail(LBM_ERR_ARG, "lbm_step_fused: strip must be 0, 64 or 128");
     if (c.chunk < 1) return fail(LBM_ERR_ARG, "lbm_step_fused: chunk must be >= 1");
     if (c.strip - 2 * (k + 1) < 16) return fail(LBM_ERR_ARG, "lbm_step_fused: strip %d too narrow for k=%d", c.strip, k);
     if (c.chunk > P.H) c.chunk = P.H;
@@ -74,8 +97,7 @@ int lbm_step_fused_cfg(const lbm_params *p, const uint8_t *mask, const float *f_
     lbm_fused_cfg c;
     int rc = fused_common(p, mask, f_now, f_out, 4, inlet_u, parity, k, cfg, ux, uy, rho, P, c);
     if (rc) return rc;
-    rc = launch_fused_fmt<FMT_F32>(P, p->model, p->bc, k, c.strip, parity, to_hip(stream), mask, f_now, f_out,
-                                   inlet_u, c.chunk, ux, uy, rho);
+    rc = launch_any<FMT_F32>(P, p, k, c, parity, to_hip(stream), mask, f_now, f_out, inlet_u, ux, uy, rho);
     if (rc) return rc;
     return check_launch("lbm_step_fused");
 }
@@ -95,13 +117,11 @@ int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t 
     lbm_fused_cfg c;
     int rc = fused_common(p, mask, h_now, h_out, 2, inlet_u, parity, k, cfg, ux, uy, rho, P, c);
     if (rc) return rc;
-    c.strip = 128; // the 16-bit kernels are built for 128-column strips only
+    if (c.strip != 0) c.strip = 128; // the 16-bit strip-sweep kernels are built for 128-column strips only
     if (fmt == LBM_FMT_F16)
-        rc = launch_fused_fmt<FMT_F16>(P, p->model, p->bc, k, c.strip, parity, to_hip(stream), mask, h_now, h_out,
-                                       inlet_u, c.chunk, ux, uy, rho);
+        rc = launch_any<FMT_F16>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);
     else
-        rc = launch_fused_fmt<FMT_BF16>(P, p->model, p->bc, k, c.strip, parity, to_hip(stream), mask, h_now, h_out,
-                                        inlet_u, c.chunk, ux, uy, rho);
+        rc = launch_any<FMT_BF16>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);
     if (rc) return rc;
     return check_launch("lbm_step_fused_h16");
 }

@@ -3,7 +3,8 @@
 The reference fixes its tile (SQUARE_A = 24) and T at compile time and mirrors them by hand in the driver
 (lbm_smemaa_main.py:93-95) -- its compile-time traversal-order table (lbm_core_smemaa_policy.cu:12-24) is
 the only tuning knob.  Here every candidate is timed on the GPU with HIP events on scratch copies of the
-state (the simulation itself is not advanced) and the winner is cached per lattice size class.
+state (the simulation itself is not advanced) and the winner is cached per lattice size class.  For k = 2
+from an even state the candidates include the two-step tile kernel (strip 0).
 """
 from __future__ import annotations
 
@@ -84,8 +85,10 @@ def time_config(sim, k, cfg, reps=6, bufs=None):
 
 
 def candidates(sim, k):
+    """(strip, chunk) pairs to time: strip-sweep shapes, plus the two-step tile kernel (strip 0, chunk = tile
+    rows) when k = 2 and the state is even."""
     H = sim.params.H
-    out = []
+    out = [(0, 8), (0, 16)] if k == 2 and sim.parity == 0 else []
     for strip in (STRIPS if sim.fmt == _lib.FMT_F32 else (128,)):
         if strip - 2 * (k + 1) < 16:
             continue
