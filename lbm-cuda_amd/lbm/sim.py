@@ -23,6 +23,10 @@ _BCS = {"channel": _lib.CHANNEL, "periodic": _lib.PERIODIC}
 _FMTS = {"f32": _lib.FMT_F32, "f16": _lib.FMT_F16, "bf16": _lib.FMT_BF16}
 
 
+class LbmHealthError(RuntimeError):
+    """A state that failed check_health (NaN/inf, negative density, mass drift)."""
+
+
 def _ptr(t):
     if t is None:
         return None
@@ -172,6 +176,29 @@ class Simulation:
                                       self.parity, _ptr(self._diag_work), _ptr(self._diag_out), _stream())
         _lib.check(rc, "lbm_diagnostics")
         return self._diag_out.cpu().numpy().copy()
+
+    def check_health(self, mass_ref=None, rel_tol=1e-3):
+        """Failure detection: raise LbmHealthError when the state holds a non-finite value, a negative density,
+        or (given mass_ref) a mass that drifted by more than rel_tol; otherwise return the figures checked.
+        Mass and momentum come from the FP64 reduction on the GPU, the density minimum from lbm_macro."""
+        d = self.diagnostics()
+        if not np.all(np.isfinite(d)):
+            raise LbmHealthError(f"non-finite state after {self.steps} steps (mass {d[0]}, momentum {d[1:3]})")
+        rho = self.macro()[0].cpu().numpy()
+        upd = self.mask_host == 0
+        if self.bc == _lib.CHANNEL:
+            upd = upd.copy()
+            upd[0, :] = upd[-1, :] = False
+            upd[:, -1] = False
+        rho_min = float(rho[upd].min()) if upd.any() else 1.0
+        if not np.isfinite(rho_min) or rho_min < 0.0:
+            raise LbmHealthError(f"negative or non-finite density {rho_min} after {self.steps} steps")
+        drift = None
+        if mass_ref is not None:
+            drift = abs(d[0] - mass_ref) / abs(mass_ref)
+            if drift > rel_tol:
+                raise LbmHealthError(f"mass drifted by {drift:.3e} (> {rel_tol}) after {self.steps} steps")
+        return {"mass": float(d[0]), "momentum": (float(d[1]), float(d[2])), "rho_min": rho_min, "mass_drift": drift}
 
     def drag_lift(self, u_ref, length, rho_ref=1.0):
         """(Cd, Cl) = 2 F / (rho_ref u_ref^2 length) from the momentum-exchange force."""

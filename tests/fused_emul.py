@@ -241,3 +241,86 @@ def _block(fin, fout, mask, inlet_u, S, CH, bx, by, p, periodic):
                         fout[:, y, W - 1] = sf[:, sl, c2]
                     elif c2 + 1 < LW:
                         fout[:, y, W - 1] = sf[:, sl, c2 + 1]
+
+
+def pair(f_in, mask, inlet_u, p, periodic, CH=8):
+    """k_pair (lbm-cuda_amd/csrc/lbm_pair.hip) on the whole grid: per 128 x CH tile, phase A (even step of the
+    130 x (CH + 2) region into an LDS image) and phase B (odd step of the core, scattered to the output).
+    Returns f_out (slots not written keep NaN); every LDS and lattice index is bounds-checked."""
+    _, H, W = f_in.shape
+    CWp, RW, RH = 128, 130, CH + 2
+    out = np.full_like(f_in, np.nan)
+    written = np.zeros(f_in.shape, np.int32)
+    for bx in range((W + CWp - 1) // CWp):
+        for by in range((H + CH - 1) // CH):
+            x0, y0 = bx * CWp, by * CH
+            sf = np.full((9, RH, RW), np.nan)
+            sm = np.ones((RH, RW), bool)
+            for r in range(RH):
+                for c in range(RW):
+                    gx, gy = x0 - 1 + c, y0 - 1 + r
+                    if periodic:
+                        gx, gy, inside = gx % W, gy % H, True
+                    else:
+                        inside = 0 <= gx < W and 0 <= gy < H
+                    solid = bool(mask[gy, gx]) if inside else True
+                    sm[r, c] = solid
+                    if periodic:
+                        comp, inlet = not solid, False
+                    else:
+                        inlet = gx == 0
+                        comp = inside and 1 <= gy <= H - 2 and gx <= W - 2 and (inlet or not solid)
+                    if not comp:
+                        continue
+                    g = f_in[:, gy, gx].astype(np.float64)[:, None].copy()
+                    o = _collide(g, np.array([inlet]), np.array([inlet_u[gy] if inlet else 0.0]), p)[:, 0]
+                    sf[OPP, r, c] = o
+                    if not periodic and gx == W - 2 and c + 1 < RW:
+                        sf[OPP, r, c + 1] = o
+            for r in range(1, CH + 1):
+                for c in range(1, CWp + 1):
+                    gx, gy = x0 + c - 1, y0 + r - 1
+                    if gx >= W or gy >= H:
+                        continue
+                    inlet = False
+                    if not periodic:
+                        if gy == 0 or gy == H - 1 or gx == W - 1:
+                            continue
+                        inlet = gx == 0
+                        if not inlet and sm[r, c]:
+                            continue
+                    elif sm[r, c]:
+                        continue
+                    solid = [False] + [bool(sm[r + CY[q], c + CX[q]]) and not (not periodic and inlet and CX[q] < 0)
+                                       for q in range(1, 9)]
+                    g = np.empty(9)
+                    g[0] = sf[0, r, c]
+                    for q in range(1, 9):
+                        o = OPP[q]
+                        rr, cc = (r, c) if solid[o] else (r - CY[q], c - CX[q])
+                        assert 0 <= rr < RH and 0 <= cc < RW
+                        g[q] = sf[q if solid[o] else o, rr, cc]
+                    if inlet:
+                        g[[1, 5, 8]] = 0.0  # arrive from x = -1: rebuilt by Zou-He, never read
+                    o9 = _collide(g[:, None], np.array([inlet]), np.array([inlet_u[gy] if inlet else 0.0]), p)[:, 0]
+
+                    def put(q, y, x, v):
+                        assert 0 <= y < H and 0 <= x < W
+                        out[q, y, x] = v
+                        written[q, y, x] += 1
+
+                    put(0, gy, gx, o9[0])
+                    for q in range(1, 9):
+                        back = solid[q] or (not periodic and inlet and CX[q] < 0)
+                        if back:
+                            put(OPP[q], gy, gx, o9[q])
+                        else:
+                            ty, tx = (gy + CY[q]) % H, (gx + CX[q]) % W
+                            put(q, ty, tx, o9[q])
+                    if not periodic and gx == W - 2:
+                        put(3, gy, gx, o9[3])
+                        if not solid[2]:
+                            put(6, gy + 1, gx, o9[6])
+                        if not solid[4]:
+                            put(7, gy - 1, gx, o9[7])
+    return out, written

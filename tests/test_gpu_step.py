@@ -234,3 +234,27 @@ def test_aa_step_bitwise_clamps_engaged(gpu, storage):
             sel = O.observable(p, sim.mask_host, par ^ 1)
             assert np.array_equal(got[sel], ref[sel]), f"step {n}"
             h_prev, par = got.copy(), par ^ 1
+
+
+def test_health_check_detects_injected_failures(gpu):
+    """Failure detection, forced from the host: a NaN, a negative density and a mass drift injected into a
+    healthy state are each reported; the healthy state passes."""
+    import torch
+    from lbm.sim import LbmHealthError
+    sim = _small_periodic()
+    mass0 = sim.diagnostics()[0]
+    sim.step(10)
+    ok = sim.check_health(mass0, rel_tol=1e-6)
+    assert ok["mass_drift"] < 1e-6 and ok["rho_min"] > 0.5
+    for inject, what in (("nan", "non-finite"), ("neg", "negative"), ("scale", "drift")):
+        s = _small_periodic()
+        s.step(4)
+        with torch.no_grad():
+            if inject == "nan":
+                s.f[3, 10, 11] = float("nan")
+            elif inject == "neg":
+                s.f[:, 10, 11] = -0.5
+            else:
+                s.f.mul_(1.01)
+        with pytest.raises(LbmHealthError, match=what):
+            s.check_health(mass0, rel_tol=1e-3)

@@ -1,0 +1,81 @@
+"""Property-based checks of the oracle (hypothesis): on random small lattices, masks and states the AA
+in-place step equals the two-buffer pull step bit for bit, periodic runs conserve mass and momentum, and the
+collision conserves both for any model; the MRT transform round-trips."""
+import numpy as np
+from hypothesis import HealthCheck, given, settings
+from hypothesis import strategies as st
+
+import oracle as O
+from lbm import geometry as G
+
+SETTINGS = settings(max_examples=25, deadline=None, suppress_health_check=[HealthCheck.function_scoped_fixture])
+
+
+def _state(W, H, seed, solid_frac, u):
+    rng = np.random.default_rng(seed)
+    mask = (rng.random((H, W)) < solid_frac).astype(np.uint8)
+    rho = 1.0 + 0.02 * rng.standard_normal((H, W))
+    ux = u * rng.standard_normal((H, W))
+    uy = u * rng.standard_normal((H, W))
+    return mask, G.feq(rho, ux, uy)
+
+
+@SETTINGS
+@given(W=st.integers(3, 12), H=st.integers(3, 10), seed=st.integers(0, 10 ** 6), solid=st.floats(0.0, 0.4),
+       model=st.sampled_from([O.MRT, O.BGK]), cs=st.sampled_from([0.0, 0.15]))
+def test_aa_equals_pull_random_periodic(oracle_lib, W, H, seed, solid, model, cs):
+    """With P_0 = K(C_0) (one even AA step), the pull state P_n equals the reversed AA state after every
+    even step, bit for bit (FP32 mirror)."""
+    mask, f0 = _state(W, H, seed, solid, 0.03)
+    p = O.params(W, H, 1.6, cs=cs if model == O.MRT else 0.0, model=model, bc=O.PERIODIC)
+    zero = np.zeros(H, np.float32)
+    C = f0.astype(np.float32)
+    O.aa_step(p, mask, C, zero, 0)
+    P = O.reverse(C).copy()
+    par = 1
+    sel = O.observable(p, mask, 1)
+    for n in range(1, 7):
+        O.aa_step(p, mask, C, zero, par)
+        par ^= 1
+        P2 = P.copy()
+        O.pull_step(p, mask, P, P2, zero)
+        P = P2
+        if par == 1:
+            assert np.array_equal(O.reverse(C)[sel], P[sel]), f"step {n}"
+
+
+@SETTINGS
+@given(W=st.integers(3, 12), H=st.integers(3, 10), seed=st.integers(0, 10 ** 6), solid=st.floats(0.0, 0.4),
+       model=st.sampled_from([O.MRT, O.BGK]))
+def test_periodic_conservation(oracle_lib, W, H, seed, solid, model):
+    mask, f0 = _state(W, H, seed, solid, 0.02)
+    op = O.params(W, H, 1.7, cs=0.1 if model == O.MRT else 0.0, model=model, bc=O.PERIODIC)
+    f = f0.copy()
+    fluid = mask == 0
+    m0 = f[:, fluid].sum()
+    O.aa_steps(op, mask, f, np.zeros(H), 0, 6)
+    assert abs(f[:, fluid].sum() - m0) < 1e-12 * max(1.0, m0)
+
+
+@SETTINGS
+@given(rho=st.floats(0.5, 1.4), ux=st.floats(-0.15, 0.15), uy=st.floats(-0.15, 0.15),
+       noise=st.floats(0.0, 0.01), seed=st.integers(0, 10 ** 6), model=st.sampled_from([O.MRT, O.BGK]),
+       cs=st.sampled_from([0.0, 0.1, 0.2]))
+def test_collision_conserves_mass_and_momentum(oracle_lib, rho, ux, uy, noise, seed, model, cs):
+    rng = np.random.default_rng(seed)
+    f = O.feq(rho, ux, uy).reshape(9) * (1.0 + noise * rng.standard_normal(9))
+    p = O.params(8, 8, 1.5, cs=cs if model == O.MRT else 0.0, model=model)
+    g, _ = O.collide(p, f.copy())
+    cx = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+    cy = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+    # the clamps (|u| <= 0.4, rho in [0, 1.5], f in [0, 10]) are inactive here
+    assert abs(g.sum() - f.sum()) < 1e-13
+    assert abs((g * cx).sum() - (f * cx).sum()) < 1e-13
+    assert abs((g * cy).sum() - (f * cy).sum()) < 1e-13
+
+
+@SETTINGS
+@given(seed=st.integers(0, 10 ** 6))
+def test_mrt_transform_round_trip(oracle_lib, seed):
+    f = np.random.default_rng(seed).standard_normal(9)
+    assert np.allclose(O.mrt_inverse(O.mrt_forward(f)), f, atol=1e-13)
