@@ -130,7 +130,7 @@ int lbm_fused_max_k(void);
 /* Launch configuration of the k-step kernel (no reference counterpart: the reference fixes its tile and
  * T at compile time, lbm_core_smemaa.cu:8-9).  strip = LDS columns per workgroup including the halo
  * (64 or 128); chunk = rows per workgroup.  strip = 0 selects the two-step tile kernel instead (k = 2 from
- * parity 0 only; chunk = 8 or 16 core rows per 128-column tile). */
+ * parity 0 only; chunk = 8, 12, 16 or 24 core rows per 128-column tile). */
 typedef struct lbm_fused_cfg {
     int32_t strip;
     int32_t chunk;

@@ -64,7 +64,8 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
     if (c.strip == 0) {
         if (k != 2 || parity != 0)
             return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: the two-step tile kernel (strip 0) needs k = 2 and parity 0");
-        if (c.chunk != 8 && c.chunk != 16) return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 needs chunk 8 or 16");
+        if (c.chunk != 8 && c.chunk != 12 && c.chunk != 16 && c.chunk != 24)
+            return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 needs chunk 8, 12, 16 or 24");
         return LBM_OK;
     }
     if (c.strip != 64 && c.strip != 128) return fail(LBM_ERR_ARG, "lbm_step_fused: strip must be 0, 64 or 128");

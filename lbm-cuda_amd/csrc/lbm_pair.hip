@@ -1,6 +1,6 @@
 // Two-step tile kernel: one even (A) and one odd (B) AA step in one launch, bit-identical to two launches of
 // the single-step kernel, for lbm_step_fused / lbm_step_fused_h16 with k = 2 from an even state (selected by a
-// launch configuration with strip = 0; chunk = tile rows).
+// launch configuration with strip = 0; chunk = tile rows, 8, 12, 16 or 24).
 //
 // The reference fuses T steps in square shared-memory tiles (lbm_core_smemaa.cu:29-71,107-223,293-445).  Here
 // a workgroup owns a core of 128 x CH cells:
@@ -168,17 +168,28 @@ template <int MODEL, bool PERIODIC, int FMT>
 static void launch_ch(const KPar &P, int CH, dim3 grid, hipStream_t st, const uint8_t *mask, const void *fin,
                       void *fout, const float *inlet_u, float *ux, float *uy, float *rho)
 {
-    if (CH == 16)
+    switch (CH) {
+    case 24:
+        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 24>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
+                           uy, rho);
+        break;
+    case 16:
         hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 16>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
                            uy, rho);
-    else
+        break;
+    case 12:
+        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 12>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
+                           uy, rho);
+        break;
+    default:
         hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 8>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
                            uy, rho);
+    }
 }
 
 } // namespace pairk
 
-// Launches the two-step tile kernel (k = 2 from an even state; CH = 8 or 16 core rows per tile).
+// Launches the two-step tile kernel (k = 2 from an even state; CH = 8, 12, 16 or 24 core rows per tile).
 template <int FMT>
 int launch_pair_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, const uint8_t *mask, const void *fin,
                     void *fout, const float *inlet_u, float *ux, float *uy, float *rho)

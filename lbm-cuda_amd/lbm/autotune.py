@@ -88,7 +88,7 @@ def candidates(sim, k):
     """(strip, chunk) pairs to time: strip-sweep shapes, plus the two-step tile kernel (strip 0, chunk = tile
     rows) when k = 2 and the state is even."""
     H = sim.params.H
-    out = [(0, 8), (0, 16)] if k == 2 and sim.parity == 0 else []
+    out = [(0, ch) for ch in (8, 12, 16, 24)] if k == 2 and sim.parity == 0 else []
     for strip in (STRIPS if sim.fmt == _lib.FMT_F32 else (128,)):
         if strip - 2 * (k + 1) < 16:
             continue

@@ -108,7 +108,7 @@ def test_fused_equals_single_cylinder_large(gpu, k):
     assert np.isfinite(d).all()
 
 
-@pytest.mark.parametrize("chunk", [8, 16])
+@pytest.mark.parametrize("chunk", [8, 12, 16, 24])
 @pytest.mark.parametrize("periodic,model", [(False, "mrt"), (True, "mrt"), (False, "bgk"), (True, "bgk")])
 def test_pair_kernel_equals_two_single_steps(gpu, chunk, periodic, model):
     """The two-step tile kernel (strip 0) equals two single steps bit for bit, on ragged sizes and on
@@ -155,4 +155,4 @@ def test_pair_kernel_macros_and_limits(gpu):
     with pytest.raises(_lib.LbmError):
         fusedsim.step(4, k=4, cfg=(0, 8))
     with pytest.raises(_lib.LbmError):
-        fusedsim.step(2, k=2, cfg=(0, 12))
+        fusedsim.step(2, k=2, cfg=(0, 10))
