@@ -42,9 +42,11 @@ def log(*a):
 
 
 def bench_ensemble(nsteps, graph_steps=20):
-    """64 periodic lattices of uneven sizes: one persistent launch per step per cache group, replayed from a
-    captured graph of `graph_steps` steps, vs the same steps as 64 separate launches per step (captured
-    step-major and lattice-major, and issued eagerly)."""
+    """64 periodic lattices of uneven sizes: one persistent launch per step (k = 1) or per two steps (k = 2,
+    the two-step tile kernel), per cache group, replayed from a captured graph of `graph_steps` steps, vs the
+    same steps as 64 separate launches per step (captured step-major and lattice-major, and issued eagerly).
+    The tuner picks k, tile rows, grid and grouping; `gbs_at_72B` prices each step at the single step's
+    72 bytes per cell, whatever k."""
     import torch
 
     from lbm import configs
@@ -52,11 +54,17 @@ def bench_ensemble(nsteps, graph_steps=20):
 
     members = configs.ensemble()
     t0 = time.perf_counter()
-    ens = Ensemble.from_members(members)
+    ens = Ensemble.from_members(members, two_step=True)
     cells = ens.cells
     log(f"ensemble: {len(members)} lattices, {cells / 1e6:.1f} M cells, built in {time.perf_counter() - t0:.1f} s")
-    blocks, group, tuned_us = tune_ensemble(ens, use_cache=False, steps=graph_steps, reps=2)
-    log(f"ensemble tuned: {blocks} workgroups, groups of <= {group >> 20} MB ({tuned_us:.1f} us/step)")
+    blocks, group, tuned_us, k, rows = tune_ensemble(ens, use_cache=False, steps=graph_steps, reps=2)
+    log(f"ensemble tuned: k={k} (tile rows {rows}), {blocks} workgroups, groups of <= {group >> 20} MB "
+        f"({tuned_us:.1f} us/step)")
+    best_by_k = {}
+    for kk, ch, b, gmb, us in ens.tune_table:
+        if kk not in best_by_k or us < best_by_k[kk]["us_per_step"]:
+            best_by_k[kk] = {"rows": ch or None, "blocks": b, "group_mb": gmb, "us_per_step": round(us, 2),
+                             "glups": round(cells / us / 1e3, 3)}
     reps = max(1, nsteps // graph_steps)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
@@ -100,11 +108,12 @@ def bench_ensemble(nsteps, graph_steps=20):
         "members": len(members), "cells": cells, "steps_timed": reps * graph_steps,
         "steps_per_graph": graph_steps,
         "sizes": "W, H each from {256,384,512,768,1024,1536,2048}; nu log-uniform [1e-4, 1e-2]; seed 4",
-        "blocks": blocks, "group_mb": group >> 20,
+        "k": k, "tile_rows": rows if k == 2 else None, "blocks": blocks, "group_mb": group >> 20,
+        "tuner_best_per_k": {f"k{kk}": v for kk, v in sorted(best_by_k.items())},
         "persistent_graph": {"glups": rate(us_p), "us_per_step": round(us_p, 2),
-                             "hbm_gbs_if_uncached": round(BYTES_PER_CELL_FP32 * cells / us_p / 1e3, 1)},
+                             "gbs_at_72B": round(BYTES_PER_CELL_FP32 * cells / us_p / 1e3, 1)},
         "persistent_graph_ungrouped": {"glups": rate(us_1), "us_per_step": round(us_1, 2),
-                                       "achieved_gbs": round(BYTES_PER_CELL_FP32 * cells / us_1 / 1e3, 1)},
+                                       "gbs_at_72B": round(BYTES_PER_CELL_FP32 * cells / us_1 / 1e3, 1)},
         "separate_graph_step_major": {"glups": rate(us_s), "us_per_step": round(us_s, 2),
                                       "launches_per_step": len(members)},
         "separate_graph_lattice_major": {"glups": rate(us_l), "us_per_step": round(us_l, 2),

@@ -208,6 +208,7 @@ typedef struct lbm_instance {
     const uint8_t *mask; /* W*H bytes */
     void *f;             /* 9*W*H FP32 populations, or 16-bit deviations for LBM_FMT_F16 / LBM_FMT_BF16 */
     const float *inlet_u; /* H floats (channel bc; may be NULL when periodic) */
+    void *f_alt;         /* a second state buffer of the same size for the two-step path (NULL: k = 1 only) */
 } lbm_instance;
 typedef struct lbm_ensemble lbm_ensemble;
 int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble **out);
@@ -223,6 +224,18 @@ int lbm_ensemble_step(lbm_ensemble *e, int parity, int blocks, lbm_stream_t stre
  */
 int lbm_ensemble_step_n(lbm_ensemble *e, int parity, int nsteps, long long group_bytes, int blocks,
                         lbm_stream_t stream);
+/*
+ * npairs two-step launches per cache group (even then odd step, the update of the two-step tile kernel of
+ * lbm_step_fused with strip 0, bit for bit) from an even state; each launch moves every member from one of its
+ * buffers into the other.  chunk = tile rows (8, 12, 16 or 24).  Every member needs f_alt.
+ * lbm_ensemble_side reports which buffer holds the state (0: f, 1: f_alt); lbm_ensemble_step(_n) update that
+ * buffer in place.
+ */
+int lbm_ensemble_pair_n(lbm_ensemble *e, int npairs, long long group_bytes, int chunk, int blocks,
+                        lbm_stream_t stream);
+int lbm_ensemble_side(const lbm_ensemble *e);
+/* tiles of the two-step list with `chunk` rows, and its default grid (resident workgroups) */
+int lbm_ensemble_pair_info(lbm_ensemble *e, int chunk, int *total_tiles, int *default_blocks);
 int lbm_ensemble_destroy(lbm_ensemble *e);
 
 /*

@@ -1,0 +1,164 @@
+// The two-step tile update shared by the single-lattice kernel (lbm_pair.hip) and the batched ensemble
+// (lbm_ensemble.hip): one even (A) and one odd (B) AA step of a 128 x CH tile, bit-identical to two single
+// steps.  Phase A performs the even step of the tile plus a one-cell halo ring from HBM into LDS; phase B the
+// odd step of the core, gathered from LDS and scattered straight to the output (every output slot has one
+// writer).  The caller owns the LDS arrays and calls it from every thread of a 512-thread workgroup.
+#pragma once
+
+#include "lbm_common.h"
+
+namespace lbm {
+namespace pairk {
+
+constexpr int CW = 128;    // core columns per tile
+constexpr int RW = CW + 2; // region columns (one halo column on each side)
+constexpr int LS = 132;    // LDS row stride (elements)
+constexpr int NT = 512;    // threads per workgroup
+
+// v mod n for v >= -n (one period at most below the lattice; region columns may lie several periods above
+// it when the lattice is narrower than the region)
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v % n : v); }
+
+template <int FMT, int CH> struct TileLds {
+    typename Conv<FMT>::T sf[9][CH + 2][LS];
+    uint8_t sm[CH + 2][LS]; // 1 = solid or outside the lattice
+};
+
+template <int MODEL, bool PERIODIC, int FMT, int CH>
+__device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restrict__ mask, const void *__restrict__ fin,
+                                          void *__restrict__ fout, const float *__restrict__ inlet_u,
+                                          float *__restrict__ oux, float *__restrict__ ouy, float *__restrict__ orho,
+                                          int bx, int by, TileLds<FMT, CH> &L)
+{
+    using S = Store<FMT>;
+    using C = Conv<FMT>;
+    using T = typename C::T;
+    constexpr unsigned B = S::BYTES;
+    constexpr int RH = CH + 2;
+    auto &sf = L.sf;
+    auto &sm = L.sm;
+
+    const int W = P.W, H = P.H;
+    const unsigned plane = (unsigned)P.plane;
+    const int x0 = bx * CW, y0 = by * CH;
+    const rsrc_t rin = make_rsrc(fin, 9u * plane * B), rout = make_rsrc(fout, 9u * plane * B);
+    const rsrc_t rm = make_rsrc(mask, plane);
+    const int t = threadIdx.x;
+
+    // ---- phase A: even step of every region cell, reversed result into LDS.  All of a thread's HBM loads
+    //      (NA cells x 9 populations) are issued before the first collision, so their latencies overlap.
+    constexpr int NA = (RH * RW + NT - 1) / NT;
+    float ga[NA][9];
+    unsigned ia[NA];
+    bool ca[NA], la[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        const int e = t + a * NT;
+        const int r = e / RW, c = e - r * RW;
+        int gx = x0 - 1 + c, gy = y0 - 1 + r;
+        bool in = e < RH * RW;
+        if constexpr (PERIODIC) {
+            gx = wrap(gx, W);
+            gy = wrap(gy, H);
+        } else {
+            in = in && gx >= 0 && gx < W && gy >= 0 && gy < H;
+        }
+        const unsigned idx = in ? (unsigned)(gy * W + gx) : 0u;
+        const bool solid = in ? __builtin_amdgcn_raw_buffer_load_b8(rm, idx, 0, 0) != 0 : true;
+        if (e < RH * RW) sm[r][c] = solid ? 1 : 0;
+        bool comp, inlet = false;
+        if constexpr (PERIODIC) {
+            comp = in && !solid;
+        } else {
+            inlet = gx == 0;
+            comp = in && gy >= 1 && gy <= H - 2 && gx <= W - 2 && (inlet || !solid);
+        }
+        ca[a] = comp;
+        la[a] = inlet;
+        ia[a] = (unsigned)gy; // the row, for the inlet speed
+        // unconditional (cells that are not updated read cell 0 and discard it), so no load waits in a branch
+#pragma unroll
+        for (int q = 0; q < 9; ++q) ga[a][q] = S::ld(rin, idx * B, (unsigned)q * plane * B, q);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        if (!ca[a]) continue;
+        const int e = t + a * NT;
+        const int r = e / RW, c = e - r * RW;
+        const bool inlet = la[a];
+        float mac[3];
+        collide_dev<MODEL>(P, ga[a], inlet, inlet ? inlet_u[ia[a]] : 0.0f, mac);
+        T o[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) o[q] = C::st(ga[a][q], q);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) sf[opp(q)][r][c] = o[q];
+        if (!PERIODIC && x0 - 1 + c == W - 2 && c + 1 < RW) { // reversed copy into the outlet ghost
+#pragma unroll
+            for (int q = 0; q < 9; ++q) sf[opp(q)][r][c + 1] = o[q];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase B: odd step of the core cells, scattered to the output ------------------------------------
+    for (int e = t; e < CH * CW; e += NT) {
+        const int r = 1 + e / CW, c = 1 + (e - (e / CW) * CW);
+        const int gx = x0 + c - 1, gy = y0 + r - 1;
+        if (gx >= W || gy >= H) continue;
+        bool inlet = false;
+        if constexpr (!PERIODIC) {
+            if (gy == 0 || gy == H - 1 || gx == W - 1) continue; // walls and the outlet ghost are not updated
+            inlet = gx == 0;
+            if (!inlet && sm[r][c]) continue;
+        } else {
+            if (sm[r][c]) continue;
+        }
+        bool solid[9];
+#pragma unroll
+        for (int q = 1; q < 9; ++q) solid[q] = sm[r + cy(q)][c + cx(q)] && !(!PERIODIC && inlet && cx(q) < 0);
+        float g[9], mac[3];
+        g[0] = C::ld(sf[0][r][c], 0);
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            const int o = opp(q);
+            // bounce-back: this cell's slot q; otherwise the source's slot opp(q).  At the inlet the q with
+            // c_qx > 0 come from x = -1 and are rebuilt by Zou-He, which never reads them.
+            g[q] = C::ld(solid[o] ? sf[q][r][c] : sf[o][r - cy(q)][c - cx(q)], q);
+        }
+        collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac);
+        const unsigned idx = (unsigned)(gy * W + gx);
+        const unsigned idxB = idx * B;
+        unsigned own[9], nbB[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) own[k] = idxB + (unsigned)k * plane * B;
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            if constexpr (PERIODIC)
+                nbB[q] = (unsigned)(wrap(gy + cy(q), H) * W + wrap(gx + cx(q), W)) * B;
+            else
+                nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
+        }
+        S::st(rout, own[0], 0, 0, g[0]);
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+            // bounce-back writes this cell's slot opp(q); populations leaving through the open inlet are parked
+            // in this cell's slot opp(q) (the canonical inlet slots 1, 5, 8, rebuilt by the next Zou-He)
+            const bool back = solid[q] || (!PERIODIC && inlet && cx(q) < 0);
+            S::st(rout, back ? own[opp(q)] : nbB[q] + (unsigned)q * plane * B, 0, q, g[q]);
+        }
+        if (!PERIODIC && gx == W - 2) {
+            // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
+            S::st(rout, own[3], 0, 3, g[3]);
+            if (!solid[2]) S::st(rout, nbB[2] + 6u * plane * B, 0, 6, g[6]);
+            if (!solid[4]) S::st(rout, nbB[4] + 7u * plane * B, 0, 7, g[7]);
+        }
+        if (orho) {
+            orho[idx] = mac[0];
+            oux[idx] = mac[1];
+            ouy[idx] = mac[2];
+        }
+    }
+}
+
+} // namespace pairk
+} // namespace lbm

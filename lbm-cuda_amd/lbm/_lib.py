@@ -89,6 +89,9 @@ SIGNATURES = {
     "lbm_ensemble_step": (_i, [_vp, _i, _i, _vp]),
     "lbm_ensemble_step_n": (_i, [_vp, _i, _i, ctypes.c_longlong, _i, _vp]),
     "lbm_ensemble_destroy": (_i, [_vp]),
+    "lbm_ensemble_pair_n": (_i, [_vp, _i, ctypes.c_longlong, _i, _i, _vp]),
+    "lbm_ensemble_side": (_i, [_vp]),
+    "lbm_ensemble_pair_info": (_i, [_vp, _i, _vp, _vp]),
     "lbm_graph_begin": (_i, [_vp]),
     "lbm_graph_end": (_i, [_vp, _vp]),
     "lbm_graph_launch": (_i, [_vp, _vp]),

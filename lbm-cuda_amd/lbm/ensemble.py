@@ -5,9 +5,11 @@ The reference advances one lattice per process and replays a captured 2-launch C
 storage format and parity), and ``lbm_ensemble_step`` advances all of them by one AA step in a single
 launch whose workgroups walk a tile list spanning every member; ``lbm_ensemble_step_n`` advances groups of
 members that fit in the Infinity Cache several steps at a time (members are independent, so the order in
-which groups advance changes nothing).  ``Graph`` records any sequence of liblbm
-launches on a side stream (``capture``) and replays it; ``Ensemble.capture(n)`` records n ensemble steps.
-Every member's state is bit-identical to what its own ``Simulation.step`` would produce.
+which groups advance changes nothing).  With ``two_step=True`` every member also holds a second buffer and
+``step(n, k=2)`` advances the ensemble two steps per launch with the two-step tile kernel
+(``lbm_ensemble_pair_n``), moving each member's state between its two buffers.  ``Graph`` records any
+sequence of liblbm launches on a side stream (``capture``) and replays it; ``Ensemble.capture(n, k)`` records
+n ensemble steps.  Every member's state is bit-identical to what its own ``Simulation.step`` would produce.
 """
 from __future__ import annotations
 
@@ -25,7 +27,7 @@ class LbmInstance(ctypes.Structure):
     """Mirror of ``lbm_instance`` (include/lbm.h)."""
 
     _fields_ = [("p", _lib.LbmParams), ("mask", ctypes.c_void_p), ("f", ctypes.c_void_p),
-                ("inlet_u", ctypes.c_void_p)]
+                ("inlet_u", ctypes.c_void_p), ("f_alt", ctypes.c_void_p)]
 
 
 class Graph:
@@ -86,9 +88,13 @@ class Graph:
 class Ensemble:
     """Independent lattices stepped together.  ``members`` are Simulation objects sharing model, bc,
     storage format and parity; their buffers must not be replaced while the ensemble exists (the
-    single-step path updates them in place, so ``Simulation.step(k=1)`` keeps them valid too)."""
+    single-step path updates them in place, so ``Simulation.step(k=1)`` keeps them valid too).
 
-    def __init__(self, members, blocks=0, group_bytes=0):
+    ``two_step=True`` gives every member a second buffer (``Simulation.f_alt``, allocated if missing) for
+    ``step(n, k=2)``; the ensemble then swaps each member's ``f`` and ``f_alt`` so that ``f`` always holds
+    the state.  ``k`` and ``chunk`` (tile rows, 8/12/16/24) are the defaults of ``step``."""
+
+    def __init__(self, members, blocks=0, group_bytes=0, two_step=False, k=1, chunk=16):
         self.lib = _lib.load()
         self.members = list(members)
         if not self.members:
@@ -101,10 +107,19 @@ class Ensemble:
         self.parity = m0.parity
         self.blocks = int(blocks)
         self.group_bytes = int(group_bytes)
+        self.two_step = bool(two_step)
+        self.k, self.chunk = int(k), int(chunk)
+        if self.two_step:
+            for m in self.members:
+                if m.f_alt is None:
+                    m.f_alt = torch.empty_like(m.f)
         self._table = (LbmInstance * len(self.members))()
         for i, m in enumerate(self.members):
-            self._table[i] = LbmInstance(m.params, m.mask.data_ptr(), m.f.data_ptr(), m.inlet_u.data_ptr())
-        self._ptrs = [m.f.data_ptr() for m in self.members]
+            self._table[i] = LbmInstance(m.params, m.mask.data_ptr(), m.f.data_ptr(), m.inlet_u.data_ptr(),
+                                         m.f_alt.data_ptr() if self.two_step else None)
+        # the buffers as registered: buffer 0 holds the state at side 0, buffer 1 at side 1
+        self._bufs = [(m.f.data_ptr(), m.f_alt.data_ptr() if self.two_step else None) for m in self.members]
+        self.side = 0
         self.handle = ctypes.c_void_p()
         _lib.check(self.lib.lbm_ensemble_create(self._table, len(self.members), self.fmt, ctypes.byref(self.handle)),
                    "lbm_ensemble_create")
@@ -114,7 +129,7 @@ class Ensemble:
         self.total_tiles, self.cells, self.default_blocks = tiles.value, cells.value, blk.value
 
     @classmethod
-    def from_members(cls, members, storage="f32", device="cuda", blocks=0, workers=8):
+    def from_members(cls, members, storage="f32", device="cuda", blocks=0, workers=8, two_step=False):
         """Build the Simulations of ``configs.ensemble`` members (periodic, MRT + LES) and group them.  The
         initial states are computed on `workers` host threads, at most `workers` of them held at once."""
         import concurrent.futures as cf
@@ -127,23 +142,52 @@ class Ensemble:
                 for m, f0 in zip(batch, ex.map(lambda mm: mm.f0(np.float32), batch)):
                     sims.append(Simulation(m.W, m.H, m.tau_inv, cs=m.cs, model="mrt", bc="periodic", f0=f0,
                                            device=device, storage=storage))
-        return cls(sims, blocks=blocks)
+        return cls(sims, blocks=blocks, two_step=two_step)
+
+    def pair_blocks(self, chunk):
+        """Workgroups of the two-step kernel with `chunk` tile rows resident at once (its default grid)."""
+        tiles, blk = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self.lib.lbm_ensemble_pair_info(self.handle, int(chunk), ctypes.byref(tiles), ctypes.byref(blk)),
+                   "lbm_ensemble_pair_info")
+        return blk.value
 
     def _check_buffers(self):
-        if [m.f.data_ptr() for m in self.members] != self._ptrs:
-            raise RuntimeError("a member's state buffer moved (fused steps swap buffers): rebuild the ensemble")
+        for m, b in zip(self.members, self._bufs):
+            if m.f.data_ptr() != b[self.side] or (self.two_step and m.f_alt.data_ptr() != b[self.side ^ 1]):
+                raise RuntimeError("a member's state buffer moved (its own fused steps swap buffers): "
+                                   "rebuild the ensemble")
 
-    def _launch(self, parity, n, stream):
-        _lib.check(self.lib.lbm_ensemble_step_n(self.handle, parity, n, self.group_bytes, self.blocks, stream),
-                   "lbm_ensemble_step_n")
+    def _check_k(self, n, k):
+        if k not in (1, 2):
+            raise ValueError("ensemble steps take k = 1 or 2")
+        if k == 2:
+            if not self.two_step:
+                raise ValueError("k = 2 needs an ensemble built with two_step=True")
+            if self.parity != 0 or n % 2:
+                raise ValueError("k = 2 advances an even number of steps from an even state")
 
-    def step(self, n=1):
+    def _launch(self, parity, n, stream, k=1):
+        if k == 1:
+            _lib.check(self.lib.lbm_ensemble_step_n(self.handle, parity, n, self.group_bytes, self.blocks, stream),
+                       "lbm_ensemble_step_n")
+            return
+        _lib.check(self.lib.lbm_ensemble_pair_n(self.handle, n // 2, self.group_bytes, self.chunk, self.blocks,
+                                                stream), "lbm_ensemble_pair_n")
+        side = self.lib.lbm_ensemble_side(self.handle)
+        if side != self.side:
+            for m in self.members:
+                m.f, m.f_alt = m.f_alt, m.f
+            self.side = side
+
+    def step(self, n=1, k=None):
         """n AA steps of every member: members grouped by ``group_bytes`` (0 = all in one group), each
-        group advanced n steps (one persistent launch per step) before the next."""
+        group advanced n steps (one persistent launch per step, or per two steps with k = 2) before the
+        next."""
         self._check_buffers()
-        n = int(n)
+        n, k = int(n), self.k if k is None else int(k)
         if n > 0:
-            self._launch(self.parity, n, _stream())
+            self._check_k(n, k)
+            self._launch(self.parity, n, _stream(), k)
             self.parity = (self.parity + n) & 1
             self._sync_members(n)
         return self
@@ -176,20 +220,26 @@ class Ensemble:
             m.parity = self.parity
             m.steps += n
 
-    def capture(self, n=2):
-        """A Graph that performs n ensemble steps from the current parity.  n must be even so that each
-        replay returns to the same parity; call ``replayed(g)`` after each replay for the bookkeeping."""
-        n = int(n)
-        if n < 2 or n % 2:
-            raise ValueError("capture an even number of steps")
+    def capture(self, n=2, k=None):
+        """A Graph that performs n ensemble steps from the current parity and side.  Each replay must return
+        to both, so n is even, and with k = 2 a multiple of 4; replay it with ``replay(g)``."""
+        n, k = int(n), self.k if k is None else int(k)
+        if n < 2 or n % (2 * k):
+            raise ValueError("capture an even number of steps (a multiple of 4 with k = 2)")
+        self._check_k(n, k)
         self._check_buffers()
         g = Graph()
         with g.capture():
-            self._launch(self.parity, n, ctypes.c_void_p(g.stream.cuda_stream))
-        g.steps = n
+            self._launch(self.parity, n, ctypes.c_void_p(g.stream.cuda_stream), k)
+        g.steps, g.at = n, (self.parity, self.side)
         return g
 
     def replay(self, g, times=1):
+        """Replay a graph from ``capture``; the ensemble must be at the parity and side it was captured at
+        (the launches' buffers and parities are fixed in the graph)."""
+        if (self.parity, self.side) != g.at:
+            raise RuntimeError(f"graph captured at (parity, side) {g.at}, ensemble at {(self.parity, self.side)}")
+        self._check_buffers()
         for _ in range(int(times)):
             g.replay()
             self._sync_members(g.steps)
@@ -223,32 +273,50 @@ def size_classes(ens):
     return ",".join(f"{k}:{v}" for k, v in sorted(hist.items()))
 
 
-def time_ensemble(ens, blocks, group_bytes, steps=20, reps=2):
-    """Mean microseconds per ensemble step with grid `blocks` and grouping `group_bytes`, in calls of
-    `steps` steps (the state advances by steps * (reps + 1))."""
-    old = ens.blocks, ens.group_bytes
-    ens.blocks, ens.group_bytes = int(blocks), int(group_bytes)
-    ens.step(steps)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        ens.step(steps)
-    e1.record()
-    torch.cuda.synchronize()
-    ens.blocks, ens.group_bytes = old
+def time_ensemble(ens, blocks, group_bytes, steps=20, reps=2, k=1, chunk=16):
+    """Mean microseconds per ensemble step with grid `blocks`, grouping `group_bytes` and `k` steps per
+    launch (tile rows `chunk` for k = 2), in calls of `steps` steps (the state advances by
+    steps * (reps + 1))."""
+    old = ens.blocks, ens.group_bytes, ens.chunk
+    ens.blocks, ens.group_bytes, ens.chunk = int(blocks), int(group_bytes), int(chunk)
+    try:
+        ens.step(steps, k=k)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            ens.step(steps, k=k)
+        e1.record()
+        torch.cuda.synchronize()
+    finally:
+        ens.blocks, ens.group_bytes, ens.chunk = old
     return e0.elapsed_time(e1) * 1e3 / (steps * reps)
 
 
 GROUPS_MB = (0, 32, 64, 128, 192)
+PAIR_CHUNKS = (8, 12, 16, 24)
+
+
+def tune_candidates(ens, steps):
+    """(k, chunk, blocks) launch shapes to time: the single-step kernel at 2, 4, 6 or 8 workgroups per CU
+    (never more than are resident at once), and, for a two-step ensemble at an even state with an even
+    `steps`, the two-step kernel at each tile height on its resident grid (blocks 0) and half of it."""
+    full = ens.default_blocks
+    cands = [(1, 0, b) for b in sorted({max(8, full * w // 8 // 8 * 8) for w in (2, 4, 6, 8)})]
+    if ens.two_step and ens.parity == 0 and steps % 2 == 0:
+        for ch in PAIR_CHUNKS:
+            pb = ens.pair_blocks(ch)
+            cands += [(2, ch, b) for b in sorted({max(8, pb // 2 // 8 * 8), pb})]
+    return cands
 
 
 def tune_ensemble(ens, use_cache=True, steps=20, reps=2, verbose=False):
-    """Pick the persistent grid (2, 4, 6 or 8 workgroups per CU, never more than are resident at once) and
-    the cache grouping (GROUPS_MB, 0 = none) for this mix of size classes and `steps` per call; sets
-    ens.blocks and ens.group_bytes and returns (blocks, group_bytes, us_per_step)."""
+    """Pick the launch shape (k, tile rows, persistent grid) and the cache grouping (GROUPS_MB, 0 = none)
+    for this mix of size classes and `steps` per call; sets ens.k, ens.chunk, ens.blocks and
+    ens.group_bytes and returns (blocks, group_bytes, us_per_step, k, chunk)."""
     dev = torch.cuda.get_device_name(ens.members[0].device)
     m0 = ens.members[0]
-    key = f"{dev}|fmt{ens.fmt}|model{m0.model}|bc{m0.bc}|steps{steps}|{size_classes(ens)}"
+    two = ens.two_step and ens.parity == 0 and steps % 2 == 0
+    key = f"{dev}|fmt{ens.fmt}|model{m0.model}|bc{m0.bc}|steps{steps}|k{2 if two else 1}|{size_classes(ens)}"
     cache = {}
     if use_cache:
         try:
@@ -257,26 +325,26 @@ def tune_ensemble(ens, use_cache=True, steps=20, reps=2, verbose=False):
         except (OSError, ValueError):
             cache = {}
     if key in cache:
-        ens.blocks, ens.group_bytes = int(cache[key][0]), int(cache[key][1])
-        return tuple(cache[key])
-    full = ens.default_blocks
-    cands = sorted({max(8, full * w // 8 // 8 * 8) for w in (2, 4, 6, 8)})
-    best = None
-    for gmb in GROUPS_MB:
-        for b in cands:
-            us = time_ensemble(ens, b, gmb << 20, steps=steps, reps=reps)
-            if verbose:
-                print(f"  blocks={b} group={gmb} MB: {us:.1f} us/step", flush=True)
-            if best is None or us < best[2]:
-                best = (b, gmb << 20, us)
-    ens.blocks, ens.group_bytes = best[0], best[1]
-    if use_cache:
-        cache[key] = list(best)
-        try:
-            os.makedirs(os.path.dirname(_cache_path()), exist_ok=True)
-            with open(_cache_path() + ".tmp", "w") as fh:
-                json.dump(cache, fh, indent=1, sort_keys=True)
-            os.replace(_cache_path() + ".tmp", _cache_path())
-        except OSError:
-            pass
+        best = tuple(cache[key])
+    else:
+        best = None
+        ens.tune_table = []
+        for gmb in GROUPS_MB:
+            for k, ch, b in tune_candidates(ens, steps):
+                us = time_ensemble(ens, b, gmb << 20, steps=steps, reps=reps, k=k, chunk=ch or ens.chunk)
+                ens.tune_table.append((k, ch, b, gmb, us))
+                if verbose:
+                    print(f"  k={k} rows={ch} blocks={b} group={gmb} MB: {us:.1f} us/step", flush=True)
+                if best is None or us < best[2]:
+                    best = (b, gmb << 20, us, k, ch or ens.chunk)
+        if use_cache:
+            cache[key] = list(best)
+            try:
+                os.makedirs(os.path.dirname(_cache_path()), exist_ok=True)
+                with open(_cache_path() + ".tmp", "w") as fh:
+                    json.dump(cache, fh, indent=1, sort_keys=True)
+                os.replace(_cache_path() + ".tmp", _cache_path())
+            except OSError:
+                pass
+    ens.blocks, ens.group_bytes, ens.k, ens.chunk = int(best[0]), int(best[1]), int(best[3]), int(best[4])
     return best

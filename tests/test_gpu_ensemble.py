@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 import torch
 
+import oracle as O
 from lbm import geometry as G
 
 pytestmark = pytest.mark.gpu
@@ -186,3 +187,131 @@ def test_ensemble_tuner(gpu, tmp_path, monkeypatch):
     _solo_steps(b, n)
     _assert_same(a, b, "after tuning")
 
+
+
+# ------------------------------------------------------------------------ two-step (k = 2) ensemble path
+def _observable(sim):
+    p = sim.params
+    op = O.params(p.W, p.H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc)
+    return torch.from_numpy(O.observable(op, sim.mask_host, sim.parity, ghost=True)).to(sim.device)
+
+
+def _assert_same_observable(ens_sims, solo_sims, what):
+    """The two-step kernel leaves slots that no later step reads (the inlet's leaving slots) unwritten in
+    its output buffer: compare every other population bit for bit."""
+    torch.cuda.synchronize()
+    for i, (s, r) in enumerate(zip(ens_sims, solo_sims)):
+        assert s.parity == r.parity, f"{what}: member {i} parity"
+        sel = _observable(r)
+        bad = int(((s.f.view(torch.int16 if s.f.dtype == torch.int16 else torch.int32) !=
+                    r.f.view(torch.int16 if r.f.dtype == torch.int16 else torch.int32)) & sel).sum())
+        assert bad == 0, f"{what}: member {i} ({s.W}x{s.H}) differs in {bad} populations"
+
+
+@pytest.mark.parametrize("storage,chunk", [("f32", 8), ("f32", 12), ("f32", 16), ("f32", 24), ("f16", 16),
+                                           ("bf16", 8)])
+def test_ensemble_two_step_periodic_bitwise(gpu, storage, chunk):
+    """k = 2 launches over every member equal single steps of each member alone, on ragged sizes
+    (including lattices narrower than one tile), mixed with k = 1 ensemble steps."""
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES, storage=storage, seed=81)
+    ens = Ensemble(a, two_step=True, k=2, chunk=chunk)
+    for n in (2, 4, 2):
+        ens.step(n)
+        _solo_steps(b, n)
+        _assert_same_observable(a, b, f"{storage} rows {chunk} after +{n} (side {ens.side})")
+    ens.step(1, k=1)  # odd parity: single steps only, in place in the current buffer
+    _solo_steps(b, 1)
+    with pytest.raises(ValueError):
+        ens.step(2, k=2)
+    ens.step(3, k=1)
+    _solo_steps(b, 3)
+    ens.step(2, k=2)
+    _solo_steps(b, 2)
+    _assert_same_observable(a, b, f"{storage} rows {chunk} mixed")
+
+
+@pytest.mark.parametrize("model", ["mrt", "bgk"])
+def test_ensemble_two_step_channel_bitwise(gpu, model):
+    from lbm.ensemble import Ensemble
+    a, b = _channel_pair(SIZES, model=model, seed=83)
+    ens = Ensemble(a, two_step=True, k=2, chunk=12)
+    ens.step(6)
+    _solo_steps(b, 6)
+    _assert_same_observable(a, b, f"channel {model}")
+    for m in a:  # every member's own single steps continue from the ensemble's buffer
+        m.step(2)
+    _solo_steps(b, 2)
+    _assert_same_observable(a, b, f"channel {model} then solo")
+
+
+@pytest.mark.parametrize("blocks,group_kb", [(8, 0), (16, 300), (4096, 1), (0, 2000)])
+def test_ensemble_two_step_grids_and_groups(gpu, blocks, group_kb):
+    """Any grid (down to one workgroup per XCD) and any cache grouping give the same state."""
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES[:6], seed=85)
+    ens = Ensemble(a, blocks=blocks, group_bytes=group_kb << 10, two_step=True, k=2, chunk=16)
+    for n in (6, 2):
+        ens.step(n)
+        _solo_steps(b, n)
+        _assert_same_observable(a, b, f"blocks {blocks} group {group_kb} KB after +{n}")
+
+
+def test_ensemble_two_step_graph_replays(gpu):
+    """A captured 8-step k = 2 graph returns to its side and parity; replayed 3 times with eager steps of
+    both kinds between replays, every replay equals the eager single-step result."""
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES, seed=87)
+    ens = Ensemble(a, two_step=True, k=2, chunk=16, group_bytes=300 << 10)
+    with pytest.raises(ValueError):
+        ens.capture(6)  # 3 two-step launches would end on the other buffer
+    g = ens.capture(8)
+    _assert_same_observable(a, b, "capture records, does not run")
+    for r in range(3):
+        ens.replay(g)
+        _solo_steps(b, 8)
+        _assert_same_observable(a, b, f"replay {r}")
+        ens.step(2)  # moves the state to the other buffer: the graph must refuse to run
+        _solo_steps(b, 2)
+        with pytest.raises(RuntimeError):
+            ens.replay(g)
+        ens.step(2)
+        ens.step(2, k=1)
+        _solo_steps(b, 4)
+        torch.cuda.synchronize()
+        _assert_same_observable(a, b, f"eager after replay {r}")
+    g.close()
+
+
+def test_ensemble_two_step_limits(gpu):
+    from lbm import _lib
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES[:3], seed=89)
+    one = Ensemble(a)
+    with pytest.raises(ValueError):
+        one.step(2, k=2)  # built without second buffers
+    one.close()
+    ens = Ensemble(a, two_step=True)
+    with pytest.raises(_lib.LbmError):
+        ens.chunk = 10
+        ens.step(2, k=2)
+    ens.chunk = 16
+    with pytest.raises(ValueError):
+        ens.step(3, k=2)
+    a[0].step(2, k=2, cfg=(0, 8))  # a member's own fused step swaps its buffers: the ensemble refuses
+    with pytest.raises(RuntimeError):
+        ens.step(2)
+
+
+def test_ensemble_tuner_two_step(gpu, tmp_path, monkeypatch):
+    from lbm.ensemble import Ensemble, tune_ensemble
+    monkeypatch.setenv("LBM_TUNE_CACHE", str(tmp_path / "fused.json"))
+    a, b = _periodic_pair(SIZES[:6], seed=91)
+    ens = Ensemble(a, two_step=True)
+    best = tune_ensemble(ens, steps=4, reps=1)
+    assert best[3] in (1, 2) and best[0] % 8 == 0 and best[2] > 0
+    assert (ens.blocks, ens.group_bytes, ens.k, ens.chunk) == (best[0], best[1], best[3], best[4])
+    assert tuple(tune_ensemble(ens, steps=4, reps=1)) == tuple(best)
+    n = ens.members[0].steps
+    _solo_steps(b, n)
+    _assert_same_observable(a, b, "after tuning")
