@@ -302,6 +302,16 @@ int lbm_step_closure(const lbm_params *p, const uint8_t *mask, float *f, int par
                      const float *params, const lbm_closure *cl, float *neq, float *nut, unsigned *fallbacks,
                      lbm_stream_t stream);
 
+/*
+ * Exhaustive self-check of the short arithmetic sequences of the collision over all 2^32 FP32 inputs (no
+ * reference counterpart).  what = 0: the reciprocal against the general correctly rounded 1/x; what = 1:
+ * the square root against the correctly rounded sqrtf on its domain (normal x >= 1e-8).  `bad` and `first`
+ * are device arrays of lbm_selftest_threads() entries: per thread, the number of mismatching inputs and
+ * the first one (0xFFFFFFFF if none).
+ */
+int lbm_selftest(int what, uint32_t *bad, uint32_t *first, lbm_stream_t stream);
+int lbm_selftest_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,7 +33,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         if (solid_at(idx)) return;
     }
     float g[9], mac[3];
-    unsigned nb[9], nbB[9], own[9];
+    unsigned nb[9], nbB[9], nb2[9];
     bool solid[9];
     if constexpr (PARITY == 0) {
 #pragma unroll
@@ -57,29 +57,32 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
             const bool s_nb = solid_at(nb[q]);
             solid[q] = s_nb && !(!PERIODIC && inlet && cx(q) < 0);
         }
-        // Bounce-back reads this cell's slot q, otherwise the source's slot opp(q).  The plane offset is part
-        // of the per-lane offset, so the choice is a select: a per-lane scalar offset would force a loop over
-        // its distinct values around every load.  At the inlet the q with c_qx > 0 arrive from x = -1; the
-        // value read for them (from an in-range or past-the-end offset, both harmless) is replaced by Zou-He,
-        // which never reads it.
-        // Every offset is this cell's byte offset plus a wave-uniform constant (kept in scalar registers); the
-        // bounce-back offsets own[k] serve the gather and the scatter alike.
-        const unsigned idxB = idx * B;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) own[k] = idxB + (unsigned)k * plane * B;
+        // Bounce-back reads this cell's slot q, otherwise the source's slot opp(q).  The choice is a per-lane
+        // select of the vector offset: a per-lane scalar offset would force a loop over its distinct values
+        // around every load.  The two candidates' planes q and opp(q) differ by 2, so the wave-uniform part
+        // min(q, opp(q)) * plane goes into the scalar offset and the vector offsets need only this cell's
+        // offset, the same two planes up (idx2), and the neighbours' offsets, two planes up for q = 3, 4, 7, 8
+        // (nb2): 13 additions for the gather and the scatter together.  At the inlet the q with c_qx > 0
+        // arrive from x = -1; the value read for them (from an in-range or past-the-end offset, both harmless)
+        // is replaced by Zou-He, which never reads it.
+        const unsigned idxB = idx * B, pB = plane * B;
+        const unsigned idx2 = idxB + 2u * pB;
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             if constexpr (PERIODIC)
                 nbB[q] = nb[q] * B;
             else
                 nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
+            nb2[q] = opp(q) < q ? nbB[q] + 2u * pB : 0u; // used for q = 3, 4, 7, 8 only
         }
-        g[0] = S::ld(rf, own[0], 0, 0);
+        g[0] = S::ld(rf, idxB, 0, 0);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
-            const unsigned off = solid[o] ? own[q] : nbB[o] + (unsigned)o * plane * B;
-            g[q] = S::ld(rf, off, 0, q);
+            // o < q: slot o at the neighbour = nbB[o] + o pB, slot q here = idx2 + o pB
+            // o > q: slot o at the neighbour = nb2[o] + q pB, slot q here = idxB + q pB
+            const unsigned off = o < q ? (solid[o] ? idx2 : nbB[o]) : (solid[o] ? idxB : nb2[o]);
+            g[q] = S::ld(rf, off, (unsigned)(o < q ? o : q) * pB, q);
         }
     }
     collide_dev<MODEL, EXT>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac, w_ext);
@@ -95,12 +98,17 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         // populations leaving through the open inlet (c_qx < 0) are parked in this cell's slot opp(q): those
         // are the canonical inlet slots q = 1, 5, 8, which no other cell reads or writes in this step and which
         // the next step's Zou-He rebuilds without reading.
-        S::st(rf, own[0], 0, 0, g[0]);
+        const unsigned idxB = idx * B, pB = plane * B;
+        const unsigned idx2 = idxB + 2u * pB;
+        S::st(rf, idxB, 0, 0, g[0]);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
+            const int o = opp(q);
             const bool back = solid[q] || (!PERIODIC && inlet && cx(q) < 0);
-            const unsigned off = back ? own[opp(q)] : nbB[q] + (unsigned)q * plane * B;
-            S::st(rf, off, 0, q, g[q]);
+            // q < o: slot q at the target = nbB[q] + q pB, slot o here = idx2 + q pB
+            // q > o: slot q at the target = nb2[q] + o pB, slot o here = idxB + o pB
+            const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nb2[q]);
+            S::st(rf, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
         }
         if (!PERIODIC && x == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2

@@ -25,7 +25,7 @@ __device__ __forceinline__ void mrt_neq(const KPar &P, const float h[9], float &
     const float drho = (h[0] + h14) + h58;
     const float rho = 1.0f + drho;
     const float jx = a13 + sx, jy = a24 + sy;
-    const float inv_rho = __fdiv_rn(1.0f, rho);
+    const float inv_rho = recip_rn(rho);
     float ux = jx * inv_rho, uy = jy * inv_rho, drc = drho;
     if (P.clamp) {
         ux = clampf(ux, -0.4f, 0.4f);
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_closure(KPar P, const uint8_
     const bool ok = trusted && raw <= cl.nu_max; // false for NaN
     const float nu = raw > 0.0f ? raw : 0.0f;
     float w_ext = -1.0f;
-    if (ok) w_ext = __fdiv_rn(1.0f, P.tau0 + 3.0f * nu);
+    if (ok) w_ext = recip_rn(P.tau0 + 3.0f * nu);
     if (nfall) {
         const unsigned long long bad = __ballot(in && !ok && !mask[(long)y * W + x]);
         if (lane == 0 && bad) atomicAdd(nfall, (unsigned)__popcll(bad));

@@ -59,6 +59,21 @@ __device__ __forceinline__ float sqrt_rn_normal(float x)
     const float t = rdn <= 0.0f ? sdn : s;
     return rup > 0.0f ? sup : t;
 }
+// Correctly rounded 1/x, equal to __fdiv_rn(1, x) for every x.  For 2^-125 <= |x| <= 2^125 the hardware
+// estimate y (within 1 ulp) takes one Newton step with an exact residual, r = fma(-x, y, 1) and
+// y + r y = fma(r, y, y) (Markstein's reciprocal iteration): 3 instructions instead of the 11 of the
+// general division, which handles scaling and special values.  Other x (zero, denormal, huge, inf, NaN)
+// take the general division.  lbm_selftest compares both over all 2^32 inputs.
+__device__ __forceinline__ float recip_rn(float x)
+{
+    const float ax = __builtin_fabsf(x);
+    if (ax >= 0x1p-125f && ax <= 0x1p125f) { // false for NaN
+        const float y = __builtin_amdgcn_rcpf(x);
+        return fma_(fma_(-x, y, 1.0f), y, y);
+    }
+    return __fdiv_rn(1.0f, x);
+}
+
 // clamp to [lo, hi] (lo <= hi) in one v_med3_f32; equal to min(max(v, lo), hi) for every non-NaN v
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
 
@@ -109,7 +124,7 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
         const float drho = (h[0] + h14) + h58;
         rho = 1.0f + drho;
         const float jx = a13 + sx, jy = a24 + sy;
-        const float inv_rho = __fdiv_rn(1.0f, rho);
+        const float inv_rho = recip_rn(rho);
         ux = jx * inv_rho;
         uy = jy * inv_rho;
         drc = drho;
@@ -160,7 +175,7 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
         const float Q = sqrt_rn_normal(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
         const float d = fma_(P.c18cs2, Q * inv_rc, P.tau0sq);
         const float tau_eff = 0.5f * (P.tau0 + sqrt_rn_normal(d));
-        w = __fdiv_rn(1.0f, tau_eff);
+        w = recip_rn(tau_eff);
     }
     // relaxation (lbm_core4.cu:180-185)
     m1d = fma_(-P.s_e, n1, m1d);
@@ -207,7 +222,7 @@ __device__ __forceinline__ void collide_bgk(const KPar &P, float h[9], bool inle
         const float a13 = h[1] - h[3], a24 = h[2] - h[4], a57 = h[5] - h[7], a86 = h[8] - h[6];
         drho = (h[0] + (s13 + s24)) + (s57 + s68);
         rho = 1.0f + drho;
-        const float inv_rho = __fdiv_rn(1.0f, rho);
+        const float inv_rho = recip_rn(rho);
         ux = (a13 + (a57 + a86)) * inv_rho;
         uy = (a24 + (a57 - a86)) * inv_rho;
     }

@@ -127,10 +127,9 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         }
         collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac);
         const unsigned idx = (unsigned)(gy * W + gx);
-        const unsigned idxB = idx * B;
-        unsigned own[9], nbB[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) own[k] = idxB + (unsigned)k * plane * B;
+        const unsigned idxB = idx * B, pB = plane * B;
+        const unsigned idx2 = idxB + 2u * pB;
+        unsigned nbB[9];
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             if constexpr (PERIODIC)
@@ -138,19 +137,23 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             else
                 nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
         }
-        S::st(rout, own[0], 0, 0, g[0]);
+        S::st(rout, idxB, 0, 0, g[0]);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             // bounce-back writes this cell's slot opp(q); populations leaving through the open inlet are parked
-            // in this cell's slot opp(q) (the canonical inlet slots 1, 5, 8, rebuilt by the next Zou-He)
+            // in this cell's slot opp(q) (the canonical inlet slots 1, 5, 8, rebuilt by the next Zou-He).  The
+            // planes q and opp(q) differ by 2: min(q, opp(q)) * plane is the scalar offset, and the vector
+            // offset is this cell's or the target's offset, two planes up for the larger index.
+            const int o = opp(q);
             const bool back = solid[q] || (!PERIODIC && inlet && cx(q) < 0);
-            S::st(rout, back ? own[opp(q)] : nbB[q] + (unsigned)q * plane * B, 0, q, g[q]);
+            const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nbB[q] + 2u * pB);
+            S::st(rout, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
         }
         if (!PERIODIC && gx == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-            S::st(rout, own[3], 0, 3, g[3]);
-            if (!solid[2]) S::st(rout, nbB[2] + 6u * plane * B, 0, 6, g[6]);
-            if (!solid[4]) S::st(rout, nbB[4] + 7u * plane * B, 0, 7, g[7]);
+            S::st(rout, idxB, 3u * pB, 3, g[3]);
+            if (!solid[2]) S::st(rout, nbB[2], 6u * pB, 6, g[6]);
+            if (!solid[4]) S::st(rout, nbB[4], 7u * pB, 7, g[7]);
         }
         if (orho) {
             orho[idx] = mac[0];

@@ -92,6 +92,8 @@ SIGNATURES = {
     "lbm_ensemble_pair_n": (_i, [_vp, _i, ctypes.c_longlong, _i, _i, _vp]),
     "lbm_ensemble_side": (_i, [_vp]),
     "lbm_ensemble_pair_info": (_i, [_vp, _i, _vp, _vp]),
+    "lbm_selftest": (_i, [_i, _vp, _vp, _vp]),
+    "lbm_selftest_threads": (_i, []),
     "lbm_graph_begin": (_i, [_vp]),
     "lbm_graph_end": (_i, [_vp, _vp]),
     "lbm_graph_launch": (_i, [_vp, _vp]),
