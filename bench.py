@@ -335,8 +335,9 @@ def main():
         sim.parity = (sim.parity + n) & 1
         sim.steps += n
 
-    if args.only == "fused":
-        args.warmup, args.steps = 2, 2
+    fused_steps = args.steps
+    if args.only == "fused":  # the plain step only as the reference rate: a short run
+        args.warmup, args.steps = 2, 20
     # ---------------- plain FP32 single AA steps
     run_aa(args.warmup)
     torch.cuda.synchronize()
@@ -403,7 +404,7 @@ def main():
             else:
                 best, table = autotune.tune_fused(sf, ks=ks, use_cache=False)
                 k, strip, chunk, _ = best
-            nlaunch = max(1, args.steps // k)
+            nlaunch = max(1, fused_steps // k)
             sf.step(2 * k, k=k, cfg=(strip, chunk))
             us = time_steps(lambda n, sf=sf, k=k, cfg=(strip, chunk): sf.step(n, k=k, cfg=cfg), nlaunch * k)
             name = f"fused_{storage}"

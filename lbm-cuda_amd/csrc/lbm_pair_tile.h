@@ -105,25 +105,33 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         const int r = 1 + e / CW, c = 1 + (e - (e / CW) * CW);
         const int gx = x0 + c - 1, gy = y0 + r - 1;
         if (gx >= W || gy >= H) continue;
+        // LDS reads relative to the corner (r - 1, c - 1) of the cell's 3 x 3 neighbourhood: every access is
+        // that base plus a non-negative constant, which the DS instructions take as an immediate offset
+        const int lb = (r - 1) * LS + (c - 1);
+        const T *sfb = &sf[0][0][0] + lb;
+        const uint8_t *smb = &sm[0][0] + lb;
+        auto at = [&](int k, int dy, int dx) -> T { return sfb[k * RH * LS + (1 + dy) * LS + (1 + dx)]; };
         bool inlet = false;
         if constexpr (!PERIODIC) {
             if (gy == 0 || gy == H - 1 || gx == W - 1) continue; // walls and the outlet ghost are not updated
             inlet = gx == 0;
-            if (!inlet && sm[r][c]) continue;
+            if (!inlet && smb[LS + 1]) continue;
         } else {
-            if (sm[r][c]) continue;
+            if (smb[LS + 1]) continue;
         }
         bool solid[9];
 #pragma unroll
-        for (int q = 1; q < 9; ++q) solid[q] = sm[r + cy(q)][c + cx(q)] && !(!PERIODIC && inlet && cx(q) < 0);
+        for (int q = 1; q < 9; ++q)
+            solid[q] = smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(!PERIODIC && inlet && cx(q) < 0);
         float g[9], mac[3];
-        g[0] = C::ld(sf[0][r][c], 0);
+        g[0] = C::ld(at(0, 0, 0), 0);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
-            // bounce-back: this cell's slot q; otherwise the source's slot opp(q).  At the inlet the q with
-            // c_qx > 0 come from x = -1 and are rebuilt by Zou-He, which never reads them.
-            g[q] = C::ld(solid[o] ? sf[q][r][c] : sf[o][r - cy(q)][c - cx(q)], q);
+            // bounce-back: this cell's slot q; otherwise the source's slot opp(q) (both read, then selected).
+            // At the inlet the q with c_qx > 0 come from x = -1 and are rebuilt by Zou-He, which never reads them.
+            const T own = at(q, 0, 0), src = at(o, -cy(q), -cx(q));
+            g[q] = C::ld(solid[o] ? own : src, q);
         }
         collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac);
         const unsigned idx = (unsigned)(gy * W + gx);
