@@ -46,11 +46,12 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     const int t = threadIdx.x;
 
     // ---- phase A: even step of every region cell, reversed result into LDS.  All of a thread's HBM loads
-    //      (NA cells x 9 populations) are issued before the first collision, so their latencies overlap.
+    //      (NA cells x (mask byte + 9 populations)) are issued before anything uses them, so their latencies
+    //      overlap; a mask test between them would make each cell's loads wait for the previous ones.
     constexpr int NA = (RH * RW + NT - 1) / NT;
     float ga[NA][9];
-    unsigned ia[NA];
-    bool ca[NA], la[NA];
+    unsigned ma[NA], ia[NA];
+    bool ina[NA], la[NA], ua[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         const int e = t + a * NT;
@@ -63,29 +64,25 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         } else {
             in = in && gx >= 0 && gx < W && gy >= 0 && gy < H;
         }
+        // unconditional (cells outside the lattice read cell 0 and discard it), so no load waits in a branch
         const unsigned idx = in ? (unsigned)(gy * W + gx) : 0u;
-        const bool solid = in ? __builtin_amdgcn_raw_buffer_load_b8(rm, idx, 0, 0) != 0 : true;
-        if (e < RH * RW) sm[r][c] = solid ? 1 : 0;
-        bool comp, inlet = false;
-        if constexpr (PERIODIC) {
-            comp = in && !solid;
-        } else {
-            inlet = gx == 0;
-            comp = in && gy >= 1 && gy <= H - 2 && gx <= W - 2 && (inlet || !solid);
-        }
-        ca[a] = comp;
-        la[a] = inlet;
-        ia[a] = (unsigned)gy; // the row, for the inlet speed
-        // unconditional (cells that are not updated read cell 0 and discard it), so no load waits in a branch
+        ma[a] = __builtin_amdgcn_raw_buffer_load_b8(rm, idx, 0, 0);
 #pragma unroll
         for (int q = 0; q < 9; ++q) ga[a][q] = S::ld(rin, idx * B, (unsigned)q * plane * B, q);
+        ina[a] = in;
+        la[a] = !PERIODIC && gx == 0;
+        // rows and columns the even step updates (all of them when periodic)
+        ua[a] = PERIODIC || (gy >= 1 && gy <= H - 2 && gx <= W - 2);
+        ia[a] = (unsigned)gy; // the row, for the inlet speed
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-        if (!ca[a]) continue;
         const int e = t + a * NT;
         const int r = e / RW, c = e - r * RW;
+        const bool solid = !ina[a] || ma[a] != 0;
+        if (e < RH * RW) sm[r][c] = solid ? 1 : 0;
         const bool inlet = la[a];
+        if (!(ina[a] && ua[a] && (inlet || !solid))) continue;
         float mac[3];
         collide_dev<MODEL>(P, ga[a], inlet, inlet ? inlet_u[ia[a]] : 0.0f, mac);
         T o[9];
