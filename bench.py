@@ -35,10 +35,54 @@ BYTES_PER_CELL_FP32 = 72  # 9 populations read + 9 written, 4 B each
 # command lines is profiles/k_aa_cylinder_4096x1024.md.
 TRAFFIC_BYTES_PER_STEP = {"f32": 304.95e6, "f16": 154.75e6, "bf16": 154.75e6}
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-calibrated, even/odd mean"
+# The same for one launch (two steps) of the two-step tile kernel, per (storage, tile rows), and its VALU
+# instructions per cell and launch (rocprofv3 --pmc SQ_INSTS_VALU x 64 / cells); the summary with the command
+# lines is profiles/k_pair_cylinder_4096x1024.md.  Configurations not listed there report null.
+PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.48e6, ("bf16", 12): 156.48e6, ("f32", 12): 309.24e6}
+PAIR_VALU_PER_CELL = {("f16", 12): 457.0, ("bf16", 12): 449.1, ("f32", 12): 472.4,
+                      ("f16", 8): 486.1, ("bf16", 8): 478.1, ("f32", 8): 490.5}
+# VALU issue capacity of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each, 2.4 GHz
+VALU_WAVE_INSTS_PER_US = 256 * 4 / 4 * 2400.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def roofline(name, form, cells, achieved_fp32):
+    """HBM roofline of the kernel behind form `name`: algorithmic bytes per launch (each population read once
+    and written once per launch) over the launch time measured with HIP events, against the 8 TB/s peak; for
+    the two-step tile kernel also its VALU issue floor (it is bound by instruction issue, not by HBM)."""
+    if name == "aa_fp32":
+        return {"bound": "hbm", "achieved": round(achieved_fp32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved_fp32 / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES_PER_STEP["f32"],
+                "traffic_source": TRAFFIC_SOURCE, "algorithmic_bytes": BYTES_PER_CELL_FP32 * cells,
+                "kernel": "k_aa FP32 (plain form; even/odd average)", "bytes_per_cell": BYTES_PER_CELL_FP32}
+    storage = name.split("_")[1]
+    elem = 4 if storage in ("f32", "fp32") else 2
+    k = form.get("k", 1)
+    launch_us = form["us_per_step"] * k
+    alg = 2 * 9 * elem * cells  # per launch, whatever k
+    ach = alg / launch_us / 1e3
+    out = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg, "bytes_per_cell": 2 * 9 * elem,
+           "launch_us": round(launch_us, 2)}
+    if name.startswith("aa_"):
+        out.update(kernel=f"k_aa {storage} (single step)", traffic=TRAFFIC_BYTES_PER_STEP.get(storage),
+                   traffic_source=TRAFFIC_SOURCE)
+        return out
+    pair = form.get("strip") == 0 and k == 2
+    key = (storage, form.get("chunk"))
+    out["kernel"] = (f"k_pair {storage} {form.get('chunk')}-row tiles (two steps per launch)" if pair else
+                     f"k_fused {storage} k={k} strip={form.get('strip')} chunk={form.get('chunk')}")
+    out["traffic"] = PAIR_TRAFFIC_PER_LAUNCH.get(key) if pair else None
+    out["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-calibrated"
+    if pair and key in PAIR_VALU_PER_CELL:
+        floor = PAIR_VALU_PER_CELL[key] * cells / 64 / VALU_WAVE_INSTS_PER_US
+        out["valu"] = {"insts_per_cell_per_launch": PAIR_VALU_PER_CELL[key], "floor_us": round(floor, 2),
+                       "frac": round(floor / launch_us, 4),
+                       "note": "issue-bound: 4 cycles per wave64 VALU instruction on 1024 SIMDs at 2.4 GHz"}
+    return out
 
 
 def bench_ensemble(nsteps, graph_steps=20):
@@ -513,10 +557,8 @@ def main():
                        "frac_of_copy": (round(achieved / ceiling["inplace_rmw_gbs"], 4) if ceiling else None)},
         "copy_ceiling": ceiling,
         "fluid_cells": fluid,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES_PER_STEP["f32"],
-                     "traffic_source": TRAFFIC_SOURCE, "algorithmic_bytes": BYTES_PER_CELL_FP32 * cells,
-                     "kernel": "k_aa FP32 (plain form; even/odd average)", "bytes_per_cell": BYTES_PER_CELL_FP32},
+        "roofline": roofline(chosen, ch, cells, achieved),
+        "roofline_plain_fp32": roofline("aa_fp32", forms["aa_fp32"], cells, achieved),
         "cpu_baseline": cpu,
         "forms": forms,
         "l2_err": l2,
