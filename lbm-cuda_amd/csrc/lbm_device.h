@@ -106,20 +106,35 @@ template <bool EXT = false>
 __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
                                             float w_ext = -1.0f)
 {
-    float rho = 0.f, ux = 0.f, uy = 0.f, drc = 0.f, rc = 0.f, inv_rc = 0.f;
-    bool les = false;
+    // one if/else that defines every value on both sides: defaults set before the branch would be
+    // materialised for every lane, while the inlet side runs only in waves that hold inlet cells
+    float rho, ux, uy, drc, rc, inv_rc;
+    float s13, s24, s57, s68, a13, a24, a57, a86, h14, h58, sx, sy;
+    bool les;
+    auto sums = [&]() {
+        s13 = h[1] + h[3];
+        s24 = h[2] + h[4];
+        s57 = h[5] + h[7];
+        s68 = h[6] + h[8];
+        a13 = h[1] - h[3];
+        a24 = h[2] - h[4];
+        a57 = h[5] - h[7];
+        a86 = h[8] - h[6];
+        h14 = s13 + s24;
+        h58 = s57 + s68;
+        sx = a57 + a86; // f5 - f6 - f7 + f8
+        sy = a57 - a86; // f5 + f6 - f7 - f8
+    };
     if (inlet) {
         drc = zou_he(h, uin, rho);
         rc = rho;
         ux = uin;
         uy = 0.0f;
-    }
-    const float s13 = h[1] + h[3], s24 = h[2] + h[4], s57 = h[5] + h[7], s68 = h[6] + h[8];
-    const float h14 = s13 + s24, h58 = s57 + s68;
-    const float a13 = h[1] - h[3], a24 = h[2] - h[4], a57 = h[5] - h[7], a86 = h[8] - h[6];
-    const float sx = a57 + a86; // f5 - f6 - f7 + f8
-    const float sy = a57 - a86; // f5 + f6 - f7 - f8
-    if (!inlet) {
+        inv_rc = 0.0f; // unused: no LES at the inlet
+        les = false;
+        sums();
+    } else {
+        sums();
         // moments and clamps (lbm_core4.cu:130-144); rho = 1 + drho
         const float drho = (h[0] + h14) + h58;
         rho = 1.0f + drho;
