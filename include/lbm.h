@@ -145,6 +145,16 @@ int lbm_step_fused_cfg(const lbm_params *p, const uint8_t *mask, const float *f_
                        float *rho, lbm_stream_t stream);
 
 /*
+ * nlaunch consecutive lbm_step_fused_cfg launches on `stream`, alternating between the two buffers: launch i
+ * reads f_a (i even) or f_b (i odd), writes the other one, and starts at parity (parity + i k) & 1.  The state
+ * ends in f_b when nlaunch is odd, in f_a otherwise.  Issued from C so that a long run costs no
+ * host-language overhead per launch; replaces the replay loop of the reference's captured graph of two fused
+ * launches (lbm_smemaa_main.py:136-151,173-174).
+ */
+int lbm_step_fused_n(const lbm_params *p, const uint8_t *mask, float *f_a, float *f_b, const float *inlet_u,
+                     int parity, int k, int nlaunch, const lbm_fused_cfg *cfg, lbm_stream_t stream);
+
+/*
  * 16-bit deviation storage (no reference counterpart: the reference stores FP32 only).  Same semantics as
  * lbm_step_aa / lbm_step_aa_n / lbm_step_fused_cfg / lbm_macro on h = f - w stored as `fmt`
  * (LBM_FMT_F16 or LBM_FMT_BF16); stores clamp h to [-w, 10 - w], the population clamp of lbm_core4.cu:203.
@@ -156,6 +166,9 @@ int lbm_step_aa_h16_n(const lbm_params *p, const uint8_t *mask, uint16_t *h, int
 int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h_now, uint16_t *h_out, int fmt,
                        const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
                        float *rho, lbm_stream_t stream);
+int lbm_step_fused_h16_n(const lbm_params *p, const uint8_t *mask, uint16_t *h_a, uint16_t *h_b, int fmt,
+                         const float *inlet_u, int parity, int k, int nlaunch, const lbm_fused_cfg *cfg,
+                         lbm_stream_t stream);
 int lbm_macro_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h, int fmt, const float *inlet_u,
                   int parity, float *ux, float *uy, float *rho, lbm_stream_t stream);
 

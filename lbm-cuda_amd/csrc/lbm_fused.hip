@@ -127,4 +127,31 @@ int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t 
     return check_launch("lbm_step_fused_h16");
 }
 
+int lbm_step_fused_n(const lbm_params *p, const uint8_t *mask, float *f_a, float *f_b, const float *inlet_u,
+                     int parity, int k, int nlaunch, const lbm_fused_cfg *cfg, lbm_stream_t stream)
+{
+    if (nlaunch < 0) return fail(LBM_ERR_ARG, "lbm_step_fused_n: nlaunch must be >= 0");
+    for (int i = 0; i < nlaunch; ++i) {
+        float *src = i & 1 ? f_b : f_a, *dst = i & 1 ? f_a : f_b;
+        int rc = lbm_step_fused_cfg(p, mask, src, dst, inlet_u, (parity + i * k) & 1, k, cfg, nullptr, nullptr,
+                                    nullptr, stream);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
+int lbm_step_fused_h16_n(const lbm_params *p, const uint8_t *mask, uint16_t *h_a, uint16_t *h_b, int fmt,
+                         const float *inlet_u, int parity, int k, int nlaunch, const lbm_fused_cfg *cfg,
+                         lbm_stream_t stream)
+{
+    if (nlaunch < 0) return fail(LBM_ERR_ARG, "lbm_step_fused_h16_n: nlaunch must be >= 0");
+    for (int i = 0; i < nlaunch; ++i) {
+        uint16_t *src = i & 1 ? h_b : h_a, *dst = i & 1 ? h_a : h_b;
+        int rc = lbm_step_fused_h16(p, mask, src, dst, fmt, inlet_u, (parity + i * k) & 1, k, cfg, nullptr, nullptr,
+                                    nullptr, stream);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
 } // extern "C"

@@ -93,6 +93,8 @@ SIGNATURES = {
     "lbm_ensemble_side": (_i, [_vp]),
     "lbm_ensemble_pair_info": (_i, [_vp, _i, _vp, _vp]),
     "lbm_selftest": (_i, [_i, _vp, _vp, _vp]),
+    "lbm_step_fused_n": (_i, [_P, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
+    "lbm_step_fused_h16_n": (_i, [_P, _vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp, _vp]),
     "lbm_selftest_threads": (_i, []),
     "lbm_graph_begin": (_i, [_vp]),
     "lbm_graph_end": (_i, [_vp, _vp]),

@@ -99,18 +99,57 @@ class Simulation:
     # ------------------------------------------------------------------ stepping
     def step(self, n=1, k=1, macros=False, cfg=None):
         """Advance n collide-stream steps; k > 1 uses the temporally blocked kernel (k steps per launch).
-        ``cfg`` = (strip, chunk) selects its launch configuration (default: the library's choice)."""
+        ``cfg`` = (strip, chunk) selects its launch configuration (default: the library's choice).  Runs of
+        several launches are issued by one C call (lbm_step_aa_n / lbm_step_fused_n); with ``macros`` only
+        the last launch writes them."""
         n = int(n)
         k = int(k)
         if k <= 1:
-            for _ in range(n):
+            if n > 1:
+                self._aa_n(n - 1 if macros else n)
+                if macros:
+                    self._aa(True)
+            elif n == 1:
                 self._aa(macros)
             return self
         if n % k:
             raise ValueError(f"n={n} is not a multiple of k={k}")
-        for _ in range(n // k):
+        nl = n // k
+        if nl > 1:
+            self._fused_n(k, nl - 1 if macros else nl, cfg)
+            if macros:
+                self._fused(k, True, cfg)
+        elif nl == 1:
             self._fused(k, macros, cfg)
         return self
+
+    def _aa_n(self, n):
+        if self.fmt == _lib.FMT_F32:
+            rc = self.lib.lbm_step_aa_n(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.inlet_u),
+                                        self.parity, n, _stream())
+        else:
+            rc = self.lib.lbm_step_aa_h16_n(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), self.fmt,
+                                            _ptr(self.inlet_u), self.parity, n, _stream())
+        _lib.check(rc, "lbm_step_aa_n")
+        self.parity = (self.parity + n) & 1
+        self.steps += n
+
+    def _fused_n(self, k, nl, cfg=None):
+        if self.f_alt is None:
+            self.f_alt = torch.empty_like(self.f)
+        c = None if cfg is None else ctypes.byref(_lib.LbmFusedCfg(int(cfg[0]), int(cfg[1])))
+        if self.fmt == _lib.FMT_F32:
+            rc = self.lib.lbm_step_fused_n(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.f_alt),
+                                           _ptr(self.inlet_u), self.parity, k, nl, c, _stream())
+        else:
+            rc = self.lib.lbm_step_fused_h16_n(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f),
+                                               _ptr(self.f_alt), self.fmt, _ptr(self.inlet_u), self.parity, k, nl, c,
+                                               _stream())
+        _lib.check(rc, "lbm_step_fused_n")
+        if nl & 1:
+            self.f, self.f_alt = self.f_alt, self.f
+        self.parity = (self.parity + k * nl) & 1
+        self.steps += k * nl
 
     def _aa(self, macros=False):
         m = (self.ux, self.uy, self.rho) if macros else (None, None, None)
