@@ -98,13 +98,31 @@ __device__ __forceinline__ float zou_he(float h[9], float u, float &rho)
     return drho;
 }
 
+// Non-inline constants of the collision and of the 16-bit store clamp.  A VOP3 instruction reads at most one
+// scalar register or literal, so med3 / select with two such constants needs one of them in a VGPR; in
+// straight-line code the compiler rematerialises it (a v_mov) at every use.  A kernel that collides many cells
+// per thread in straight-line code takes opaque_consts() once: its values stay in VGPRs.
+struct CollideK {
+    float ulim = 0.4f;                                    // velocity clamp
+    float inv15 = __builtin_bit_cast(float, 0x3F2AAAABu); // 1 / 1.5 rounded
+    float inf = __builtin_inff();                         // 1 / 0
+    float hi[3] = {10.0f - LBM_W0, 10.0f - LBM_W1, 10.0f - LBM_W5}; // the population clamp 10 - w_q per weight
+};
+__device__ __forceinline__ CollideK opaque_consts()
+{
+    CollideK k;
+    asm volatile("" : "+v"(k.ulim), "+v"(k.inv15), "+v"(k.inf), "+v"(k.hi[0]), "+v"(k.hi[1]), "+v"(k.hi[2]));
+    return k;
+}
+__device__ __forceinline__ constexpr int wclass(int q) { return q == 0 ? 0 : (q < 5 ? 1 : 2); }
+
 // MRT + Smagorinsky collision (lbm_core4.cu:124-199) on deviations, h -> h' (unclamped).  `inlet`
 // selects Zou-He and the inlet collision of lbm_core4.cu:54-93 (no LES, no clamps, m3 = rho u, m5 = 0).
 // mac = (rho, ux, uy) as written by the reference (lbm_core4.cu:59-61,145-147).
 // EXT: the stress moments relax at w_ext when w_ext >= 0 (a learned closure's rate), else as usual.
 template <bool EXT = false>
 __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
-                                            float w_ext = -1.0f)
+                                            float w_ext = -1.0f, const CollideK &K = CollideK())
 {
     // one if/else that defines every value on both sides: defaults set before the branch would be
     // materialised for every lane, while the inlet side runs only in waves that hold inlet cells
@@ -144,14 +162,14 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
         uy = jy * inv_rho;
         drc = drho;
         if (P.clamp) {
-            ux = clampf(ux, -0.4f, 0.4f);
-            uy = clampf(uy, -0.4f, 0.4f);
+            ux = clampf(ux, -K.ulim, K.ulim);
+            uy = clampf(uy, -K.ulim, K.ulim);
             drc = clampf(drho, -1.0f, 0.5f); // rho in [0, 1.5]
         }
         rc = 1.0f + drc;
         // 1/rc == 1/rho bit for bit unless the density clamp engaged, and then rc is 1.5 or 0 exactly:
         // 1/1.5 rounds to 0x3F2AAAAB and 1/0 is +inf, so no second division is needed
-        inv_rc = drc == drho ? inv_rho : (drc > 0.0f ? __builtin_bit_cast(float, 0x3F2AAAABu) : __builtin_inff());
+        inv_rc = drc == drho ? inv_rho : (drc > 0.0f ? K.inv15 : K.inf);
         les = P.les != 0;
     }
     mac[0] = rc;
@@ -256,12 +274,12 @@ __device__ __forceinline__ void collide_bgk(const KPar &P, float h[9], bool inle
 
 template <int MODEL, bool EXT = false>
 __device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
-                                            float w_ext = -1.0f)
+                                            float w_ext = -1.0f, const CollideK &K = CollideK())
 {
     if constexpr (MODEL == MODEL_BGK)
         collide_bgk(P, h, inlet, uin, mac);
     else
-        collide_mrt<EXT>(P, h, inlet, uin, mac, w_ext);
+        collide_mrt<EXT>(P, h, inlet, uin, mac, w_ext, K);
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -292,6 +310,7 @@ template <> struct Conv<FMT_F32> {
     static constexpr unsigned BYTES = 4;
     __device__ static float ld(T v, int q) { return v - wq(q); }
     __device__ static T st(float h, int q) { return clampf(wq(q) + h, 0.0f, 10.0f); }
+    __device__ static T st(float h, int q, const CollideK &) { return st(h, q); }
     __device__ static float pop(T v, int) { return v; } // the population f itself
     __device__ static T from_pop(float f, int) { return f; }
 };
@@ -304,6 +323,10 @@ template <> struct Conv<FMT_F16> {
     {
         return __builtin_bit_cast(uint16_t, (_Float16)clampf(h, -wq(q), 10.0f - wq(q)));
     }
+    __device__ static T st(float h, int q, const CollideK &K)
+    {
+        return __builtin_bit_cast(uint16_t, (_Float16)clampf(h, -wq(q), K.hi[wclass(q)]));
+    }
     __device__ static float pop(T v, int q) { return wq(q) + ld(v, q); }
     __device__ static T from_pop(float f, int q) { return __builtin_bit_cast(uint16_t, (_Float16)(f - wq(q))); }
 };
@@ -315,6 +338,10 @@ template <> struct Conv<FMT_BF16> {
     __device__ static T st(float h, int q)
     {
         return __builtin_bit_cast(uint16_t, (__bf16)clampf(h, -wq(q), 10.0f - wq(q)));
+    }
+    __device__ static T st(float h, int q, const CollideK &K)
+    {
+        return __builtin_bit_cast(uint16_t, (__bf16)clampf(h, -wq(q), K.hi[wclass(q)]));
     }
     __device__ static float pop(T v, int q) { return wq(q) + ld(v, q); }
     __device__ static T from_pop(float f, int q) { return __builtin_bit_cast(uint16_t, (__bf16)(f - wq(q))); }

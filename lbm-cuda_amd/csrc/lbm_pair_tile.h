@@ -44,6 +44,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     const rsrc_t rin = make_rsrc(fin, 9u * plane * B), rout = make_rsrc(fout, 9u * plane * B);
     const rsrc_t rm = make_rsrc(mask, plane);
     const int t = threadIdx.x;
+    const CollideK K = opaque_consts(); // kept in VGPRs across the unrolled cells
 
     // ---- phase A: even step of every region cell, reversed result into LDS.  All of a thread's HBM loads
     //      (NA cells x (mask byte + 9 populations)) are issued before anything uses them, so their latencies
@@ -62,7 +63,8 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             gx = wrap(gx, W);
             gy = wrap(gy, H);
         } else {
-            in = in && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            // 0 <= v < n as one unsigned comparison (n > 0)
+            in = in && (unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H;
         }
         // unconditional (cells outside the lattice read cell 0 and discard it), so no load waits in a branch
         const unsigned idx = in ? (unsigned)(gy * W + gx) : 0u;
@@ -71,8 +73,8 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         for (int q = 0; q < 9; ++q) ga[a][q] = S::ld(rin, idx * B, (unsigned)q * plane * B, q);
         ina[a] = in;
         la[a] = !PERIODIC && gx == 0;
-        // rows and columns the even step updates (all of them when periodic)
-        ua[a] = PERIODIC || (gy >= 1 && gy <= H - 2 && gx <= W - 2);
+        // rows and columns the even step updates (all of them when periodic): 1 <= gy <= H - 2, gx <= W - 2
+        ua[a] = PERIODIC || ((unsigned)(gy - 1) < (unsigned)(H - 2) && gx <= W - 2);
         ia[a] = (unsigned)gy; // the row, for the inlet speed
     }
 #pragma unroll
@@ -84,10 +86,10 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         const bool inlet = la[a];
         if (!(ina[a] && ua[a] && (inlet || !solid))) continue;
         float mac[3];
-        collide_dev<MODEL>(P, ga[a], inlet, inlet ? inlet_u[ia[a]] : 0.0f, mac);
+        collide_dev<MODEL>(P, ga[a], inlet, inlet ? inlet_u[ia[a]] : 0.0f, mac, -1.0f, K);
         T o[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) o[q] = C::st(ga[a][q], q);
+        for (int q = 0; q < 9; ++q) o[q] = C::st(ga[a][q], q, K);
 #pragma unroll
         for (int q = 0; q < 9; ++q) sf[opp(q)][r][c] = o[q];
         if (!PERIODIC && x0 - 1 + c == W - 2 && c + 1 < RW) { // reversed copy into the outlet ghost
@@ -101,7 +103,13 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     for (int e = t; e < CH * CW; e += NT) {
         const int r = 1 + e / CW, c = 1 + (e - (e / CW) * CW);
         const int gx = x0 + c - 1, gy = y0 + r - 1;
-        if (gx >= W || gy >= H) continue;
+        if constexpr (PERIODIC) {
+            if (gx >= W || gy >= H) continue;
+        } else {
+            // updated cells only: 1 <= gy <= H - 2 and gx <= W - 2 (walls, the outlet ghost and cells past the
+            // lattice are not updated; gx, gy >= 0 here)
+            if ((unsigned)(gy - 1) >= (unsigned)(H - 2) || gx >= W - 1) continue;
+        }
         // LDS reads relative to the corner (r - 1, c - 1) of the cell's 3 x 3 neighbourhood: every access is
         // that base plus a non-negative constant, which the DS instructions take as an immediate offset
         const int lb = (r - 1) * LS + (c - 1);
@@ -110,7 +118,6 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         auto at = [&](int k, int dy, int dx) -> T { return sfb[k * RH * LS + (1 + dy) * LS + (1 + dx)]; };
         bool inlet = false;
         if constexpr (!PERIODIC) {
-            if (gy == 0 || gy == H - 1 || gx == W - 1) continue; // walls and the outlet ghost are not updated
             inlet = gx == 0;
             if (!inlet && smb[LS + 1]) continue;
         } else {
@@ -130,7 +137,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             const T own = at(q, 0, 0), src = at(o, -cy(q), -cx(q));
             g[q] = C::ld(solid[o] ? own : src, q);
         }
-        collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac);
+        collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac, -1.0f, K);
         const unsigned idx = (unsigned)(gy * W + gx);
         const unsigned idxB = idx * B, pB = plane * B;
         const unsigned idx2 = idxB + 2u * pB;
