@@ -269,9 +269,21 @@ def bench_precision(steps_full, steps_scaled=1000):
     m64 = O.macro(op, sc.mask, f64, np.zeros(sc.H), par)
     scaled = {"W": sc.W, "H": sc.H, "steps": steps_scaled, "vs": "FP64 oracle",
               "oracle_seconds": round(time.perf_counter() - t1, 1)}
+    plain = {}
     for st, s in sims(sc).items():
         s.step(steps_scaled)
         scaled[st] = _l2_fields(macros(s), m64)
+        plain[st] = s.f.clone()
+    # the same with compensated (Neumaier) sums of rho - 1 and j, and how many stored values they change
+    for st in ("f32", "f16", "bf16"):
+        s = Simulation(sc.W, sc.H, sc.tau_inv, **sc.sim_kwargs(), storage=st, sums="neumaier")
+        s.step(steps_scaled)
+        r = _l2_fields(macros(s), m64)
+        r["values_differing_from_plain"] = int((s.f != plain[st]).sum())
+        r["values"] = int(s.f.numel())
+        scaled[f"{st}_neumaier"] = r
+        del s
+    del plain
     out["shear_layer_scaled"] = scaled
     log(f"precision: scaled shear layer vs FP64 {scaled}")
     return out

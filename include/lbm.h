@@ -73,6 +73,9 @@ typedef struct lbm_params {
     float tau_inv;  /* relaxation rate of the stress moments (MRT) / BGK rate */
     float cs;       /* Smagorinsky constant; 0 = plain MRT (no LES) */
     float s_e, s_eps, s_q; /* MRT rates of e, eps and q_x/q_y (reference: 1.1) */
+    int32_t sums;   /* 0: plain FP32 sums of rho - 1 and j in the MRT collision; 1: compensated (Neumaier) sums of
+                       both, for the precision study.  1 is supported by the single-step entry points (lbm_step_aa*)
+                       only; the others return LBM_ERR_UNSUPPORTED */
 } lbm_params;
 
 /* ABI version of this header (bumped on any signature change). */

@@ -141,8 +141,8 @@ int lbm_neq(const lbm_params *p, const uint8_t *mask, const float *f, int parity
     KPar P;
     int rc = make_kpar(p, &P);
     if (rc) return rc;
-    if (p->bc != LBM_BC_PERIODIC || p->model != LBM_MRT_LES)
-        return fail(LBM_ERR_UNSUPPORTED, "lbm_neq: periodic MRT lattices only");
+    if (p->bc != LBM_BC_PERIODIC || p->model != LBM_MRT_LES || P.sums)
+        return fail(LBM_ERR_UNSUPPORTED, "lbm_neq: periodic MRT lattices with plain sums only");
     if (!mask || !f || !neq) return fail(LBM_ERR_ARG, "lbm_neq: NULL buffer");
     if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_neq: parity must be 0 or 1");
     dim3 grid((P.W + TILE_X - 1) / TILE_X, (P.H + TILE_Y - 1) / TILE_Y);

@@ -23,7 +23,8 @@ __host__ __device__ constexpr int opp(int q) { return q == 0 ? 0 : (q <= 4 ? ((q
 static_assert(opp(1) == 3 && opp(2) == 4 && opp(3) == 1 && opp(4) == 2, "opp");
 static_assert(opp(5) == 7 && opp(6) == 8 && opp(7) == 5 && opp(8) == 6, "opp");
 
-enum { MODEL_MRT = 0, MODEL_BGK = 1 };
+// MODEL_MRT_SUMS: MRT + LES with compensated (Neumaier) sums of rho - 1 and j (lbm_params.sums = 1)
+enum { MODEL_MRT = 0, MODEL_BGK = 1, MODEL_MRT_SUMS = 2 };
 
 // Kernel-side parameters, derived on the host in FP32 (tau0 = 1/tau_inv etc., as lbm_core4.cu:175-176).
 struct KPar {
@@ -32,6 +33,7 @@ struct KPar {
     int les;   // MRT: Smagorinsky on (cs != 0)
     int clamp; // MRT: velocity and density clamps
     float tau_inv, tau0, tau0sq, c18cs2, s_e, s_eps, s_q;
+    int sums; // lbm_params.sums (single-step kernels only)
 };
 
 // The reference's FP32 literals (lbm_core4.cu:15-16,55-57; lbm_core_smemaa.cu:4-7)
@@ -116,11 +118,44 @@ __device__ __forceinline__ CollideK opaque_consts()
 }
 __device__ __forceinline__ constexpr int wclass(int q) { return q == 0 ? 0 : (q < 5 ? 1 : 2); }
 
+// Neumaier's compensated summation (Neumaier 1974, the improved Kahan-Babuska scheme): s + t rounded, the
+// rounding error of that addition accumulated in c; the sum is s + c at the end.
+__device__ __forceinline__ void neu_add(float &s, float &c, float t)
+{
+    const float u = s + t;
+    c += __builtin_fabsf(s) >= __builtin_fabsf(t) ? (s - u) + t : (t - u) + s;
+    s = u;
+}
+
+// rho - 1, jx, jy as compensated sums over the deviations, terms in index order
+__device__ __forceinline__ void moments_neu(const float h[9], float &drho, float &jx, float &jy)
+{
+    float s = h[0], c = 0.0f;
+#pragma unroll
+    for (int q = 1; q < 9; ++q) neu_add(s, c, h[q]);
+    drho = s + c;
+    s = h[1], c = 0.0f;
+    neu_add(s, c, -h[3]);
+    neu_add(s, c, h[5]);
+    neu_add(s, c, -h[6]);
+    neu_add(s, c, -h[7]);
+    neu_add(s, c, h[8]);
+    jx = s + c;
+    s = h[2], c = 0.0f;
+    neu_add(s, c, -h[4]);
+    neu_add(s, c, h[5]);
+    neu_add(s, c, h[6]);
+    neu_add(s, c, -h[7]);
+    neu_add(s, c, -h[8]);
+    jy = s + c;
+}
+
 // MRT + Smagorinsky collision (lbm_core4.cu:124-199) on deviations, h -> h' (unclamped).  `inlet`
 // selects Zou-He and the inlet collision of lbm_core4.cu:54-93 (no LES, no clamps, m3 = rho u, m5 = 0).
 // mac = (rho, ux, uy) as written by the reference (lbm_core4.cu:59-61,145-147).
 // EXT: the stress moments relax at w_ext when w_ext >= 0 (a learned closure's rate), else as usual.
-template <bool EXT = false>
+// SUMS: rho - 1 and j as compensated sums (moments_neu) instead of the pairwise FP32 sums.
+template <bool EXT = false, bool SUMS = false>
 __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
                                             float w_ext = -1.0f, const CollideK &K = CollideK())
 {
@@ -154,9 +189,10 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
     } else {
         sums();
         // moments and clamps (lbm_core4.cu:130-144); rho = 1 + drho
-        const float drho = (h[0] + h14) + h58;
+        float drho = (h[0] + h14) + h58;
+        float jx = a13 + sx, jy = a24 + sy;
+        if constexpr (SUMS) moments_neu(h, drho, jx, jy);
         rho = 1.0f + drho;
-        const float jx = a13 + sx, jy = a24 + sy;
         const float inv_rho = recip_rn(rho);
         ux = jx * inv_rho;
         uy = jy * inv_rho;
@@ -279,7 +315,7 @@ __device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inle
     if constexpr (MODEL == MODEL_BGK)
         collide_bgk(P, h, inlet, uin, mac);
     else
-        collide_mrt<EXT>(P, h, inlet, uin, mac, w_ext, K);
+        collide_mrt<EXT, MODEL == MODEL_MRT_SUMS>(P, h, inlet, uin, mac, w_ext, K);
 }
 
 // ------------------------------------------------------------------------------------------------------

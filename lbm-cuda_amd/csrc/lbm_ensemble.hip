@@ -227,6 +227,10 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
             delete e;
             return rc;
         }
+        if (d.P.sums) {
+            delete e;
+            return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_create: compensated sums are single-step (AA) only");
+        }
         if (inst[i].p.model != e->model || inst[i].p.bc != e->bc) {
             delete e;
             return fail(LBM_ERR_ARG, "lbm_ensemble_create: instance %d differs in model or bc", i);

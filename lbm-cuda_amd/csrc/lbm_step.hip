@@ -66,6 +66,8 @@ int make_kpar(const lbm_params *p, KPar *k)
     k->s_e = p->s_e;
     k->s_eps = p->s_eps;
     k->s_q = p->s_q;
+    if (p->sums != 0 && p->sums != 1) return fail(LBM_ERR_ARG, "sums %d must be 0 or 1", p->sums);
+    k->sums = p->sums;
     return LBM_OK;
 }
 
@@ -137,6 +139,8 @@ static void dispatch_aa_fmt(const KPar &P, int model, int bc, int parity, dim3 g
 {
     if (model == LBM_BGK)
         dispatch_aa<MODEL_BGK, FMT>(P, bc, parity, grid, s, mask, f, inlet_u, ux, uy, rho);
+    else if (P.sums)
+        dispatch_aa<MODEL_MRT_SUMS, FMT>(P, bc, parity, grid, s, mask, f, inlet_u, ux, uy, rho);
     else
         dispatch_aa<MODEL_MRT, FMT>(P, bc, parity, grid, s, mask, f, inlet_u, ux, uy, rho);
 }
@@ -376,7 +380,7 @@ using namespace lbm;
 
 extern "C" {
 
-int lbm_abi_version(void) { return 1; }
+int lbm_abi_version(void) { return 2; }
 
 const char *lbm_last_error(void) { return g_err; }
 
@@ -419,6 +423,7 @@ int lbm_step_pull(const lbm_params *p, const uint8_t *mask, const float *f_now, 
     KPar P;
     int rc = make_kpar(p, &P);
     if (rc) return rc;
+    if (P.sums) return fail(LBM_ERR_UNSUPPORTED, "lbm_step_pull: compensated sums are single-step (AA) only");
     if (!mask || !f_now || !f_out || !ux || !uy || !rho)
         return fail(LBM_ERR_ARG, "lbm_step_pull: mask, f_now, f_out, ux, uy, rho must be non-NULL");
     if (f_now == f_out) return fail(LBM_ERR_ARG, "lbm_step_pull: f_now and f_out must differ");

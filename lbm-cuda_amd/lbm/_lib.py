@@ -15,7 +15,7 @@ MRT_LES, BGK = 0, 1
 CHANNEL, PERIODIC = 0, 1
 FMT_F32, FMT_F16, FMT_BF16 = 0, 1, 2
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class LbmParams(ctypes.Structure):
@@ -32,6 +32,7 @@ class LbmParams(ctypes.Structure):
         ("s_e", ctypes.c_float),
         ("s_eps", ctypes.c_float),
         ("s_q", ctypes.c_float),
+        ("sums", ctypes.c_int32),
     ]
 
 
@@ -139,6 +140,6 @@ def check(rc, what):
         raise LbmError(f"{what} failed ({rc}): {msg}")
 
 
-def params(W, H, tau_inv, cs=0.0, model=MRT_LES, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1):
+def params(W, H, tau_inv, cs=0.0, model=MRT_LES, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1, sums=0):
     return LbmParams(int(W), int(H), int(model), int(bc), int(clamp), float(tau_inv), float(cs), float(s_e),
-                     float(s_eps), float(s_q))
+                     float(s_eps), float(s_q), int(sums))

@@ -48,13 +48,14 @@ class Simulation:
     """
 
     def __init__(self, W, H, tau_inv, *, cs=0.0, model="mrt", bc="channel", mask=None, inlet_u=None, f0=None,
-                 clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda", storage="f32"):
+                 clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda", storage="f32", sums="plain"):
         self.lib = _lib.load()
         self.W, self.H = int(W), int(H)
         self.model = _MODELS[model] if isinstance(model, str) else int(model)
         self.bc = _BCS[bc] if isinstance(bc, str) else int(bc)
+        # sums="neumaier": compensated sums of rho - 1 and j in the MRT collision (single steps only)
         self.params = _lib.params(self.W, self.H, tau_inv, cs, self.model, self.bc, int(bool(clamp)), s_e, s_eps,
-                                  s_q)
+                                  s_q, {"plain": 0, "neumaier": 1}[sums])
         self.device = torch.device(device)
         if mask is None:
             mask = geo.channel_mask(W, H) if self.bc == _lib.CHANNEL else geo.periodic_mask(W, H)

@@ -40,12 +40,13 @@ class OracleParams(ctypes.Structure):
         ("s_e", ctypes.c_float),
         ("s_eps", ctypes.c_float),
         ("s_q", ctypes.c_float),
+        ("sums", ctypes.c_int32),
     ]
 
 
-def params(W, H, tau_inv, cs=0.0, model=MRT, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1):
+def params(W, H, tau_inv, cs=0.0, model=MRT, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1, sums=0):
     return OracleParams(int(W), int(H), int(model), int(bc), int(clamp), float(tau_inv), float(cs),
-                        float(s_e), float(s_eps), float(s_q))
+                        float(s_e), float(s_eps), float(s_q), int(sums))
 
 
 def build(verbose=False):
@@ -77,6 +78,8 @@ def lib():
             getattr(L, "oracle_aa_step_rate" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
             getattr(L, "oracle_neq" + suf).argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_neumaier_f32.argtypes = [vp, ctypes.c_int]
+        L.oracle_neumaier_f32.restype = ctypes.c_float
         L.oracle_mrt_forward_f64.argtypes = [vp, vp]
         L.oracle_mrt_inverse_f64.argtypes = [vp, vp]
         L.oracle_diagnostics_f32.argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
@@ -277,3 +280,10 @@ def observable(p, mask, parity, ghost=False):
 def reverse(f):
     """Read a reversed (odd-parity) state as post-collision populations: out[q] = f[opp q]."""
     return f[OPP]
+
+
+def neumaier_f32(x):
+    """Neumaier-compensated FP32 sum of x in index order: the summation of the collision with sums = 1."""
+    import numpy as np
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    return float(lib().oracle_neumaier_f32(a.ctypes.data_as(ctypes.c_void_p), int(a.size)))

@@ -31,6 +31,7 @@ typedef struct {
     float tau_inv;
     float cs;
     float s_e, s_eps, s_q;
+    int32_t sums; /* 1: compensated (Neumaier) sums of rho - 1 and j in the MRT collision */
 } oracle_params;
 
 /* ---------------------------- FP64 oracle ---------------------------- */
@@ -38,6 +39,7 @@ typedef struct {
 #define SUFFIX _f64
 #define FMA fma
 #define SQRT sqrt
+#define FABS fabs
 #define C_2_3 (2.0 / 3.0)
 #define C_1_6 (1.0 / 6.0)
 #define C_INV9 (1.0 / 9.0)
@@ -51,6 +53,7 @@ typedef struct {
 #undef SUFFIX
 #undef FMA
 #undef SQRT
+#undef FABS
 #undef C_2_3
 #undef C_1_6
 #undef C_INV9
@@ -66,6 +69,7 @@ typedef struct {
 #define SUFFIX _f32
 #define FMA fmaf
 #define SQRT sqrtf
+#define FABS fabsf
 #define C_2_3 0.666666667f
 #define C_1_6 0.16666666666f
 #define C_INV9 0.1111111111f
@@ -319,4 +323,15 @@ int oracle_set_threads(int n)
     if (!dflt) dflt = omp_get_max_threads();
     omp_set_num_threads(n > 0 ? n : dflt);
     return omp_get_max_threads();
+}
+
+
+/* Neumaier-compensated FP32 sum of x[0..n) in index order (the summation the collision uses with sums = 1);
+   exported for the tests that pin it against an exactly rounded sum. */
+float oracle_neumaier_f32(const float *x, int n)
+{
+    if (n <= 0) return 0.0f;
+    float s = x[0], c = 0.0f;
+    for (int i = 1; i < n; ++i) neu_add_f32(&s, &c, x[i]);
+    return s + c;
 }

@@ -38,7 +38,7 @@
 /* ---------------------------------------------------------------------------------------------------- */
 
 typedef struct {
-    int W, H, model, bc, clamp, les;
+    int W, H, model, bc, clamp, les, sums;
     REAL tau_inv, tau0, tau0sq, c18cs2, s_e, s_eps, s_q;
 } FN(kpar_t);
 
@@ -49,6 +49,7 @@ static void FN(make_kpar)(const oracle_params *p, FN(kpar_t) * k)
     k->model = p->model;
     k->bc = p->bc;
     k->clamp = p->clamp;
+    k->sums = p->sums;
     k->les = (p->model == ORACLE_MRT && p->cs != 0.0f) ? 1 : 0;
     k->tau_inv = (REAL)p->tau_inv;
     /* tau_0 = 1/tau_inv and 18*Cs*Cs as in lbm_core4.cu:175-176, evaluated once in the working precision */
@@ -90,6 +91,37 @@ static inline REAL FN(zou_he)(REAL *h, REAL uin, REAL *rho)
     return drho;
 }
 
+/* Neumaier's compensated summation (Neumaier 1974, the improved Kahan-Babuska scheme): s + t rounded, the
+   rounding error of that addition accumulated in c; the sum is s + c at the end. */
+static inline void FN(neu_add)(REAL *s, REAL *c, REAL t)
+{
+    REAL u = *s + t;
+    *c += FABS(*s) >= FABS(t) ? (*s - u) + t : (t - u) + *s;
+    *s = u;
+}
+
+/* rho - 1, jx, jy as compensated sums over the deviations, terms in index order */
+static void FN(moments_neu)(const REAL *h, REAL *drho, REAL *jx, REAL *jy)
+{
+    REAL s = h[0], c = 0;
+    for (int q = 1; q < 9; ++q) FN(neu_add)(&s, &c, h[q]);
+    *drho = s + c;
+    s = h[1], c = 0;
+    FN(neu_add)(&s, &c, -h[3]);
+    FN(neu_add)(&s, &c, h[5]);
+    FN(neu_add)(&s, &c, -h[6]);
+    FN(neu_add)(&s, &c, -h[7]);
+    FN(neu_add)(&s, &c, h[8]);
+    *jx = s + c;
+    s = h[2], c = 0;
+    FN(neu_add)(&s, &c, -h[4]);
+    FN(neu_add)(&s, &c, h[5]);
+    FN(neu_add)(&s, &c, h[6]);
+    FN(neu_add)(&s, &c, -h[7]);
+    FN(neu_add)(&s, &c, -h[8]);
+    *jy = s + c;
+}
+
 /* w_ext >= 0: the stress moments relax at w_ext (an external closure's rate) instead of the model's own */
 static void FN(collide_mrt)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin, REAL *mac, REAL w_ext)
 {
@@ -109,9 +141,10 @@ static void FN(collide_mrt)(const FN(kpar_t) * P, REAL h[9], int inlet, REAL uin
     if (!inlet) {
         /* moments and clamps, lbm_core4.cu:130-144 (rho = 1 + drho) */
         REAL drho = (h[0] + h14) + h58;
-        rho = (REAL)1 + drho;
         REAL jx = a13 + sx;
         REAL jy = a24 + sy;
+        if (P->sums) FN(moments_neu)(h, &drho, &jx, &jy);
+        rho = (REAL)1 + drho;
         REAL inv_rho = (REAL)1 / rho;
         ux = jx * inv_rho;
         uy = jy * inv_rho;

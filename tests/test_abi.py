@@ -34,9 +34,19 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_params_struct_layout():
-    # int32 W, H, model, bc, clamp; float tau_inv, cs, s_e, s_eps, s_q
-    assert ctypes.sizeof(_lib.LbmParams) == 40
+    # int32 W, H, model, bc, clamp; float tau_inv, cs, s_e, s_eps, s_q; int32 sums
+    assert ctypes.sizeof(_lib.LbmParams) == 44
     assert _lib.LbmParams.tau_inv.offset == 20
+    assert _lib.LbmParams.sums.offset == 40
+    import oracle as O
+    assert ctypes.sizeof(O.OracleParams) == 44 and O.OracleParams.sums.offset == 40
+
+
+def test_sums_argument_checked():
+    L = lbm.load()
+    p = _lib.params(16, 8, 1.5, sums=2)
+    assert L.lbm_step_aa(ctypes.byref(p), None, None, None, 0, None, None, None, None) == -1
+    assert b"sums" in L.lbm_last_error()
 
 
 def test_argument_errors_return_codes():

@@ -18,7 +18,7 @@ def _oparams(case_or_sim, **kw):
     s = case_or_sim
     p = s.params
     return O.params(p.W, p.H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc, clamp=p.clamp, s_e=p.s_e, s_eps=p.s_eps,
-                    s_q=p.s_q)
+                    s_q=p.s_q, sums=p.sums)
 
 
 def _sim(case, **kw):
@@ -26,20 +26,21 @@ def _sim(case, **kw):
     return Simulation(case.W, case.H, case.tau_inv, **case.sim_kwargs(), **kw)
 
 
-def _small_channel(W=96, H=40, model="mrt", cs=0.1, seed=4):
+def _small_channel(W=96, H=40, model="mrt", cs=0.1, seed=4, **kw):
     from lbm.sim import Simulation
     mask = G.channel_mask(W, H, cylinder=(W // 4, H // 2, 5))
     prof = G.parabolic_profile(H, 0.05)
     f0 = G.channel_initial_state(W, H, prof, noise=1e-3, seed=seed)
-    return Simulation(W, H, 1.8, cs=cs, model=model, bc="channel", mask=mask, inlet_u=prof, f0=f0)
+    return Simulation(W, H, 1.8, cs=cs, model=model, bc="channel", mask=mask, inlet_u=prof, f0=f0, **kw)
 
 
-def _small_periodic(W=80, H=48, cs=0.12, model="mrt"):
+def _small_periodic(W=80, H=48, cs=0.12, model="mrt", **kw):
     from lbm.sim import Simulation
     mask = G.periodic_mask(W, H)
     mask[20:26, 30:37] = 1
     ux, uy = G.random_phase_velocity(W, H, k0=4, u_rms=0.04, seed=5)
-    return Simulation(W, H, 1.7, cs=cs, model=model, bc="periodic", mask=mask, f0=G.feq(np.ones((H, W)), ux, uy))
+    return Simulation(W, H, 1.7, cs=cs, model=model, bc="periodic", mask=mask, f0=G.feq(np.ones((H, W)), ux, uy),
+                      **kw)
 
 
 def _check_steps_bitwise(gpu, sim, nsteps, macros=False):
@@ -258,3 +259,33 @@ def test_health_check_detects_injected_failures(gpu):
                 s.f.mul_(1.01)
         with pytest.raises(LbmHealthError, match=what):
             s.check_health(mass0, rel_tol=1e-3)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_aa_step_compensated_sums_bitwise(gpu, periodic):
+    """sums = 1 (Neumaier sums of rho - 1 and j): every step equals the FP32 mirror in the same mode bit for bit,
+    and differs from the plain-sum step (the mode is live)."""
+    sim = (_small_periodic if periodic else _small_channel)(sums="neumaier")
+    assert sim.params.sums == 1
+    _check_steps_bitwise(gpu, sim, 10, macros=True)
+    if periodic:  # a strong flow, where the plain sums round: the compensated steps differ and still match
+        from lbm.sim import Simulation
+        W, H = 64, 32
+        rng = np.random.default_rng(3)
+        ux, uy = G.random_phase_velocity(W, H, 4, 0.25, seed=3)
+        f0 = G.feq(1.0 + 0.08 * rng.standard_normal((H, W)), ux, uy)
+        kw = dict(cs=0.1, bc="periodic", mask=G.periodic_mask(W, H), f0=f0)
+        plain, comp = Simulation(W, H, 1.7, **kw), Simulation(W, H, 1.7, sums="neumaier", **kw)
+        _check_steps_bitwise(gpu, comp, 4)
+        plain.step(4)
+        assert not np.array_equal(plain.f.cpu().numpy(), comp.f.cpu().numpy())
+
+
+def test_compensated_sums_single_step_only(gpu):
+    from lbm import _lib
+    sim = _small_channel(sums="neumaier")
+    with pytest.raises(_lib.LbmError):
+        sim.step(2, k=2)
+    from lbm.ensemble import Ensemble
+    with pytest.raises(_lib.LbmError):
+        Ensemble([_small_periodic(sums="neumaier")])

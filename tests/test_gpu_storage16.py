@@ -12,29 +12,30 @@ pytestmark = pytest.mark.gpu
 FMTS = [("f16", O.F16), ("bf16", O.BF16)]
 
 
-def _sim(storage, periodic=False, W=160, H=72, cs=0.12, seed=6):
+def _sim(storage, periodic=False, W=160, H=72, cs=0.12, seed=6, sums="plain"):
     from lbm.sim import Simulation
     if periodic:
         mask = G.periodic_mask(W, H)
         mask[20:25, 60:66] = 1
         ux, uy = G.random_phase_velocity(W, H, 4, 0.04, seed=seed)
         return Simulation(W, H, 1.7, cs=cs, bc="periodic", mask=mask, f0=G.feq(np.ones((H, W)), ux, uy),
-                          storage=storage)
+                          storage=storage, sums=sums)
     mask = G.channel_mask(W, H, cylinder=(W // 4, H // 2, 6))
     prof = G.parabolic_profile(H, 0.05)
     return Simulation(W, H, 1.75, cs=cs, mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, seed),
-                      storage=storage)
+                      storage=storage, sums=sums)
 
 
 def _op(sim):
     p = sim.params
-    return O.params(p.W, p.H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc, clamp=p.clamp)
+    return O.params(p.W, p.H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc, clamp=p.clamp, sums=p.sums)
 
 
 @pytest.mark.parametrize("storage,fmt", FMTS)
 @pytest.mark.parametrize("periodic", [False, True])
-def test_h16_single_step_bitwise(gpu, storage, fmt, periodic):
-    sim = _sim(storage, periodic)
+@pytest.mark.parametrize("sums", ["plain", "neumaier"])
+def test_h16_single_step_bitwise(gpu, storage, fmt, periodic, sums):
+    sim = _sim(storage, periodic, sums=sums)
     op = _op(sim)
     prev, par = sim.state()
     for n in range(10):
