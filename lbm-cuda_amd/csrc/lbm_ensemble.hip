@@ -105,7 +105,7 @@ __global__ __launch_bounds__(pairk::NT) void k_ensemble_pair(const EnsDesc *__re
         const EnsDesc &e = d[ens_member(d, m0, m1, t, [chi](const EnsDesc &x) { return x.ptile0[chi]; })];
         const int lt = t - e.ptile0[chi];
         const int ty = lt / e.ptiles_x, tx = lt - ty * e.ptiles_x;
-        pairk::pair_tile<MODEL, PERIODIC, FMT, CH>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u, nullptr, nullptr,
+        pairk::pair_tile_any<MODEL, PERIODIC, FMT, CH>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u, nullptr, nullptr,
                                                    nullptr, tx, ty, L);
         __syncthreads(); // the next tile's phase A overwrites the LDS this tile's phase B read
     }

@@ -26,7 +26,7 @@ __global__ __launch_bounds__(NT) void k_pair(KPar P, const uint8_t *__restrict__
     __shared__ TileLds<FMT, CH> L;
     int bx, by;
     xcd_tile(bx, by);
-    pair_tile<MODEL, PERIODIC, FMT, CH>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L);
+    pair_tile_any<MODEL, PERIODIC, FMT, CH>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L);
 }
 
 template <int MODEL, bool PERIODIC, int FMT>

@@ -24,7 +24,11 @@ template <int FMT, int CH> struct TileLds {
     uint8_t sm[CH + 2][LS]; // 1 = solid or outside the lattice
 };
 
-template <int MODEL, bool PERIODIC, int FMT, int CH>
+// INTERIOR: the tile's region (core plus halo ring) lies at least one cell inside the walls, the inlet column
+// and the outlet cells x >= W - 2 (pair_interior), so every region cell is in the lattice, is updated by the
+// even step, is no inlet cell and feeds no outlet ghost, and no index wraps.  That tile-uniform fact removes
+// the per-cell boundary tests; the arithmetic is the same, so both forms give the same bits.
+template <int MODEL, bool PERIODIC, int FMT, int CH, bool INTERIOR>
 __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restrict__ mask, const void *__restrict__ fin,
                                           void *__restrict__ fout, const float *__restrict__ inlet_u,
                                           float *__restrict__ oux, float *__restrict__ ouy, float *__restrict__ orho,
@@ -35,6 +39,8 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     using T = typename C::T;
     constexpr unsigned B = S::BYTES;
     constexpr int RH = CH + 2;
+    constexpr bool OPEN = !PERIODIC && !INTERIOR; // inlet, outlet and walls may lie in the region
+    constexpr bool WRAP = PERIODIC && !INTERIOR;  // region indices may wrap around the lattice
     auto &sf = L.sf;
     auto &sm = L.sm;
 
@@ -59,10 +65,10 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         const int r = e / RW, c = e - r * RW;
         int gx = x0 - 1 + c, gy = y0 - 1 + r;
         bool in = e < RH * RW;
-        if constexpr (PERIODIC) {
+        if constexpr (WRAP) {
             gx = wrap(gx, W);
             gy = wrap(gy, H);
-        } else {
+        } else if constexpr (OPEN) {
             // 0 <= v < n as one unsigned comparison (n > 0)
             in = in && (unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H;
         }
@@ -72,9 +78,9 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
 #pragma unroll
         for (int q = 0; q < 9; ++q) ga[a][q] = S::ld(rin, idx * B, (unsigned)q * plane * B, q);
         ina[a] = in;
-        la[a] = !PERIODIC && gx == 0;
+        la[a] = OPEN && gx == 0;
         // rows and columns the even step updates (all of them when periodic): 1 <= gy <= H - 2, gx <= W - 2
-        ua[a] = PERIODIC || ((unsigned)(gy - 1) < (unsigned)(H - 2) && gx <= W - 2);
+        ua[a] = !OPEN || ((unsigned)(gy - 1) < (unsigned)(H - 2) && gx <= W - 2);
         ia[a] = (unsigned)gy; // the row, for the inlet speed
     }
 #pragma unroll
@@ -92,7 +98,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         for (int q = 0; q < 9; ++q) o[q] = C::st(ga[a][q], q, K);
 #pragma unroll
         for (int q = 0; q < 9; ++q) sf[opp(q)][r][c] = o[q];
-        if (!PERIODIC && x0 - 1 + c == W - 2 && c + 1 < RW) { // reversed copy into the outlet ghost
+        if (OPEN && x0 - 1 + c == W - 2 && c + 1 < RW) { // reversed copy into the outlet ghost
 #pragma unroll
             for (int q = 0; q < 9; ++q) sf[opp(q)][r][c + 1] = o[q];
         }
@@ -103,9 +109,9 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     for (int e = t; e < CH * CW; e += NT) {
         const int r = 1 + e / CW, c = 1 + (e - (e / CW) * CW);
         const int gx = x0 + c - 1, gy = y0 + r - 1;
-        if constexpr (PERIODIC) {
+        if constexpr (WRAP) {
             if (gx >= W || gy >= H) continue;
-        } else {
+        } else if constexpr (OPEN) {
             // updated cells only: 1 <= gy <= H - 2 and gx <= W - 2 (walls, the outlet ghost and cells past the
             // lattice are not updated; gx, gy >= 0 here)
             if ((unsigned)(gy - 1) >= (unsigned)(H - 2) || gx >= W - 1) continue;
@@ -117,7 +123,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         const uint8_t *smb = &sm[0][0] + lb;
         auto at = [&](int k, int dy, int dx) -> T { return sfb[k * RH * LS + (1 + dy) * LS + (1 + dx)]; };
         bool inlet = false;
-        if constexpr (!PERIODIC) {
+        if constexpr (OPEN) {
             inlet = gx == 0;
             if (!inlet && smb[LS + 1]) continue;
         } else {
@@ -126,7 +132,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         bool solid[9];
 #pragma unroll
         for (int q = 1; q < 9; ++q)
-            solid[q] = smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(!PERIODIC && inlet && cx(q) < 0);
+            solid[q] = smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(OPEN && inlet && cx(q) < 0);
         float g[9], mac[3];
         g[0] = C::ld(at(0, 0, 0), 0);
 #pragma unroll
@@ -144,7 +150,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         unsigned nbB[9];
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
-            if constexpr (PERIODIC)
+            if constexpr (WRAP)
                 nbB[q] = (unsigned)(wrap(gy + cy(q), H) * W + wrap(gx + cx(q), W)) * B;
             else
                 nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
@@ -157,11 +163,11 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             // planes q and opp(q) differ by 2: min(q, opp(q)) * plane is the scalar offset, and the vector
             // offset is this cell's or the target's offset, two planes up for the larger index.
             const int o = opp(q);
-            const bool back = solid[q] || (!PERIODIC && inlet && cx(q) < 0);
+            const bool back = solid[q] || (OPEN && inlet && cx(q) < 0);
             const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nbB[q] + 2u * pB);
             S::st(rout, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
         }
-        if (!PERIODIC && gx == W - 2) {
+        if (OPEN && gx == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
             S::st(rout, idxB, 3u * pB, 3, g[3]);
             if (!solid[2]) S::st(rout, nbB[2], 6u * pB, 6, g[6]);
@@ -173,6 +179,29 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             ouy[idx] = mac[2];
         }
     }
+}
+
+// True when tile (bx, by) of a 128 x CH tiling is interior in the sense of pair_tile's INTERIOR: its region
+// [x0 - 1, x0 + 128] x [y0 - 1, y0 + CH] lies in 1 <= x <= W - 3, 1 <= y <= H - 2.  (For a periodic lattice
+// this is stricter than needed: it only has to avoid wrapping.)
+template <int CH> __device__ __forceinline__ bool pair_interior(int W, int H, int bx, int by)
+{
+    const int x0 = bx * CW, y0 = by * CH;
+    return x0 >= 2 && x0 + CW + 3 <= W && y0 >= 2 && y0 + CH + 2 <= H;
+}
+
+// One tile, with the interior form where it applies (a workgroup-uniform branch).
+template <int MODEL, bool PERIODIC, int FMT, int CH>
+__device__ __forceinline__ void pair_tile_any(const KPar &P, const uint8_t *__restrict__ mask,
+                                              const void *__restrict__ fin, void *__restrict__ fout,
+                                              const float *__restrict__ inlet_u, float *__restrict__ oux,
+                                              float *__restrict__ ouy, float *__restrict__ orho, int bx, int by,
+                                              TileLds<FMT, CH> &L)
+{
+    if (pair_interior<CH>(P.W, P.H, bx, by))
+        pair_tile<MODEL, PERIODIC, FMT, CH, true>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L);
+    else
+        pair_tile<MODEL, PERIODIC, FMT, CH, false>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L);
 }
 
 } // namespace pairk

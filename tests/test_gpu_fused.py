@@ -123,6 +123,42 @@ def test_pair_kernel_equals_two_single_steps(gpu, chunk, periodic, model):
             _assert_equal(single, fusedsim, sel, f"{W}x{H} chunk {chunk} launch {launch}")
 
 
+@pytest.mark.parametrize("storage,chunk", [("f32", 8), ("f16", 12), ("bf16", 24), ("f16", 16)])
+def test_pair_kernel_interior_tiles(gpu, storage, chunk):
+    """Tiles clear of the walls, the inlet and the outlet take the two-step kernel's interior form: obstacles
+    inside such tiles, across their edges and touching their halo rows, bit for bit against single steps."""
+    from lbm.sim import Simulation
+    W, H = 700, 150
+    mask = G.channel_mask(W, H, cylinder=(300, 70, 20))  # inside the x = 256..383 tiles
+    mask[40:60, 380:395] = 1                               # across the x = 384 tile edge
+    mask[chunk - 1:chunk + 1, 500:520] = 1                 # on the first interior tile row's halo
+    prof = G.parabolic_profile(H, 0.05)
+    kw = dict(cs=0.12, bc="channel", mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, 11),
+              storage=storage)
+    a, b = Simulation(W, H, 1.75, **kw), Simulation(W, H, 1.75, **kw)
+    sel = _sel(a)
+    for launch in range(4):
+        a.step(2)
+        b.step(2, k=2, cfg=(0, chunk))
+        _assert_equal(a, b, sel, f"{storage} chunk {chunk} launch {launch}")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_pair_kernel_cylinder_large(gpu, storage):
+    """4096 x 1024 cylinder (the benchmark lattice), 400 steps with the 12-row two-step kernel: bitwise equal
+    to single steps at every 20th launch."""
+    from lbm.sim import Simulation
+    case = configs.cylinder_large()
+    a = Simulation(case.W, case.H, case.tau_inv, storage=storage, **case.sim_kwargs())
+    b = Simulation(case.W, case.H, case.tau_inv, storage=storage, **case.sim_kwargs())
+    sel = _sel(a)
+    for launch in range(200):
+        a.step(2)
+        b.step(2, k=2, cfg=(0, 12))
+        if launch % 20 == 19:
+            _assert_equal(a, b, sel, f"launch {launch}")
+
+
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 def test_pair_kernel_16bit(gpu, storage):
     from lbm.sim import Simulation
