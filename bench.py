@@ -39,8 +39,8 @@ TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-
 # The same for one launch (two steps) of the two-step tile kernel, per (storage, tile rows), and its VALU
 # instructions per cell and launch (rocprofv3 --pmc SQ_INSTS_VALU x 64 / cells); the summary with the command
 # lines is profiles/k_pair_cylinder_4096x1024.md.  Configurations not listed there report null.
-PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.46e6, ("bf16", 12): 156.44e6, ("f32", 12): 309.45e6}
-PAIR_VALU_PER_CELL = {("f16", 12): 440.2, ("bf16", 12): 432.2, ("f32", 12): 451.6}
+PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.86e6, ("bf16", 12): 156.82e6, ("f32", 12): 310.40e6}
+PAIR_VALU_PER_CELL = {("f16", 12): 417.3, ("bf16", 12): 416.7, ("f32", 12): 428.0}
 # VALU issue capacity of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each, 2.4 GHz
 VALU_WAVE_INSTS_PER_US = 256 * 4 / 4 * 2400.0
 
