@@ -7,8 +7,10 @@
 namespace lbm {
 
 // One AA step of cell (x, y); cells outside the lattice return at once.  EXT: the stress moments relax at
-// w_ext when w_ext >= 0 (a learned closure's rate), else by the model's own rule.
-template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS, bool EXT = false>
+// w_ext when w_ext >= 0 (a learned closure's rate), else by the model's own rule.  INTERIOR: the caller knows
+// that 1 <= x <= W - 3 and 1 <= y <= H - 2 (no wall row, inlet, outlet ghost or ghost source, and no neighbour
+// index wraps), which removes the per-cell boundary tests; the arithmetic is the same.
+template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS, bool EXT = false, bool INTERIOR = false>
 __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict__ mask, void *__restrict__ f,
                                         const float *__restrict__ inlet_u, float *__restrict__ oux,
                                         float *__restrict__ ouy, float *__restrict__ orho, int x, int y,
@@ -16,7 +18,9 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
 {
     using S = Store<FMT>;
     constexpr unsigned B = S::BYTES;
-    if (x >= P.W || y >= P.H) return;
+    constexpr bool OPEN = !PERIODIC && !INTERIOR; // walls, inlet and outlet may be in reach
+    constexpr bool WRAP = PERIODIC && !INTERIOR;  // neighbour indices may wrap around the lattice
+    if (!INTERIOR && (x >= P.W || y >= P.H)) return;
     const int W = P.W;
     const unsigned plane = (unsigned)P.plane;
     const unsigned idx = (unsigned)(y * W + x);
@@ -24,7 +28,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
     const rsrc_t rm = make_rsrc(mask, plane);
     auto solid_at = [&](unsigned cell) { return __builtin_amdgcn_raw_buffer_load_b8(rm, cell, 0, 0) != 0; };
     bool inlet = false;
-    if constexpr (!PERIODIC) {
+    if constexpr (OPEN) {
         // wall rows and the outlet ghost are never updated (lbm_core4.cu:20-24)
         if (y == 0 || y == P.H - 1 || x == W - 1) return;
         inlet = x == 0;
@@ -43,7 +47,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         solid[0] = false;
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
-            if constexpr (PERIODIC) {
+            if constexpr (WRAP) {
                 int nx = x + cx(q), ny = y + cy(q);
                 nx = nx < 0 ? nx + W : (nx >= W ? nx - W : nx);
                 ny = ny < 0 ? ny + P.H : (ny >= P.H ? ny - P.H : ny);
@@ -55,7 +59,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
             }
             // all eight mask loads are issued unconditionally, so that they share one wait
             const bool s_nb = solid_at(nb[q]);
-            solid[q] = s_nb && !(!PERIODIC && inlet && cx(q) < 0);
+            solid[q] = s_nb && !(OPEN && inlet && cx(q) < 0);
         }
         // Bounce-back reads this cell's slot q, otherwise the source's slot opp(q).  The choice is a per-lane
         // select of the vector offset: a per-lane scalar offset would force a loop over its distinct values
@@ -69,7 +73,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         const unsigned idx2 = idxB + 2u * pB;
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
-            if constexpr (PERIODIC)
+            if constexpr (WRAP)
                 nbB[q] = nb[q] * B;
             else
                 nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
@@ -89,7 +93,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
     if constexpr (PARITY == 0) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) S::st(rf, idx * B, (unsigned)opp(q) * plane * B, q, g[q]);
-        if (!PERIODIC && x == W - 2) { // reversed copy into the outlet ghost (lbm_core4.cu:114-116)
+        if (OPEN && x == W - 2) { // reversed copy into the outlet ghost (lbm_core4.cu:114-116)
 #pragma unroll
             for (int q = 0; q < 9; ++q) S::st(rf, (idx + 1) * B, (unsigned)opp(q) * plane * B, q, g[q]);
         }
@@ -104,13 +108,13 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
-            const bool back = solid[q] || (!PERIODIC && inlet && cx(q) < 0);
+            const bool back = solid[q] || (OPEN && inlet && cx(q) < 0);
             // q < o: slot q at the target = nbB[q] + q pB, slot o here = idx2 + q pB
             // q > o: slot q at the target = nb2[q] + o pB, slot o here = idxB + o pB
             const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nb2[q]);
             S::st(rf, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
         }
-        if (!PERIODIC && x == W - 2) {
+        if (OPEN && x == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
             S::st(rf, idx * B, 3u * plane * B, 3, g[3]);
             if (!solid[2]) S::st(rf, nb[2] * B, 6u * plane * B, 6, g[6]);
@@ -122,6 +126,21 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         oux[idx] = mac[1];
         ouy[idx] = mac[2];
     }
+}
+
+// One AA step of the thread's cell of the TILE_X x TILE_Y tile (bx, by), in the interior form when the whole
+// tile satisfies aa_cell's INTERIOR condition (a workgroup-uniform branch).
+template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS>
+__device__ __forceinline__ void aa_tile_cell(const KPar &P, const uint8_t *__restrict__ mask, void *__restrict__ f,
+                                             const float *__restrict__ inlet_u, float *__restrict__ oux,
+                                             float *__restrict__ ouy, float *__restrict__ orho, int bx, int by)
+{
+    const int x0 = bx * TILE_X, y0 = by * TILE_Y;
+    const int x = x0 + (int)threadIdx.x, y = y0 + (int)threadIdx.y;
+    if (x0 >= 1 && x0 + TILE_X + 2 <= P.W && y0 >= 1 && y0 + TILE_Y + 1 <= P.H)
+        aa_cell<MODEL, FMT, PARITY, PERIODIC, MACROS, false, true>(P, mask, f, inlet_u, oux, ouy, orho, x, y);
+    else
+        aa_cell<MODEL, FMT, PARITY, PERIODIC, MACROS>(P, mask, f, inlet_u, oux, ouy, orho, x, y);
 }
 
 } // namespace lbm

@@ -84,8 +84,8 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_ensemble(const EnsDesc *__re
         const EnsDesc &e = d[ens_member(d, m0, m1, t, [](const EnsDesc &x) { return x.tile0; })];
         const int lt = t - e.tile0;
         const int ty = lt / e.tiles_x, tx = lt - ty * e.tiles_x;
-        aa_cell<MODEL, FMT, PARITY, PERIODIC, false>(e.P, e.mask, e.f[side], e.inlet_u, nullptr, nullptr, nullptr,
-                                                     tx * TILE_X + threadIdx.x, ty * TILE_Y + threadIdx.y);
+        aa_tile_cell<MODEL, FMT, PARITY, PERIODIC, false>(e.P, e.mask, e.f[side], e.inlet_u, nullptr, nullptr, nullptr,
+                                                          tx, ty);
     }
 }
 

@@ -94,8 +94,7 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_aa(KPar P, const uint8_t *__
 {
     int bx, by;
     xcd_tile(bx, by);
-    aa_cell<MODEL, FMT, PARITY, PERIODIC, MACROS>(P, mask, f, inlet_u, oux, ouy, orho, bx * TILE_X + threadIdx.x,
-                                                  by * TILE_Y + threadIdx.y);
+    aa_tile_cell<MODEL, FMT, PARITY, PERIODIC, MACROS>(P, mask, f, inlet_u, oux, ouy, orho, bx, by);
 }
 
 template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS>

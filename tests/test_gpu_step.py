@@ -79,6 +79,22 @@ def test_aa_step_bitwise_periodic(gpu, model):
     _check_steps_bitwise(gpu, _small_periodic(model=model, cs=0.12 if model == "mrt" else 0.0), 12)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_aa_step_bitwise_interior_tiles(gpu, periodic):
+    """Lattices wide enough for interior 128 x 2 tiles (x = 128..255 here, taking the single-step kernel's
+    interior form), with obstacles inside and across them: every step equals the FP32 mirror."""
+    W, H = 400, 40
+    if periodic:
+        sim = _small_periodic(W=W, H=H)
+    else:
+        sim = _small_channel(W=W, H=H)
+    sim.mask[10:16, 150:160] = 1  # inside an interior tile
+    sim.mask[22:27, 250:262] = 1  # across the x = 256 tile edge
+    sim.mask_host[10:16, 150:160] = 1
+    sim.mask_host[22:27, 250:262] = 1
+    _check_steps_bitwise(gpu, sim, 10)
+
+
 def test_aa_step_odd_sizes(gpu):
     """Lattices that do not fill the 64 x 4 tiles (ragged right and bottom edges)."""
     from lbm.sim import Simulation

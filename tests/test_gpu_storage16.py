@@ -12,15 +12,18 @@ pytestmark = pytest.mark.gpu
 FMTS = [("f16", O.F16), ("bf16", O.BF16)]
 
 
-def _sim(storage, periodic=False, W=160, H=72, cs=0.12, seed=6, sums="plain"):
+def _sim(storage, periodic=False, W=300, H=72, cs=0.12, seed=6, sums="plain"):
+    """W = 300: the 128 x 2 tiles at x = 128..255 are interior (single-step interior form), with an obstacle."""
     from lbm.sim import Simulation
     if periodic:
         mask = G.periodic_mask(W, H)
         mask[20:25, 60:66] = 1
+        mask[40:47, 180:186] = 1
         ux, uy = G.random_phase_velocity(W, H, 4, 0.04, seed=seed)
         return Simulation(W, H, 1.7, cs=cs, bc="periodic", mask=mask, f0=G.feq(np.ones((H, W)), ux, uy),
                           storage=storage, sums=sums)
     mask = G.channel_mask(W, H, cylinder=(W // 4, H // 2, 6))
+    mask[20:26, 200:206] = 1
     prof = G.parabolic_profile(H, 0.05)
     return Simulation(W, H, 1.75, cs=cs, mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, seed),
                       storage=storage, sums=sums)
