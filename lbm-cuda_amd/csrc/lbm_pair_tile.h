@@ -5,6 +5,8 @@
 // writer).  The caller owns the LDS arrays and calls it from every thread of a 512-thread workgroup.
 #pragma once
 
+#include <type_traits>
+
 #include "lbm_common.h"
 
 namespace lbm {
@@ -14,6 +16,7 @@ constexpr int CW = 128;    // core columns per tile
 constexpr int RW = CW + 2; // region columns (one halo column on each side)
 constexpr int LS = 132;    // LDS row stride (elements)
 constexpr int NT = 512;    // threads per workgroup
+static_assert(NT / 64 == 8, "the per-wave votes are read as one 8-byte word");
 
 // v mod n for v >= -n (one period at most below the lattice; region columns may lie several periods above
 // it when the lattice is narrower than the region)
@@ -21,7 +24,8 @@ __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >=
 
 template <int FMT, int CH> struct TileLds {
     typename Conv<FMT>::T sf[9][CH + 2][LS];
-    uint8_t sm[CH + 2][LS]; // 1 = solid or outside the lattice
+    uint8_t sm[CH + 2][LS];           // 1 = solid or outside the lattice
+    alignas(8) uint8_t wsolid[NT / 64]; // per wave: 1 if any of its region cells is solid (interior form)
 };
 
 // INTERIOR: the tile's region (core plus halo ring) lies at least one cell inside the walls, the inlet column
@@ -51,6 +55,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     const rsrc_t rm = make_rsrc(mask, plane);
     const int t = threadIdx.x;
     const CollideK K = opaque_consts(); // kept in VGPRs across the unrolled cells
+    bool any_solid = false;             // interior form: a region cell of this thread is solid
 
     // ---- phase A: even step of every region cell, reversed result into LDS.  All of a thread's HBM loads
     //      (NA cells x (mask byte + 9 populations)) are issued before anything uses them, so their latencies
@@ -89,6 +94,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         const int r = e / RW, c = e - r * RW;
         const bool solid = !ina[a] || ma[a] != 0;
         if (e < RH * RW) sm[r][c] = solid ? 1 : 0;
+        if constexpr (INTERIOR) any_solid = any_solid || (ina[a] && ma[a] != 0);
         const bool inlet = la[a];
         if (!(ina[a] && ua[a] && (inlet || !solid))) continue;
         float mac[3];
@@ -103,81 +109,100 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             for (int q = 0; q < 9; ++q) sf[opp(q)][r][c + 1] = o[q];
         }
     }
+    if constexpr (INTERIOR) {
+        // every lane of a wave writes the wave's vote (the same value) to the wave's byte
+        const bool wave_solid = __ballot(any_solid) != 0;
+        L.wsolid[t / 64] = wave_solid ? 1 : 0;
+    }
     __syncthreads();
 
     // ---- phase B: odd step of the core cells, scattered to the output ------------------------------------
-    for (int e = t; e < CH * CW; e += NT) {
-        const int r = 1 + e / CW, c = 1 + (e - (e / CW) * CW);
-        const int gx = x0 + c - 1, gy = y0 + r - 1;
-        if constexpr (WRAP) {
-            if (gx >= W || gy >= H) continue;
-        } else if constexpr (OPEN) {
-            // updated cells only: 1 <= gy <= H - 2 and gx <= W - 2 (walls, the outlet ghost and cells past the
-            // lattice are not updated; gx, gy >= 0 here)
-            if ((unsigned)(gy - 1) >= (unsigned)(H - 2) || gx >= W - 1) continue;
+    // FLUID (interior tiles whose region holds no solid cell): no bounce-back anywhere, so the mask tests and
+    // the gather / scatter selects drop out; the values moved and the arithmetic are the same.
+    auto phase_b = [&](auto fluid_tag) {
+        constexpr bool FLUID = decltype(fluid_tag)::value;
+        for (int e = t; e < CH * CW; e += NT) {
+            const int r = 1 + e / CW, c = 1 + (e - (e / CW) * CW);
+            const int gx = x0 + c - 1, gy = y0 + r - 1;
+            if constexpr (WRAP) {
+                if (gx >= W || gy >= H) continue;
+            } else if constexpr (OPEN) {
+                // updated cells only: 1 <= gy <= H - 2 and gx <= W - 2 (walls, the outlet ghost and cells past the
+                // lattice are not updated; gx, gy >= 0 here)
+                if ((unsigned)(gy - 1) >= (unsigned)(H - 2) || gx >= W - 1) continue;
+            }
+            // LDS reads relative to the corner (r - 1, c - 1) of the cell's 3 x 3 neighbourhood: every access is
+            // that base plus a non-negative constant, which the DS instructions take as an immediate offset
+            const int lb = (r - 1) * LS + (c - 1);
+            const T *sfb = &sf[0][0][0] + lb;
+            const uint8_t *smb = &sm[0][0] + lb;
+            auto at = [&](int k, int dy, int dx) -> T { return sfb[k * RH * LS + (1 + dy) * LS + (1 + dx)]; };
+            bool inlet = false;
+            if constexpr (OPEN) {
+                inlet = gx == 0;
+                if (!inlet && smb[LS + 1]) continue;
+            } else if constexpr (!FLUID) {
+                if (smb[LS + 1]) continue;
+            }
+            bool solid[9];
+    #pragma unroll
+            for (int q = 1; q < 9; ++q)
+                solid[q] = !FLUID && smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(OPEN && inlet && cx(q) < 0);
+            float g[9], mac[3];
+            g[0] = C::ld(at(0, 0, 0), 0);
+    #pragma unroll
+            for (int q = 1; q < 9; ++q) {
+                const int o = opp(q);
+                // bounce-back: this cell's slot q; otherwise the source's slot opp(q) (both read, then selected).
+                // At the inlet the q with c_qx > 0 come from x = -1 and are rebuilt by Zou-He, which never reads them.
+                const T own = at(q, 0, 0), src = at(o, -cy(q), -cx(q));
+                g[q] = C::ld(solid[o] ? own : src, q);
+            }
+            collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac, -1.0f, K);
+            const unsigned idx = (unsigned)(gy * W + gx);
+            const unsigned idxB = idx * B, pB = plane * B;
+            const unsigned idx2 = idxB + 2u * pB;
+            unsigned nbB[9];
+    #pragma unroll
+            for (int q = 1; q < 9; ++q) {
+                if constexpr (WRAP)
+                    nbB[q] = (unsigned)(wrap(gy + cy(q), H) * W + wrap(gx + cx(q), W)) * B;
+                else
+                    nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
+            }
+            S::st(rout, idxB, 0, 0, g[0]);
+    #pragma unroll
+            for (int q = 1; q < 9; ++q) {
+                // bounce-back writes this cell's slot opp(q); populations leaving through the open inlet are parked
+                // in this cell's slot opp(q) (the canonical inlet slots 1, 5, 8, rebuilt by the next Zou-He).  The
+                // planes q and opp(q) differ by 2: min(q, opp(q)) * plane is the scalar offset, and the vector
+                // offset is this cell's or the target's offset, two planes up for the larger index.
+                const int o = opp(q);
+                const bool back = solid[q] || (OPEN && inlet && cx(q) < 0);
+                const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nbB[q] + 2u * pB);
+                S::st(rout, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
+            }
+            if (OPEN && gx == W - 2) {
+                // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
+                S::st(rout, idxB, 3u * pB, 3, g[3]);
+                if (!solid[2]) S::st(rout, nbB[2], 6u * pB, 6, g[6]);
+                if (!solid[4]) S::st(rout, nbB[4], 7u * pB, 7, g[7]);
+            }
+            if (orho) {
+                orho[idx] = mac[0];
+                oux[idx] = mac[1];
+                ouy[idx] = mac[2];
+            }
         }
-        // LDS reads relative to the corner (r - 1, c - 1) of the cell's 3 x 3 neighbourhood: every access is
-        // that base plus a non-negative constant, which the DS instructions take as an immediate offset
-        const int lb = (r - 1) * LS + (c - 1);
-        const T *sfb = &sf[0][0][0] + lb;
-        const uint8_t *smb = &sm[0][0] + lb;
-        auto at = [&](int k, int dy, int dx) -> T { return sfb[k * RH * LS + (1 + dy) * LS + (1 + dx)]; };
-        bool inlet = false;
-        if constexpr (OPEN) {
-            inlet = gx == 0;
-            if (!inlet && smb[LS + 1]) continue;
-        } else {
-            if (smb[LS + 1]) continue;
-        }
-        bool solid[9];
-#pragma unroll
-        for (int q = 1; q < 9; ++q)
-            solid[q] = smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(OPEN && inlet && cx(q) < 0);
-        float g[9], mac[3];
-        g[0] = C::ld(at(0, 0, 0), 0);
-#pragma unroll
-        for (int q = 1; q < 9; ++q) {
-            const int o = opp(q);
-            // bounce-back: this cell's slot q; otherwise the source's slot opp(q) (both read, then selected).
-            // At the inlet the q with c_qx > 0 come from x = -1 and are rebuilt by Zou-He, which never reads them.
-            const T own = at(q, 0, 0), src = at(o, -cy(q), -cx(q));
-            g[q] = C::ld(solid[o] ? own : src, q);
-        }
-        collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac, -1.0f, K);
-        const unsigned idx = (unsigned)(gy * W + gx);
-        const unsigned idxB = idx * B, pB = plane * B;
-        const unsigned idx2 = idxB + 2u * pB;
-        unsigned nbB[9];
-#pragma unroll
-        for (int q = 1; q < 9; ++q) {
-            if constexpr (WRAP)
-                nbB[q] = (unsigned)(wrap(gy + cy(q), H) * W + wrap(gx + cx(q), W)) * B;
-            else
-                nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
-        }
-        S::st(rout, idxB, 0, 0, g[0]);
-#pragma unroll
-        for (int q = 1; q < 9; ++q) {
-            // bounce-back writes this cell's slot opp(q); populations leaving through the open inlet are parked
-            // in this cell's slot opp(q) (the canonical inlet slots 1, 5, 8, rebuilt by the next Zou-He).  The
-            // planes q and opp(q) differ by 2: min(q, opp(q)) * plane is the scalar offset, and the vector
-            // offset is this cell's or the target's offset, two planes up for the larger index.
-            const int o = opp(q);
-            const bool back = solid[q] || (OPEN && inlet && cx(q) < 0);
-            const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nbB[q] + 2u * pB);
-            S::st(rout, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
-        }
-        if (OPEN && gx == W - 2) {
-            // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-            S::st(rout, idxB, 3u * pB, 3, g[3]);
-            if (!solid[2]) S::st(rout, nbB[2], 6u * pB, 6, g[6]);
-            if (!solid[4]) S::st(rout, nbB[4], 7u * pB, 7, g[7]);
-        }
-        if (orho) {
-            orho[idx] = mac[0];
-            oux[idx] = mac[1];
-            ouy[idx] = mac[2];
-        }
+    };
+    if constexpr (INTERIOR) {
+        const uint64_t votes = *reinterpret_cast<const uint64_t *>(L.wsolid);
+        if (__builtin_amdgcn_readfirstlane((unsigned)(votes | (votes >> 32))) == 0)
+            phase_b(std::true_type{});
+        else
+            phase_b(std::false_type{});
+    } else {
+        phase_b(std::false_type{});
     }
 }
 
