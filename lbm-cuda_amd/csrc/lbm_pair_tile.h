@@ -57,15 +57,15 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     const CollideK K = opaque_consts(); // kept in VGPRs across the unrolled cells
     bool any_solid = false;             // interior form: a region cell of this thread is solid
 
-    // ---- phase A: even step of every region cell, reversed result into LDS.  All of a thread's HBM loads
-    //      (NA cells x (mask byte + 9 populations)) are issued before anything uses them, so their latencies
-    //      overlap; a mask test between them would make each cell's loads wait for the previous ones.
+    // ---- phase A: even step of every region cell, reversed result into LDS.  A thread's HBM loads (mask byte
+    //      + 9 populations per cell) are issued PD cells ahead of their use, and unconditionally, so their
+    //      latencies overlap and no load waits behind a mask test; PD = 4 bounds the registers they hold.
     constexpr int NA = (RH * RW + NT - 1) / NT;
+    constexpr int PD = NA < 4 ? NA : 4;
     float ga[NA][9];
     unsigned ma[NA], ia[NA];
     bool ina[NA], la[NA], ua[NA];
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
+    auto load_cell = [&](int a) {
         const int e = t + a * NT;
         const int r = e / RW, c = e - r * RW;
         int gx = x0 - 1 + c, gy = y0 - 1 + r;
@@ -87,16 +87,15 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         // rows and columns the even step updates (all of them when periodic): 1 <= gy <= H - 2, gx <= W - 2
         ua[a] = !OPEN || ((unsigned)(gy - 1) < (unsigned)(H - 2) && gx <= W - 2);
         ia[a] = (unsigned)gy; // the row, for the inlet speed
-    }
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
+    };
+    auto collide_cell = [&](int a) {
         const int e = t + a * NT;
         const int r = e / RW, c = e - r * RW;
         const bool solid = !ina[a] || ma[a] != 0;
         if (e < RH * RW) sm[r][c] = solid ? 1 : 0;
         if constexpr (INTERIOR) any_solid = any_solid || (ina[a] && ma[a] != 0);
         const bool inlet = la[a];
-        if (!(ina[a] && ua[a] && (inlet || !solid))) continue;
+        if (!(ina[a] && ua[a] && (inlet || !solid))) return;
         float mac[3];
         collide_dev<MODEL>(P, ga[a], inlet, inlet ? inlet_u[ia[a]] : 0.0f, mac, -1.0f, K);
         T o[9];
@@ -108,6 +107,13 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
 #pragma unroll
             for (int q = 0; q < 9; ++q) sf[opp(q)][r][c + 1] = o[q];
         }
+    };
+#pragma unroll
+    for (int a = 0; a < PD; ++a) load_cell(a);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        collide_cell(a);
+        if (a + PD < NA) load_cell(a + PD);
     }
     if constexpr (INTERIOR) {
         // every lane of a wave writes the wave's vote (the same value) to the wave's byte
