@@ -23,8 +23,10 @@ __host__ __device__ constexpr int opp(int q) { return q == 0 ? 0 : (q <= 4 ? ((q
 static_assert(opp(1) == 3 && opp(2) == 4 && opp(3) == 1 && opp(4) == 2, "opp");
 static_assert(opp(5) == 7 && opp(6) == 8 && opp(7) == 5 && opp(8) == 6, "opp");
 
-// MODEL_MRT_SUMS: MRT + LES with compensated (Neumaier) sums of rho - 1 and j (lbm_params.sums = 1)
-enum { MODEL_MRT = 0, MODEL_BGK = 1, MODEL_MRT_SUMS = 2 };
+// MODEL_MRT_SUMS: MRT + LES with compensated (Neumaier) sums of rho - 1 and j (lbm_params.sums = 1).
+// MODEL_MRT_CLAMP0 / _CLAMP1: MODEL_MRT with the velocity/density clamps known off / on at compile time (the
+// same arithmetic as MODEL_MRT with that lbm_params.clamp, without the run-time select).
+enum { MODEL_MRT = 0, MODEL_BGK = 1, MODEL_MRT_SUMS = 2, MODEL_MRT_CLAMP0 = 3, MODEL_MRT_CLAMP1 = 4 };
 
 // Kernel-side parameters, derived on the host in FP32 (tau0 = 1/tau_inv etc., as lbm_core4.cu:175-176).
 struct KPar {
@@ -155,7 +157,7 @@ __device__ __forceinline__ void moments_neu(const float h[9], float &drho, float
 // mac = (rho, ux, uy) as written by the reference (lbm_core4.cu:59-61,145-147).
 // EXT: the stress moments relax at w_ext when w_ext >= 0 (a learned closure's rate), else as usual.
 // SUMS: rho - 1 and j as compensated sums (moments_neu) instead of the pairwise FP32 sums.
-template <bool EXT = false, bool SUMS = false>
+template <bool EXT = false, bool SUMS = false, int CLAMP = -1>
 __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
                                             float w_ext = -1.0f, const CollideK &K = CollideK())
 {
@@ -197,7 +199,7 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
         ux = jx * inv_rho;
         uy = jy * inv_rho;
         drc = drho;
-        if (P.clamp) {
+        if (CLAMP > 0 || (CLAMP < 0 && P.clamp)) { // CLAMP: 1 on, 0 off, -1 lbm_params.clamp
             ux = clampf(ux, -K.ulim, K.ulim);
             uy = clampf(uy, -K.ulim, K.ulim);
             drc = clampf(drho, -1.0f, 0.5f); // rho in [0, 1.5]
@@ -315,7 +317,9 @@ __device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inle
     if constexpr (MODEL == MODEL_BGK)
         collide_bgk(P, h, inlet, uin, mac);
     else
-        collide_mrt<EXT, MODEL == MODEL_MRT_SUMS>(P, h, inlet, uin, mac, w_ext, K);
+        collide_mrt<EXT, MODEL == MODEL_MRT_SUMS,
+                    MODEL == MODEL_MRT_CLAMP1 ? 1 : (MODEL == MODEL_MRT_CLAMP0 ? 0 : -1)>(P, h, inlet, uin, mac,
+                                                                                              w_ext, K);
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -339,7 +343,12 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *p, unsigned nbytes)
 }
 
 // Register <-> storage conversions of one population (the storage word holds f for FMT_F32, h otherwise).
+// The 16-bit formats also convert two populations at once (st2: one packed conversion instruction, the
+// same round-to-nearest-even as the single form; the word's low half is population q0, the high half q1).
 template <int FMT> struct Conv;
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
 template <> struct Conv<FMT_F32> {
     using T = float;
@@ -363,6 +372,11 @@ template <> struct Conv<FMT_F16> {
     {
         return __builtin_bit_cast(uint16_t, (_Float16)clampf(h, -wq(q), K.hi[wclass(q)]));
     }
+    __device__ static unsigned st2(float h0, int q0, float h1, int q1, const CollideK &K)
+    {
+        const f32x2_t v = {clampf(h0, -wq(q0), K.hi[wclass(q0)]), clampf(h1, -wq(q1), K.hi[wclass(q1)])};
+        return __builtin_bit_cast(unsigned, __builtin_convertvector(v, f16x2_t));
+    }
     __device__ static float pop(T v, int q) { return wq(q) + ld(v, q); }
     __device__ static T from_pop(float f, int q) { return __builtin_bit_cast(uint16_t, (_Float16)(f - wq(q))); }
 };
@@ -378,6 +392,11 @@ template <> struct Conv<FMT_BF16> {
     __device__ static T st(float h, int q, const CollideK &K)
     {
         return __builtin_bit_cast(uint16_t, (__bf16)clampf(h, -wq(q), K.hi[wclass(q)]));
+    }
+    __device__ static unsigned st2(float h0, int q0, float h1, int q1, const CollideK &K)
+    {
+        const f32x2_t v = {clampf(h0, -wq(q0), K.hi[wclass(q0)]), clampf(h1, -wq(q1), K.hi[wclass(q1)])};
+        return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_t));
     }
     __device__ static float pop(T v, int q) { return wq(q) + ld(v, q); }
     __device__ static T from_pop(float f, int q) { return __builtin_bit_cast(uint16_t, (__bf16)(f - wq(q))); }

@@ -99,8 +99,19 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         float mac[3];
         collide_dev<MODEL>(P, ga[a], inlet, inlet ? inlet_u[ia[a]] : 0.0f, mac, -1.0f, K);
         T o[9];
+        if constexpr (B == 2) {
+            // pairs (0,1) (2,3) (4,5) (6,7) in one conversion each; the high halves go to LDS as they are
 #pragma unroll
-        for (int q = 0; q < 9; ++q) o[q] = C::st(ga[a][q], q, K);
+            for (int q = 0; q < 8; q += 2) {
+                const unsigned pk = C::st2(ga[a][q], q, ga[a][q + 1], q + 1, K);
+                o[q] = (T)pk;
+                o[q + 1] = (T)(pk >> 16);
+            }
+            o[8] = C::st(ga[a][8], 8, K);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) o[q] = C::st(ga[a][q], q, K);
+        }
 #pragma unroll
         for (int q = 0; q < 9; ++q) sf[opp(q)][r][c] = o[q];
         if (OPEN && x0 - 1 + c == W - 2 && c + 1 < RW) { // reversed copy into the outlet ghost

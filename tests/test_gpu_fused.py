@@ -144,6 +144,30 @@ def test_pair_kernel_interior_tiles(gpu, storage, chunk):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("clamp", [True, False])
+def test_pair_kernel_clamp_forms(gpu, storage, clamp):
+    """The two-step kernel takes the clamp setting as a template argument: with the clamps on, a state whose
+    densities (0.2 .. 1.9) and speeds (up to 0.6) engage them; with them off, a milder state.  Both equal two
+    single steps (run-time clamp flag) bit for bit, on a lattice with interior and edge tiles."""
+    from lbm.sim import Simulation
+    W, H = 300, 40
+    rng = np.random.default_rng(21)
+    lo, hi, um = (0.2, 1.9, 0.6) if clamp else (0.95, 1.05, 0.08)
+    rho = rng.uniform(lo, hi, (H, W))
+    ux, uy = rng.uniform(-um, um, (H, W)), rng.uniform(-um, um, (H, W))
+    mask = G.periodic_mask(W, H)
+    mask[15:20, 150:158] = 1
+    kw = dict(cs=0.15, model="mrt", bc="periodic", mask=mask, f0=G.feq(rho, ux, uy), storage=storage,
+              clamp=clamp)
+    a, b = Simulation(W, H, 1.7, **kw), Simulation(W, H, 1.7, **kw)
+    sel = _sel(a)
+    for launch in range(3):
+        a.step(2)
+        b.step(2, k=2, cfg=(0, 8))
+        _assert_equal(a, b, sel, f"clamp {clamp} launch {launch}")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_pair_kernel_cylinder_large(gpu, storage):
     """4096 x 1024 cylinder (the benchmark lattice), 400 steps with the 12-row two-step kernel: bitwise equal
     to single steps at every 20th launch."""
