@@ -102,20 +102,30 @@ __device__ __forceinline__ float zou_he(float h[9], float u, float &rho)
     return drho;
 }
 
-// Non-inline constants of the collision and of the 16-bit store clamp.  A VOP3 instruction reads at most one
-// scalar register or literal, so med3 / select with two such constants needs one of them in a VGPR; in
-// straight-line code the compiler rematerialises it (a v_mov) at every use.  A kernel that collides many cells
-// per thread in straight-line code takes opaque_consts() once: its values stay in VGPRs.
+// Non-inline constants of the collision and of the 16-bit store clamp, and the two collision parameters that
+// meet another scalar in one instruction.  A VOP3 instruction reads at most one scalar register or literal, so
+// med3 / select / fma with two such operands needs one of them in a VGPR; in straight-line code the compiler
+// rematerialises it (a v_mov) at every use.  Kernels take collide_consts(P); a kernel that collides many cells
+// per thread in straight-line code takes opaque_consts(P) once instead: its values stay in VGPRs.
 struct CollideK {
     float ulim = 0.4f;                                    // velocity clamp
     float inv15 = __builtin_bit_cast(float, 0x3F2AAAABu); // 1 / 1.5 rounded
     float inf = __builtin_inff();                         // 1 / 0
     float hi[3] = {10.0f - LBM_W0, 10.0f - LBM_W1, 10.0f - LBM_W5}; // the population clamp 10 - w_q per weight
+    float tau0sq = 0.0f, tau_inv = 0.0f;                  // KPar::tau0sq, KPar::tau_inv
 };
-__device__ __forceinline__ CollideK opaque_consts()
+__device__ __forceinline__ CollideK collide_consts(const KPar &P)
 {
     CollideK k;
+    k.tau0sq = P.tau0sq;
+    k.tau_inv = P.tau_inv;
+    return k;
+}
+__device__ __forceinline__ CollideK opaque_consts(const KPar &P)
+{
+    CollideK k = collide_consts(P);
     asm volatile("" : "+v"(k.ulim), "+v"(k.inv15), "+v"(k.inf), "+v"(k.hi[0]), "+v"(k.hi[1]), "+v"(k.hi[2]));
+    asm volatile("" : "+v"(k.tau0sq), "+v"(k.tau_inv));
     return k;
 }
 __device__ __forceinline__ constexpr int wclass(int q) { return q == 0 ? 0 : (q < 5 ? 1 : 2); }
@@ -159,7 +169,7 @@ __device__ __forceinline__ void moments_neu(const float h[9], float &drho, float
 // SUMS: rho - 1 and j as compensated sums (moments_neu) instead of the pairwise FP32 sums.
 template <bool EXT = false, bool SUMS = false, int CLAMP = -1>
 __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
-                                            float w_ext = -1.0f, const CollideK &K = CollideK())
+                                            float w_ext, const CollideK &K)
 {
     // one if/else that defines every value on both sides: defaults set before the branch would be
     // materialised for every lane, while the inlet side runs only in waves that hold inlet cells
@@ -233,7 +243,7 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
     const float m8eq = m3 * uy;
     const float n7 = m7 - m7eq;
     const float n8 = m8 - m8eq;
-    float w = P.tau_inv;
+    float w = K.tau_inv;
     bool smag = les;
     if constexpr (EXT) {
         if (w_ext >= 0.0f) {
@@ -244,7 +254,7 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
     if (smag) {
         // Smagorinsky (lbm_core4.cu:172-177)
         const float Q = sqrt_rn_normal(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
-        const float d = fma_(P.c18cs2, Q * inv_rc, P.tau0sq);
+        const float d = fma_(P.c18cs2, Q * inv_rc, K.tau0sq);
         const float tau_eff = 0.5f * (P.tau0 + sqrt_rn_normal(d));
         w = recip_rn(tau_eff);
     }
@@ -312,7 +322,7 @@ __device__ __forceinline__ void collide_bgk(const KPar &P, float h[9], bool inle
 
 template <int MODEL, bool EXT = false>
 __device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
-                                            float w_ext = -1.0f, const CollideK &K = CollideK())
+                                            float w_ext, const CollideK &K)
 {
     if constexpr (MODEL == MODEL_BGK)
         collide_bgk(P, h, inlet, uin, mac);
@@ -320,6 +330,13 @@ __device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inle
         collide_mrt<EXT, MODEL == MODEL_MRT_SUMS,
                     MODEL == MODEL_MRT_CLAMP1 ? 1 : (MODEL == MODEL_MRT_CLAMP0 ? 0 : -1)>(P, h, inlet, uin, mac,
                                                                                               w_ext, K);
+}
+
+template <int MODEL, bool EXT = false>
+__device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
+                                            float w_ext = -1.0f)
+{
+    collide_dev<MODEL, EXT>(P, h, inlet, uin, mac, w_ext, collide_consts(P));
 }
 
 // ------------------------------------------------------------------------------------------------------

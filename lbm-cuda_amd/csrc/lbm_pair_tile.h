@@ -54,7 +54,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     const rsrc_t rin = make_rsrc(fin, 9u * plane * B), rout = make_rsrc(fout, 9u * plane * B);
     const rsrc_t rm = make_rsrc(mask, plane);
     const int t = threadIdx.x;
-    const CollideK K = opaque_consts(); // kept in VGPRs across the unrolled cells
+    const CollideK K = opaque_consts(P); // kept in VGPRs across the unrolled cells
     bool any_solid = false;             // interior form: a region cell of this thread is solid
 
     // ---- phase A: even step of every region cell, reversed result into LDS.  A thread's HBM loads (mask byte
@@ -162,12 +162,12 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                 if (smb[LS + 1]) continue;
             }
             bool solid[9];
-    #pragma unroll
+#pragma unroll
             for (int q = 1; q < 9; ++q)
                 solid[q] = !FLUID && smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(OPEN && inlet && cx(q) < 0);
             float g[9], mac[3];
             g[0] = C::ld(at(0, 0, 0), 0);
-    #pragma unroll
+#pragma unroll
             for (int q = 1; q < 9; ++q) {
                 const int o = opp(q);
                 // bounce-back: this cell's slot q; otherwise the source's slot opp(q) (both read, then selected).
@@ -178,32 +178,44 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac, -1.0f, K);
             const unsigned idx = (unsigned)(gy * W + gx);
             const unsigned idxB = idx * B, pB = plane * B;
-            const unsigned idx2 = idxB + 2u * pB;
-            unsigned nbB[9];
-    #pragma unroll
-            for (int q = 1; q < 9; ++q) {
-                if constexpr (WRAP)
-                    nbB[q] = (unsigned)(wrap(gy + cy(q), H) * W + wrap(gx + cx(q), W)) * B;
-                else
-                    nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
-            }
-            S::st(rout, idxB, 0, 0, g[0]);
-    #pragma unroll
-            for (int q = 1; q < 9; ++q) {
-                // bounce-back writes this cell's slot opp(q); populations leaving through the open inlet are parked
-                // in this cell's slot opp(q) (the canonical inlet slots 1, 5, 8, rebuilt by the next Zou-He).  The
-                // planes q and opp(q) differ by 2: min(q, opp(q)) * plane is the scalar offset, and the vector
-                // offset is this cell's or the target's offset, two planes up for the larger index.
-                const int o = opp(q);
-                const bool back = solid[q] || (OPEN && inlet && cx(q) < 0);
-                const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nbB[q] + 2u * pB);
-                S::st(rout, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
-            }
-            if (OPEN && gx == W - 2) {
-                // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-                S::st(rout, idxB, 3u * pB, 3, g[3]);
-                if (!solid[2]) S::st(rout, nbB[2], 6u * pB, 6, g[6]);
-                if (!solid[4]) S::st(rout, nbB[4], 7u * pB, 7, g[7]);
+            if constexpr (FLUID) {
+                // every population streams to its target's slot q: one vector offset for all nine stores, the
+                // lowest target's (x - 1, y - 1; interior cells have idx >= W + 1), and the wave-uniform rest,
+                // (c_q,x + c_q,y W + W + 1) B + q plane B >= 0, as the scalar offset
+                const unsigned lowB = (unsigned)(W + 1) * B;
+                const unsigned vb = idxB - lowB;
+                S::st(rout, vb, lowB, 0, g[0]);
+#pragma unroll
+                for (int q = 1; q < 9; ++q)
+                    S::st(rout, vb, (unsigned)((cx(q) + cy(q) * W) * (int)B) + lowB + (unsigned)q * pB, q, g[q]);
+            } else {
+                const unsigned idx2 = idxB + 2u * pB;
+                unsigned nbB[9];
+#pragma unroll
+                for (int q = 1; q < 9; ++q) {
+                    if constexpr (WRAP)
+                        nbB[q] = (unsigned)(wrap(gy + cy(q), H) * W + wrap(gx + cx(q), W)) * B;
+                    else
+                        nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
+                }
+                S::st(rout, idxB, 0, 0, g[0]);
+#pragma unroll
+                for (int q = 1; q < 9; ++q) {
+                    // bounce-back writes this cell's slot opp(q); populations leaving through the open inlet are parked
+                    // in this cell's slot opp(q) (the canonical inlet slots 1, 5, 8, rebuilt by the next Zou-He).  The
+                    // planes q and opp(q) differ by 2: min(q, opp(q)) * plane is the scalar offset, and the vector
+                    // offset is this cell's or the target's offset, two planes up for the larger index.
+                    const int o = opp(q);
+                    const bool back = solid[q] || (OPEN && inlet && cx(q) < 0);
+                    const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nbB[q] + 2u * pB);
+                    S::st(rout, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
+                }
+                if (OPEN && gx == W - 2) {
+                    // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
+                    S::st(rout, idxB, 3u * pB, 3, g[3]);
+                    if (!solid[2]) S::st(rout, nbB[2], 6u * pB, 6, g[6]);
+                    if (!solid[4]) S::st(rout, nbB[4], 7u * pB, 7, g[7]);
+                }
             }
             if (orho) {
                 orho[idx] = mac[0];
