@@ -225,7 +225,8 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         }
     };
     if constexpr (INTERIOR) {
-        const uint64_t votes = *reinterpret_cast<const uint64_t *>(L.wsolid);
+        uint64_t votes;
+        __builtin_memcpy(&votes, L.wsolid, sizeof(votes)); // one 8-byte LDS read
         if (__builtin_amdgcn_readfirstlane((unsigned)(votes | (votes >> 32))) == 0)
             phase_b(std::true_type{});
         else
