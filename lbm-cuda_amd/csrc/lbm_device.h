@@ -217,7 +217,8 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
         rc = 1.0f + drc;
         // 1/rc == 1/rho bit for bit unless the density clamp engaged, and then rc is 1.5 or 0 exactly:
         // 1/1.5 rounds to 0x3F2AAAAB and 1/0 is +inf, so no second division is needed
-        inv_rc = drc == drho ? inv_rho : (drc > 0.0f ? K.inv15 : K.inf);
+        inv_rc = inv_rho;
+        if (__builtin_expect(drc != drho, 0)) inv_rc = drc > 0.0f ? K.inv15 : K.inf; // the density clamp engaged
         les = P.les != 0;
     }
     mac[0] = rc;
