@@ -184,10 +184,28 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                 // (c_q,x + c_q,y W + W + 1) B + q plane B >= 0, as the scalar offset
                 const unsigned lowB = (unsigned)(W + 1) * B;
                 const unsigned vb = idxB - lowB;
-                S::st(rout, vb, lowB, 0, g[0]);
+                if constexpr (B == 2) {
+                    // 16-bit: pairs (0,1) (2,3) (4,5) (6,7) converted in one instruction each; global stores take
+                    // the high half as it is (the wave-uniform part of each address is the scalar base)
+                    char *const ob = static_cast<char *>(fout);
+                    auto put = [&](int q, uint16_t v) {
+                        char *const bq = ob + ((cx(q) + cy(q) * W) * (int)B + (int)lowB) + (size_t)q * pB;
+                        *reinterpret_cast<uint16_t *>(bq + vb) = v;
+                    };
 #pragma unroll
-                for (int q = 1; q < 9; ++q)
-                    S::st(rout, vb, (unsigned)((cx(q) + cy(q) * W) * (int)B) + lowB + (unsigned)q * pB, q, g[q]);
+                    for (int q = 0; q < 8; q += 2) {
+                        const unsigned pk = C::st2(g[q], q, g[q + 1], q + 1, K);
+                        put(q, (uint16_t)pk);
+                        put(q + 1, (uint16_t)(pk >> 16));
+                    }
+                    put(8, C::st(g[8], 8, K));
+                } else {
+                    S::st(rout, vb, lowB, 0, g[0]);
+#pragma unroll
+                    for (int q = 1; q < 9; ++q)
+                        S::st(rout, vb, (unsigned)((cx(q) + cy(q) * W) * (int)B) + lowB + (unsigned)q * pB, q,
+                              g[q]);
+                }
             } else {
                 const unsigned idx2 = idxB + 2u * pB;
                 unsigned nbB[9];
