@@ -39,8 +39,8 @@ TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-
 # The same for one launch (two steps) of the two-step tile kernel, per (storage, tile rows), and its VALU
 # instructions per cell and launch (rocprofv3 --pmc SQ_INSTS_VALU x 64 / cells); the summary with the command
 # lines is profiles/k_pair_cylinder_4096x1024.md.  Configurations not listed there report null.
-PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.93e6, ("bf16", 12): 156.91e6, ("f32", 12): 311.22e6}
-PAIR_VALU_PER_CELL = {("f16", 12): 369.3, ("bf16", 12): 368.9, ("f32", 12): 376.9}
+PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.93e6, ("bf16", 12): 156.97e6, ("f32", 12): 311.19e6}
+PAIR_VALU_PER_CELL = {("f16", 12): 363.8, ("bf16", 12): 363.4, ("f32", 12): 376.9}
 # VALU issue capacity of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each, 2.4 GHz
 VALU_WAVE_INSTS_PER_US = 256 * 4 / 4 * 2400.0
 
