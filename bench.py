@@ -169,11 +169,23 @@ def bench_ensemble(nsteps, graph_steps=20):
     best_sep = min(us_s, us_l)
     res["speedup_vs_best_separate_graph"] = round(best_sep / us_p, 3)
     res["speedup_vs_separate_eager"] = round(wall_e / us_p, 3)
-    log("ensemble GLUPS: " + ", ".join(f"{k} {v['glups']}" for k, v in res.items()
-                                       if isinstance(v, dict) and "glups" in v))
     ens.close()
     del ens
     torch.cuda.empty_cache()
+    # the same members with FP16 deviation storage, tuned and timed the same way
+    ens = Ensemble.from_members(members, storage="f16", two_step=True)
+    b16, group16, _, k16, rows16 = tune_ensemble(ens, use_cache=False, steps=graph_steps, reps=2)
+    g = ens.capture(graph_steps)
+    us16, _ = timed(lambda: ens.replay(g))
+    g.close()
+    res["persistent_graph_f16"] = {"glups": rate(us16), "us_per_step": round(us16, 2), "k": k16,
+                                   "tile_rows": rows16 if k16 == 2 else None, "blocks": b16,
+                                   "group_mb": group16 >> 20}
+    ens.close()
+    del ens
+    torch.cuda.empty_cache()
+    log("ensemble GLUPS: " + ", ".join(f"{k} {v['glups']}" for k, v in res.items()
+                                       if isinstance(v, dict) and "glups" in v))
     return res
 
 
