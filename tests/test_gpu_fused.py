@@ -143,6 +143,37 @@ def test_pair_kernel_interior_tiles(gpu, storage, chunk):
         _assert_equal(a, b, sel, f"{storage} chunk {chunk} launch {launch}")
 
 
+@pytest.mark.parametrize("seed", range(12))
+def test_pair_kernel_random_configs(gpu, seed):
+    """Seeded random lattices (130..900 x 20..260), random rectangular obstacles (some on tile edges and halo
+    rows), either boundary mode, any storage and tile height: the two-step kernel's general, interior and
+    all-fluid forms together equal two single steps bit for bit."""
+    from lbm.sim import Simulation
+    rng = np.random.default_rng(1000 + seed)
+    W, H = int(rng.integers(130, 900)), int(rng.integers(20, 260))
+    periodic = bool(rng.integers(0, 2))
+    storage = ["f32", "f16", "bf16"][int(rng.integers(0, 3))]
+    chunk = [8, 12, 16, 24][int(rng.integers(0, 4))]
+    mask = G.periodic_mask(W, H) if periodic else G.channel_mask(W, H)
+    for _ in range(int(rng.integers(0, 6))):
+        h, w = int(rng.integers(1, 8)), int(rng.integers(1, 12))
+        y, x = int(rng.integers(1, H - h)), int(rng.integers(2, W - w - 2))
+        mask[y:y + h, x:x + w] = 1
+    if periodic:
+        ux, uy = G.random_phase_velocity(W, H, 4, 0.04, seed=seed)
+        kw = dict(bc="periodic", mask=mask, f0=G.feq(np.ones((H, W)), ux, uy))
+    else:
+        prof = G.parabolic_profile(H, 0.05)
+        kw = dict(bc="channel", mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, seed))
+    kw.update(cs=0.12, storage=storage)
+    a, b = Simulation(W, H, 1.75, **kw), Simulation(W, H, 1.75, **kw)
+    sel = _sel(a)
+    for launch in range(3):
+        a.step(2)
+        b.step(2, k=2, cfg=(0, chunk))
+        _assert_equal(a, b, sel, f"{W}x{H} periodic={periodic} {storage} rows {chunk} launch {launch}")
+
+
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 @pytest.mark.parametrize("clamp", [True, False])
 def test_pair_kernel_clamp_forms(gpu, storage, clamp):
