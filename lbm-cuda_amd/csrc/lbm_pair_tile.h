@@ -21,6 +21,8 @@ static_assert(NT / 64 == 8, "the per-wave votes are read as one 8-byte word");
 // v mod n for v >= -n (one period at most below the lattice; region columns may lie several periods above
 // it when the lattice is narrower than the region)
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v % n : v); }
+// v mod n for -n <= v < 2n (at most one period outside the lattice): no division
+__device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 template <int FMT, int CH> struct TileLds {
     typename Conv<FMT>::T sf[9][CH + 2][LS];
@@ -71,8 +73,15 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         int gx = x0 - 1 + c, gy = y0 - 1 + r;
         bool in = e < RH * RW;
         if constexpr (WRAP) {
-            gx = wrap(gx, W);
-            gy = wrap(gy, H);
+            // a region cell lies at most one period outside the lattice (x0 - 1 <= gx <= x0 + 128 < 2W for
+            // W >= 130, likewise gy) unless the lattice is narrower or lower than the region (lattice-uniform)
+            if (W >= RW && H >= RH) {
+                gx = wrap1(gx, W);
+                gy = wrap1(gy, H);
+            } else {
+                gx = wrap(gx, W);
+                gy = wrap(gy, H);
+            }
         } else if constexpr (OPEN) {
             // 0 <= v < n as one unsigned comparison (n > 0)
             in = in && (unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H;
@@ -212,7 +221,8 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
 #pragma unroll
                 for (int q = 1; q < 9; ++q) {
                     if constexpr (WRAP)
-                        nbB[q] = (unsigned)(wrap(gy + cy(q), H) * W + wrap(gx + cx(q), W)) * B;
+                        // core cells lie in the lattice, so a neighbour is at most one period outside it
+                        nbB[q] = (unsigned)(wrap1(gy + cy(q), H) * W + wrap1(gx + cx(q), W)) * B;
                     else
                         nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
                 }
