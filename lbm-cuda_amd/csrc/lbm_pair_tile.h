@@ -27,7 +27,7 @@ __device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >
 template <int FMT, int CH> struct TileLds {
     typename Conv<FMT>::T sf[9][CH + 2][LS];
     uint8_t sm[CH + 2][LS];           // 1 = solid or outside the lattice
-    alignas(8) uint8_t wsolid[NT / 64]; // per wave: 1 if any of its region cells is solid (interior form)
+    alignas(8) uint8_t wsolid[NT / 64]; // per wave: 1 if any of its region cells is solid (no open boundary)
 };
 
 // INTERIOR: the tile's region (core plus halo ring) lies at least one cell inside the walls, the inlet column
@@ -57,7 +57,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     const rsrc_t rm = make_rsrc(mask, plane);
     const int t = threadIdx.x;
     const CollideK K = opaque_consts(P); // kept in VGPRs across the unrolled cells
-    bool any_solid = false;             // interior form: a region cell of this thread is solid
+    bool any_solid = false;              // no open boundary: a region cell of this thread is solid
 
     // ---- phase A: even step of every region cell, reversed result into LDS.  A thread's HBM loads (mask byte
     //      + 9 populations per cell) are issued PD cells ahead of their use, and unconditionally, so their
@@ -102,7 +102,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         const int r = e / RW, c = e - r * RW;
         const bool solid = !ina[a] || ma[a] != 0;
         if (e < RH * RW) sm[r][c] = solid ? 1 : 0;
-        if constexpr (INTERIOR) any_solid = any_solid || (ina[a] && ma[a] != 0);
+        if constexpr (!OPEN) any_solid = any_solid || (ina[a] && ma[a] != 0);
         const bool inlet = la[a];
         if (!(ina[a] && ua[a] && (inlet || !solid))) return;
         float mac[3];
@@ -135,7 +135,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         collide_cell(a);
         if (a + PD < NA) load_cell(a + PD);
     }
-    if constexpr (INTERIOR) {
+    if constexpr (!OPEN) {
         // every lane of a wave writes the wave's vote (the same value) to the wave's byte
         const bool wave_solid = __ballot(any_solid) != 0;
         L.wsolid[t / 64] = wave_solid ? 1 : 0;
@@ -143,8 +143,8 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     __syncthreads();
 
     // ---- phase B: odd step of the core cells, scattered to the output ------------------------------------
-    // FLUID (interior tiles whose region holds no solid cell): no bounce-back anywhere, so the mask tests and
-    // the gather / scatter selects drop out; the values moved and the arithmetic are the same.
+    // FLUID (interior or periodic tiles whose region holds no solid cell): no bounce-back anywhere, so the mask
+    // tests and the gather / scatter selects drop out; the values moved and the arithmetic are the same.
     auto phase_b = [&](auto fluid_tag) {
         constexpr bool FLUID = decltype(fluid_tag)::value;
         for (int e = t; e < CH * CW; e += NT) {
@@ -187,7 +187,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac, -1.0f, K);
             const unsigned idx = (unsigned)(gy * W + gx);
             const unsigned idxB = idx * B, pB = plane * B;
-            if constexpr (FLUID) {
+            if constexpr (FLUID && INTERIOR) {
                 // every population streams to its target's slot q: one vector offset for all nine stores, the
                 // lowest target's (x - 1, y - 1; interior cells have idx >= W + 1), and the wave-uniform rest,
                 // (c_q,x + c_q,y W + W + 1) B + q plane B >= 0, as the scalar offset
@@ -252,7 +252,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             }
         }
     };
-    if constexpr (INTERIOR) {
+    if constexpr (!OPEN) { // interior tiles and periodic edge tiles: every region cell is in the lattice
         uint64_t votes;
         __builtin_memcpy(&votes, L.wsolid, sizeof(votes)); // one 8-byte LDS read
         if (__builtin_amdgcn_readfirstlane((unsigned)(votes | (votes >> 32))) == 0)
