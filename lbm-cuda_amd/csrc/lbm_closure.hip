@@ -139,7 +139,7 @@ extern "C" {
 int lbm_neq(const lbm_params *p, const uint8_t *mask, const float *f, int parity, float *neq, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 4);
     if (rc) return rc;
     if (p->bc != LBM_BC_PERIODIC || p->model != LBM_MRT_LES || P.sums)
         return fail(LBM_ERR_UNSUPPORTED, "lbm_neq: periodic MRT lattices with plain sums only");
@@ -163,7 +163,8 @@ int lbm_step_closure(const lbm_params *p, const uint8_t *mask, float *f, int par
     int rc = lbm_neq(p, mask, f, parity, neq, stream);
     if (rc) return rc;
     KPar P;
-    make_kpar(p, &P);
+    rc = make_kpar(p, &P, 4);
+    if (rc) return rc;
     dim3 grid((P.W + TILE_X - 1) / TILE_X, (P.H + TILE_Y - 1) / TILE_Y);
     if (parity)
         hipLaunchKernelGGL(k_closure<1>, grid, dim3(TILE_X, TILE_Y), 0, to_hip(stream), P, mask, f, neq, packed,

@@ -17,8 +17,11 @@ void set_error(const char *fmt, ...);
 int fail(int code, const char *fmt, ...);
 int check_launch(const char *what);
 
-// Validates the parameter block and derives the kernel constants (FP32, host side).
-int make_kpar(const lbm_params *p, KPar *k);
+// Validates the parameter block and derives the kernel constants (FP32, host side).  pop_bytes is the
+// size of one stored population (4 FP32, 2 for 16-bit deviations) for kernels that address the populations
+// through buffer resources, whose record size and byte offsets are unsigned 32-bit: 9*W*H*pop_bytes must
+// stay below 2^32.  pop_bytes = 0 checks only the element limit 9*W*H < 2^31 (pointer-indexed kernels).
+int make_kpar(const lbm_params *p, KPar *k, unsigned pop_bytes);
 
 inline hipStream_t to_hip(lbm_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 

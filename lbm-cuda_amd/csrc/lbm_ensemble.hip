@@ -222,7 +222,7 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
     long long tiles = 0, ptiles[ENS_NCH] = {0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         EnsDesc &d = e->h_desc[i];
-        int rc = make_kpar(&inst[i].p, &d.P);
+        int rc = make_kpar(&inst[i].p, &d.P, fmt == LBM_FMT_F32 ? 4u : 2u);
         if (rc) {
             delete e;
             return rc;

@@ -44,7 +44,7 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
                         const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
                         float *rho, KPar &P, lbm_fused_cfg &c)
 {
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, bytes);
     if (rc) return rc;
     if (P.sums) return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: compensated sums are single-step (AA) only");
     if (!mask || !f_now || !f_out) return fail(LBM_ERR_ARG, "lbm_step_fused: mask, f_now, f_out must be non-NULL");
@@ -87,6 +87,8 @@ int lbm_fused_max_k(void) { return FUSED_MAXK; }
 int lbm_fused_default_cfg(const lbm_params *p, int k, lbm_fused_cfg *cfg)
 {
     if (!p || !cfg) return fail(LBM_ERR_ARG, "lbm_fused_default_cfg: NULL pointer");
+    if (k < 1 || k > FUSED_MAXK) return fail(LBM_ERR_ARG, "lbm_fused_default_cfg: k=%d outside [1, %d]", k, FUSED_MAXK);
+    if (p->W < 3 || p->H < 3) return fail(LBM_ERR_SIZE, "lbm_fused_default_cfg: lattice %dx%d too small", p->W, p->H);
     default_cfg(p->W, p->H, k, cfg);
     return LBM_OK;
 }
@@ -119,7 +121,8 @@ int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t 
     lbm_fused_cfg c;
     int rc = fused_common(p, mask, h_now, h_out, 2, inlet_u, parity, k, cfg, ux, uy, rho, P, c);
     if (rc) return rc;
-    if (c.strip != 0) c.strip = 128; // the 16-bit strip-sweep kernels are built for 128-column strips only
+    if (c.strip != 0 && c.strip != 128) // the 16-bit strip-sweep kernels are built for 128-column strips only
+        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: 16-bit storage takes strip 0 or 128, not %d", c.strip);
     if (fmt == LBM_FMT_F16)
         rc = launch_any<FMT_F16>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);
     else

@@ -42,13 +42,17 @@ int check_launch(const char *what)
     return LBM_OK;
 }
 
-int make_kpar(const lbm_params *p, KPar *k)
+int make_kpar(const lbm_params *p, KPar *k, unsigned pop_bytes)
 {
     if (!p) return fail(LBM_ERR_ARG, "params is NULL");
     if (p->W < 3 || p->H < 3) return fail(LBM_ERR_SIZE, "lattice %dx%d too small (need W, H >= 3)", p->W, p->H);
-    // element offsets q*plane + idx are kept in 32 bits
+    // element offsets q*plane + idx are kept in signed 32 bits
     if ((long long)p->W * p->H * 9 >= (1LL << 31))
         return fail(LBM_ERR_SIZE, "lattice %dx%d too large (9*W*H must be < 2^31)", p->W, p->H);
+    // buffer resources: the record size 9*plane*bytes and every byte offset are unsigned 32-bit
+    if ((long long)p->W * p->H * 9 * pop_bytes >= (1LL << 32))
+        return fail(LBM_ERR_SIZE, "lattice %dx%d too large for %u-byte populations (9*W*H*%u must be < 2^32)", p->W,
+                    p->H, pop_bytes, pop_bytes);
     if (p->model != LBM_MRT_LES && p->model != LBM_BGK) return fail(LBM_ERR_ARG, "unknown model %d", p->model);
     if (p->bc != LBM_BC_CHANNEL && p->bc != LBM_BC_PERIODIC) return fail(LBM_ERR_ARG, "unknown bc %d", p->bc);
     if (!(p->tau_inv > 0.0f && p->tau_inv < 2.0f)) return fail(LBM_ERR_ARG, "tau_inv %g outside (0,2)", p->tau_inv);
@@ -394,7 +398,7 @@ int lbm_step_aa(const lbm_params *p, const uint8_t *mask, float *f, const float 
                 float *uy, float *rho, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 4);
     if (rc) return rc;
     if (!mask || !f) return fail(LBM_ERR_ARG, "lbm_step_aa: mask and f must be non-NULL");
     if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_step_aa: channel mode needs inlet_u");
@@ -420,7 +424,7 @@ int lbm_step_pull(const lbm_params *p, const uint8_t *mask, const float *f_now, 
                   float *rho, const float *inlet_u, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 0);
     if (rc) return rc;
     if (P.sums) return fail(LBM_ERR_UNSUPPORTED, "lbm_step_pull: compensated sums are single-step (AA) only");
     if (!mask || !f_now || !f_out || !ux || !uy || !rho)
@@ -448,7 +452,7 @@ int lbm_macro(const lbm_params *p, const uint8_t *mask, const float *f, const fl
               float *uy, float *rho, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 0);
     if (rc) return rc;
     if (!mask || !f || !ux || !uy || !rho) return fail(LBM_ERR_ARG, "lbm_macro: NULL pointer");
     if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_macro: channel mode needs inlet_u");
@@ -466,7 +470,7 @@ int lbm_visualize(const lbm_params *p, const float *ux, const float *uy, const u
                   float vort_scale, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 0);
     if (rc) return rc;
     if (!ux || !uy || !mask || !rgba) return fail(LBM_ERR_ARG, "lbm_visualize: NULL pointer");
     const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);
@@ -478,7 +482,7 @@ int lbm_step_aa_h16(const lbm_params *p, const uint8_t *mask, uint16_t *h, int f
                     float *ux, float *uy, float *rho, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 2);
     if (rc) return rc;
     if (!mask || !h) return fail(LBM_ERR_ARG, "lbm_step_aa_h16: mask and h must be non-NULL");
     if (fmt != LBM_FMT_F16 && fmt != LBM_FMT_BF16) return fail(LBM_ERR_ARG, "lbm_step_aa_h16: bad format %d", fmt);
@@ -510,7 +514,7 @@ int lbm_macro_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h, i
                   int parity, float *ux, float *uy, float *rho, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 0);
     if (rc) return rc;
     if (!mask || !h || !ux || !uy || !rho) return fail(LBM_ERR_ARG, "lbm_macro_h16: NULL pointer");
     if (fmt != LBM_FMT_F16 && fmt != LBM_FMT_BF16) return fail(LBM_ERR_ARG, "lbm_macro_h16: bad format %d", fmt);
@@ -536,7 +540,7 @@ int lbm_macro_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h, i
 int lbm_convert(const lbm_params *p, const void *src, int src_fmt, void *dst, int dst_fmt, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 0);
     if (rc) return rc;
     if (!src || !dst) return fail(LBM_ERR_ARG, "lbm_convert: NULL pointer");
     auto bytes = [](int f) { return f == LBM_FMT_F32 ? 4u : 2u; };
@@ -572,7 +576,7 @@ int lbm_diagnostics(const lbm_params *p, const uint8_t *mask, const float *f, co
                     double *work, double *out, lbm_stream_t stream)
 {
     KPar P;
-    int rc = make_kpar(p, &P);
+    int rc = make_kpar(p, &P, 0);
     if (rc) return rc;
     if (!mask || !f || !work || !out) return fail(LBM_ERR_ARG, "lbm_diagnostics: NULL pointer");
     if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_diagnostics: channel mode needs inlet_u");

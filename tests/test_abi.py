@@ -69,6 +69,88 @@ def test_fused_max_k():
     assert lbm.load().lbm_fused_max_k() == 8
 
 
+# Kernels addressing populations through buffer resources keep the record size 9*W*H*bytes and every byte
+# offset in unsigned 32 bits, so FP32 lattices stop at W*H < 2^32 / 36 = 119 304 647.1 cells and 16-bit
+# ones at the element limit 9*W*H < 2^31 (238 609 294 cells).
+FP32_MAX_CELLS = (2 ** 32 - 1) // 36          # 119 304 647
+H16_MAX_CELLS = (2 ** 31 - 1) // 9            # 238 609 294
+
+
+def _size_rc(L, fn, W, H, *rest):
+    p = _lib.params(W, H, 1.5)
+    return getattr(L, fn)(ctypes.byref(p), *rest)
+
+
+def test_fp32_byte_offset_limit():
+    """16384 x 8192 FP32 (2^27 cells, 4.8 GB) overflows 32-bit byte offsets: every FP32 entry point that
+    addresses f through buffer resources refuses it with LBM_ERR_SIZE before any argument check."""
+    L = lbm.load()
+    W, H = 16384, 8192
+    assert 9 * W * H * 4 >= 2 ** 32 and 9 * W * H < 2 ** 31
+    assert _size_rc(L, "lbm_step_aa", W, H, None, None, None, 0, None, None, None, None) == -2
+    assert b"2^32" in L.lbm_last_error()
+    assert _size_rc(L, "lbm_step_aa_n", W, H, None, None, None, 0, 1, None) == -2
+    assert _size_rc(L, "lbm_step_fused", W, H, None, None, None, None, 0, 2, None, None, None, None) == -2
+    assert _size_rc(L, "lbm_step_fused_cfg", W, H, None, None, None, None, 0, 2, None, None, None, None,
+                    None) == -2
+    assert _size_rc(L, "lbm_step_fused_n", W, H, None, None, None, None, 0, 2, 1, None, None) == -2
+    # the same lattice with 16-bit deviations passes the size check and fails on the NULL buffers instead
+    assert _size_rc(L, "lbm_step_aa_h16", W, H, None, None, 1, None, 0, None, None, None, None) == -1
+    assert _size_rc(L, "lbm_step_fused_h16", W, H, None, None, None, 1, None, 0, 2, None, None, None, None,
+                    None) == -1
+    # the closure steps FP32 populations through buffer resources too
+    assert _size_rc(L, "lbm_neq", W, H, None, None, 0, None, None) == -2
+
+
+def test_fp32_byte_offset_limit_boundary():
+    """One row either side of the limit: the largest FP32 lattice accepted, the next one refused."""
+    L = lbm.load()
+    W = 16384
+    H_ok = FP32_MAX_CELLS // W                    # 7281 rows: 9*W*H*4 < 2^32
+    assert 9 * W * H_ok * 4 < 2 ** 32 <= 9 * W * (H_ok + 1) * 4
+    assert _size_rc(L, "lbm_step_aa", W, H_ok, None, None, None, 0, None, None, None, None) == -1  # NULL mask
+    assert _size_rc(L, "lbm_step_aa", W, H_ok + 1, None, None, None, 0, None, None, None, None) == -2
+    # 16-bit: the element limit decides
+    W2 = 32768
+    H2 = H16_MAX_CELLS // W2
+    assert _size_rc(L, "lbm_step_aa_h16", W2, H2, None, None, 1, None, 0, None, None, None, None) == -1
+    assert _size_rc(L, "lbm_step_aa_h16", W2, H2 + 1, None, None, 1, None, 0, None, None, None, None) == -2
+
+
+def test_ensemble_rejects_oversized_fp32_member():
+    from lbm.ensemble import LbmInstance
+    L = lbm.load()
+    out = ctypes.c_void_p()
+    inst = (LbmInstance * 1)()
+    inst[0] = LbmInstance(_lib.params(16384, 8192, 1.5, bc=_lib.PERIODIC), 1, 1, 0)
+    assert L.lbm_ensemble_create(inst, 1, _lib.FMT_F32, ctypes.byref(out)) == -2
+    assert not out.value
+
+
+def test_fused_default_cfg_validates():
+    L = lbm.load()
+    cfg = _lib.LbmFusedCfg()
+    p = _lib.params(4096, 1024, 1.5)
+    assert L.lbm_fused_default_cfg(ctypes.byref(p), 2, ctypes.byref(cfg)) == 0
+    assert cfg.strip == 128 and cfg.chunk >= 1
+    for k in (0, 9, 63, 100):
+        assert L.lbm_fused_default_cfg(ctypes.byref(p), k, ctypes.byref(cfg)) == -1
+    small = _lib.params(2, 2, 1.5)
+    assert L.lbm_fused_default_cfg(ctypes.byref(small), 2, ctypes.byref(cfg)) == -2
+
+
+def test_fused_h16_rejects_strip_64():
+    """16-bit strip kernels exist for 128-column strips only: 64 is refused, not rewritten."""
+    L = lbm.load()
+    p = _lib.params(64, 32, 1.5)
+    cfg = _lib.LbmFusedCfg(64, 16)
+    # non-NULL dummy pointers that are never dereferenced: the configuration check comes first
+    m, a, b, u = 0x1000, 0x100000, 0x200000, 0x300000
+    rc = L.lbm_step_fused_h16(ctypes.byref(p), m, a, b, _lib.FMT_F16, u, 0, 2, ctypes.byref(cfg), None, None,
+                              None, None)
+    assert rc == -3 and b"strip" in L.lbm_last_error()
+
+
 def test_ensemble_and_graph_argument_errors():
     """Ensemble / graph entry points reject bad arguments before touching the GPU."""
     from lbm.ensemble import LbmInstance
