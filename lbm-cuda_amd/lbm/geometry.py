@@ -121,11 +121,11 @@ def shear_layer_velocity(W, H, u0=0.04, delta=80.0, amp=0.002, noise=1e-4, seed=
     return ux, uy
 
 
-def taylor_green_velocity(W, H, u0, t=0.0, nu=0.0):
-    """Decaying Taylor-Green vortex with kx = 2 pi / W, ky = 2 pi / H:
-    ux = -u0 (ky/kx)... normalised so that ux = -u0 cos(kx x) sin(ky y) * (ky/k), uy = u0 sin(kx x) cos(ky y) * (kx/k)
-    is divergence free; amplitude decays as exp(-nu (kx^2 + ky^2) t)."""
-    kx, ky = 2.0 * math.pi / W, 2.0 * math.pi / H
+def taylor_green_velocity(W, H, u0, t=0.0, nu=0.0, mx=1, my=1):
+    """Decaying Taylor-Green vortex array with kx = 2 pi mx / W, ky = 2 pi my / H (mx, my whole periods across
+    the lattice): ux = -u0 (ky/k) cos(kx x) sin(ky y) e^(-nu k^2 t), uy = u0 (kx/k) sin(kx x) cos(ky y) e^(-nu k^2 t),
+    k^2 = kx^2 + ky^2 -- divergence free, amplitude u0 at t = 0."""
+    kx, ky = 2.0 * math.pi * mx / W, 2.0 * math.pi * my / H
     x = np.arange(W, dtype=np.float64)[None, :]
     y = np.arange(H, dtype=np.float64)[:, None]
     k = math.hypot(kx, ky)
@@ -135,11 +135,11 @@ def taylor_green_velocity(W, H, u0, t=0.0, nu=0.0):
     return np.broadcast_to(ux, (H, W)).copy(), np.broadcast_to(uy, (H, W)).copy()
 
 
-def taylor_green_density(W, H, u0):
+def taylor_green_density(W, H, u0, mx=1, my=1):
     """rho = 1 + 3 p of the Taylor-Green pressure p = -(A^2 cos(2 kx x) + B^2 cos(2 ky y)) / 4, A = u0 ky / k,
-    B = u0 kx / k (the velocity of taylor_green_velocity); starting from it avoids the acoustic transient of a
-    uniform density."""
-    kx, ky = 2.0 * math.pi / W, 2.0 * math.pi / H
+    B = u0 kx / k (the velocity of taylor_green_velocity with the same modes); starting from it avoids the
+    acoustic transient of a uniform density."""
+    kx, ky = 2.0 * math.pi * mx / W, 2.0 * math.pi * my / H
     k = math.hypot(kx, ky)
     a, b = u0 * ky / k, u0 * kx / k
     x = np.arange(W, dtype=np.float64)[None, :]

@@ -49,10 +49,23 @@ def params(W, H, tau_inv, cs=0.0, model=MRT, bc=CHANNEL, clamp=1, s_e=1.1, s_eps
                         float(s_e), float(s_eps), float(s_q), int(sums))
 
 
+# -mfma: fma()/fmaf() become the hardware fused multiply-add instead of libm's software emulation (both are
+# correctly rounded, so the results are the same bits); -ffp-contract=off still forbids contracting anything
+# that is not written as an fma; -fno-math-errno drops the errno branch around sqrt (inputs are positive).
+CFLAGS = ["-O2", "-mfma", "-fno-math-errno", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-Wall"]
+
+
+def _host_has_fma():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            return any(line.startswith("flags") and " fma " in line + " " for line in fh)
+    except OSError:
+        return True  # no /proc: let the loader fail loudly if the instruction is missing
+
+
 def build(verbose=False):
     """Compile the C restatement with gcc (no GPU, no reference sources involved)."""
-    cmd = ["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-Wall",
-           "-o", LIB_PATH, os.path.join(_HERE, "lbm_oracle.c"), "-lm"]
+    cmd = ["gcc"] + CFLAGS + ["-o", LIB_PATH, os.path.join(_HERE, "lbm_oracle.c"), "-lm"]
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True, cwd=_HERE)
@@ -66,6 +79,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        if not _host_has_fma():
+            raise RuntimeError("the oracle is built with -mfma and this host CPU has no FMA3 instructions")
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER(OracleParams)
         vp = ctypes.c_void_p

@@ -105,6 +105,39 @@ def _mrt(f, rho, ux, uy, omega_stress, s_e, s_eps, s_q):
     return np.minimum(10.0, np.maximum(0.0, out))
 
 
+def _moments(g, clamp):
+    """rho, ux, uy of gathered populations g [9, n] with the clamps of lbm_core4.cu:139-144."""
+    rho = g.sum(axis=0)
+    ux = (g * CX[:, None]).sum(axis=0) / rho
+    uy = (g * CY[:, None]).sum(axis=0) / rho
+    if clamp:
+        ux = np.minimum(0.4, np.maximum(-0.4, ux))
+        uy = np.minimum(0.4, np.maximum(-0.4, uy))
+        rho = np.minimum(1.5, np.maximum(0.0, rho))
+    return rho, ux, uy
+
+
+def _neq(g, rho, ux, uy):
+    """Non-equilibrium parts of the stress moments (lbm_core4.cu:172-173): m7 - m7eq, m8 - m8eq."""
+    m7 = g[1] - g[2] + g[3] - g[4]
+    m8 = g[5] - g[6] + g[7] - g[8]
+    return m7 - rho * (ux * ux - uy * uy), m8 - rho * ux * uy
+
+
+def neq(f, mask, parity, clamp=True, periodic=True):
+    """m7neq, m8neq [2, H, W] of every fluid cell's pre-collision populations (0 elsewhere), for an AA state
+    ``f`` of the given parity: canonical (0) holds the pre-collision populations in place; reversed (1) holds
+    post-collision ones in opposite slots, f[opp q] = f*_q, which stream in as a pull step gathers them."""
+    _, H, W = f.shape
+    g = f if parity == 0 else _gather(f[OPP], mask, periodic)
+    fluid, _ = _roles(mask, periodic)
+    out = np.zeros((2, H, W))
+    gi = g[:, fluid]
+    rho, ux, uy = _moments(gi, clamp)
+    out[0][fluid], out[1][fluid] = _neq(gi, rho, ux, uy)
+    return out
+
+
 def pull_step(f_now, mask, inlet_u, tau_inv, cs, s_e=1.1, s_eps=1.1, s_q=1.1, clamp=True, periodic=False):
     """One lbm_core4 step; returns (f_out, rho, ux, uy).  Macro arrays are 0 where not written."""
     _, H, W = f_now.shape
@@ -115,22 +148,11 @@ def pull_step(f_now, mask, inlet_u, tau_inv, cs, s_e=1.1, s_eps=1.1, s_q=1.1, cl
 
     # interior fluid, lbm_core4.cu:124-199
     gi = g[:, fluid]
-    rho = gi.sum(axis=0)
-    ux = (gi * CX[:, None]).sum(axis=0) / rho
-    uy = (gi * CY[:, None]).sum(axis=0) / rho
-    if clamp:
-        ux = np.minimum(0.4, np.maximum(-0.4, ux))
-        uy = np.minimum(0.4, np.maximum(-0.4, uy))
-        rho = np.minimum(1.5, np.maximum(0.0, rho))
+    rho, ux, uy = _moments(gi, clamp)
     rho_o[fluid], ux_o[fluid], uy_o[fluid] = rho, ux, uy
     omega = np.full(rho.shape, float(tau_inv))
     if cs != 0.0:
-        f14 = gi[1] + gi[2] + gi[3] + gi[4]
-        m7 = gi[1] - gi[2] + gi[3] - gi[4]
-        m8 = gi[5] - gi[6] + gi[7] - gi[8]
-        m7n = m7 - rho * (ux * ux - uy * uy)
-        m8n = m8 - rho * ux * uy
-        del f14
+        m7n, m8n = _neq(gi, rho, ux, uy)
         Q = np.sqrt(m7n * m7n + m8n * m8n + 1e-8)
         tau0 = 1.0 / tau_inv
         tau_eff = 0.5 * (tau0 + np.sqrt(tau0 * tau0 + 18.0 * cs * cs * Q / rho))

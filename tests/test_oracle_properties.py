@@ -79,3 +79,29 @@ def test_collision_conserves_mass_and_momentum(oracle_lib, rho, ux, uy, noise, s
 def test_mrt_transform_round_trip(oracle_lib, seed):
     f = np.random.default_rng(seed).standard_normal(9)
     assert np.allclose(O.mrt_inverse(O.mrt_forward(f)), f, atol=1e-13)
+
+
+def _uniform_flow_growth(u, W=128, H=32, steps=4000, seed=1):
+    """rms velocity perturbation after `steps` FP64 steps of a uniform flow u along x with 1e-6 noise, at
+    nu = 1e-4 (tau_inv = 1.99880..., MRT without LES, the reference's rates s = 1.1), over the initial one."""
+    tau_inv = float(np.float32(G.nu_tau_inv(1e-4)))
+    op = O.params(W, H, tau_inv, cs=0.0, model=O.MRT, bc=O.PERIODIC)
+    rng = np.random.default_rng(seed)
+    ux = np.full((H, W), u) + rng.uniform(-1e-6, 1e-6, (H, W))
+    uy = rng.uniform(-1e-6, 1e-6, (H, W))
+    f = np.ascontiguousarray(G.feq(np.ones((H, W)), ux, uy))
+    mask, z = G.periodic_mask(W, H), np.zeros(H)
+    rms0 = float(np.sqrt(((ux - ux.mean()) ** 2 + (uy - uy.mean()) ** 2).mean()))
+    par = O.aa_steps(op, mask, f, z, 0, steps)
+    _, a, b = O.macro(op, mask, f, z, par)
+    return float(np.sqrt(((a - a.mean()) ** 2 + (b - b.mean()) ** 2).mean())) / rms0
+
+
+def test_mrt_low_viscosity_stability_limit(oracle_lib):
+    """A property of the reference's scheme that the precision workload (nu = 1e-4, u ~ 0.04) runs into: with
+    the MRT rates s_e = s_eps = s_q = 1.1 (lbm_core4.cu:6) and the stress rate 1.9988, a uniform flow is stable at
+    u = 0.02 (perturbations decay) but linearly unstable at u = 0.039 (grid-scale modes grow; on 256 x 64 about
+    4x per 1000 steps), in FP64.  The GPU kernels reproduce the same arithmetic bit for bit, so their long shear-layer and
+    Taylor-Green runs inherit it."""
+    assert _uniform_flow_growth(0.02) < 1.0
+    assert _uniform_flow_growth(0.039) > 3.0  # measured 7.3 (128 x 32, 4000 steps); 0.19 at u = 0.02
