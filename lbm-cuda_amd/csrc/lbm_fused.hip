@@ -14,14 +14,25 @@ int launch_fused_fmt(const KPar &P, int model, int bc, int k, int strip, int par
 template <int FMT>
 int launch_pair_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, const uint8_t *mask, const void *fin,
                     void *fout, const float *inlet_u, float *ux, float *uy, float *rho);
+// defined in lbm_quad.hip (FP32 populations)
+template <int FMT>
+int launch_quad_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, const uint8_t *mask, const void *fin,
+                    void *fout, const float *inlet_u, float *ux, float *uy, float *rho);
 
-// strip = 0 selects the two-step tile kernel (k = 2 from an even state, chunk = 8 or 16 tile rows); otherwise
-// the strip-sweep kernel
+// strip = 0 selects a tile kernel from an even state -- the two-step kernel for k = 2 (chunk = 8, 12, 16 or 24
+// tile rows), the four-step kernel for k = 4 (chunk = 8, 12 or 16 rows of 128-wide tiles, 20, 24 or 32 rows of
+// 64-wide ones; FP32 populations); otherwise the strip-sweep kernel
 template <int FMT>
 static int launch_any(const KPar &P, const lbm_params *p, int k, const lbm_fused_cfg &c, int parity, hipStream_t st,
                       const uint8_t *mask, const void *fin, void *fout, const float *inlet_u, float *ux, float *uy,
                       float *rho)
 {
+    if (c.strip == 0 && k == 4) {
+        if constexpr (FMT == FMT_F32)
+            return launch_quad_fmt<FMT>(P, p->model, p->bc, c.chunk, st, mask, fin, fout, inlet_u, ux, uy, rho);
+        else
+            return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: the four-step tile kernel stores FP32 populations only");
+    }
     if (c.strip == 0) return launch_pair_fmt<FMT>(P, p->model, p->bc, c.chunk, st, mask, fin, fout, inlet_u, ux, uy, rho);
     return launch_fused_fmt<FMT>(P, p->model, p->bc, k, c.strip, parity, st, mask, fin, fout, inlet_u, c.chunk, ux,
                                  uy, rho);
@@ -63,10 +74,16 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
     else
         default_cfg(P.W, P.H, k, &c);
     if (c.strip == 0) {
-        if (k != 2 || parity != 0)
-            return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: the two-step tile kernel (strip 0) needs k = 2 and parity 0");
-        if (c.chunk != 8 && c.chunk != 12 && c.chunk != 16 && c.chunk != 24)
-            return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 needs chunk 8, 12, 16 or 24");
+        if ((k != 2 && k != 4) || parity != 0)
+            return fail(LBM_ERR_UNSUPPORTED,
+                        "lbm_step_fused: the tile kernels (strip 0) need k = 2 or 4 and parity 0");
+        if (k == 2 && c.chunk != 8 && c.chunk != 12 && c.chunk != 16 && c.chunk != 24)
+            return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 with k = 2 needs chunk 8, 12, 16 or 24");
+        if (k == 4 && c.chunk != 8 && c.chunk != 12 && c.chunk != 16 && c.chunk != 20 && c.chunk != 24 &&
+            c.chunk != 32)
+            return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 with k = 4 needs chunk 8, 12, 16, 20, 24 or 32");
+        if (k == 4 && bytes != 4)
+            return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: the four-step tile kernel stores FP32 populations only");
         return LBM_OK;
     }
     if (c.strip != 64 && c.strip != 128) return fail(LBM_ERR_ARG, "lbm_step_fused: strip must be 0, 64 or 128");

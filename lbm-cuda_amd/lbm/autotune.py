@@ -4,7 +4,8 @@ The reference fixes its tile (SQUARE_A = 24) and T at compile time and mirrors t
 (lbm_smemaa_main.py:93-95) -- its compile-time traversal-order table (lbm_core_smemaa_policy.cu:12-24) is
 the only tuning knob.  Here every candidate is timed on the GPU with HIP events on scratch copies of the
 state (the simulation itself is not advanced) and the winner is cached per lattice size class.  For k = 2
-from an even state the candidates include the two-step tile kernel (strip 0).
+from an even state the candidates include the two-step tile kernel (strip 0), and for k = 4 with FP32 populations
+the four-step tile kernel.
 """
 from __future__ import annotations
 
@@ -94,10 +95,14 @@ def time_config(sim, k, cfg, reps=10, bufs=None):
 
 
 def candidates(sim, k):
-    """(strip, chunk) pairs to time: strip-sweep shapes, plus the two-step tile kernel (strip 0, chunk = tile
-    rows) when k = 2 and the state is even."""
+    """(strip, chunk) pairs to time: strip-sweep shapes, plus the tile kernels (strip 0, chunk = tile rows) from
+    an even state: the two-step kernel for k = 2, the four-step kernel for k = 4 (FP32 populations)."""
     H = sim.params.H
-    out = [(0, ch) for ch in (8, 12, 16, 24)] if k == 2 and sim.parity == 0 else []
+    out = []
+    if sim.parity == 0 and k == 2:
+        out = [(0, ch) for ch in (8, 12, 16, 24)]
+    elif sim.parity == 0 and k == 4 and sim.fmt == _lib.FMT_F32:
+        out = [(0, ch) for ch in (8, 12, 16, 20, 24, 32)]
     for strip in (STRIPS if sim.fmt == _lib.FMT_F32 else (128,)):
         if strip - 2 * (k + 1) < 16:
             continue

@@ -324,3 +324,183 @@ def pair(f_in, mask, inlet_u, p, periodic, CH=8):
                         if not solid[4]:
                             put(7, gy - 1, gx, o9[7])
     return out, written
+
+
+def quad(f_in, mask, inlet_u, p, periodic, CH=8):
+    """k_quad (lbm-cuda_amd/csrc/lbm_quad.hip) on the whole grid: per CW x CH tile with a 3-cell halo,
+    A1 (even step of the (CW + 6) x (CH + 6) region into an LDS image), B1 (odd step of ring <= 2 in place in the
+    image), A2 (even step of ring <= 1 in place), B2 (odd step of the core, scattered to the output).  Returns
+    (f_out, writers): slots not written keep NaN.  Every LDS and lattice index is bounds-checked, and each
+    in-place phase is checked race free: no LDS location is written by two cells, and no cell reads a
+    location that another cell of the same phase writes."""
+    _, H, W = f_in.shape
+    CWq, R = (128 if CH <= 16 else 64), 3  # the kernel's shape table: 128 x {8, 12, 16}, 64 x {20, 24, 32}
+    RW, RH = CWq + 2 * R, CH + 2 * R
+    out = np.full_like(f_in, np.nan)
+    written = np.zeros(f_in.shape, np.int32)
+
+    def coords(x0, y0, r, c):
+        gx, gy = x0 - R + c, y0 - R + r
+        if periodic:
+            return gx % W, gy % H, True, gx, gy
+        return gx, gy, 0 <= gx < W and 0 <= gy < H, gx, gy
+
+    def updated(gx, gy, inside, solid):
+        if periodic:
+            return not solid, False
+        inlet = gx == 0
+        return inside and 1 <= gy <= H - 2 and 0 <= gx <= W - 2 and (inlet or not solid), inlet
+
+    def col(g9, inlets, uins):
+        return _collide(np.array(g9).T.copy(), np.array(inlets), np.array(uins), p)
+
+    for bx in range((W + CWq - 1) // CWq):
+        for by in range((H + CH - 1) // CH):
+            x0, y0 = bx * CWq, by * CH
+            sf = np.full((9, RH, RW), np.nan)
+            sm = np.ones((RH, RW), bool)
+            # ---- A1
+            cells, g9, inl, uin = [], [], [], []
+            for r in range(RH):
+                for c in range(RW):
+                    gx, gy, inside, _, _ = coords(x0, y0, r, c)
+                    solid = bool(mask[gy, gx]) if inside else True
+                    sm[r, c] = solid
+                    comp, inlet = updated(gx, gy, inside, solid)
+                    if comp:
+                        cells.append((r, c, gx))
+                        g9.append(f_in[:, gy, gx].astype(np.float64))
+                        inl.append(inlet)
+                        uin.append(inlet_u[gy] if inlet else 0.0)
+            if cells:
+                o = col(g9, inl, uin)
+                for i, (r, c, gx) in enumerate(cells):
+                    sf[OPP, r, c] = o[:, i]
+                    if not periodic and gx == W - 2 and c + 1 < RW:
+                        sf[OPP, r, c + 1] = o[:, i]
+
+            def b_gather(r, c, gx, inlet):
+                solid = [False] + [bool(sm[r + CY[q], c + CX[q]]) and not (not periodic and inlet and CX[q] < 0)
+                                   for q in range(1, 9)]
+                g, reads = np.empty(9), [(0, r, c)]
+                g[0] = sf[0, r, c]
+                for q in range(1, 9):
+                    o = OPP[q]
+                    loc = (q, r, c) if solid[o] else (o, r - CY[q], c - CX[q])
+                    assert 0 <= loc[1] < RH and 0 <= loc[2] < RW
+                    g[q] = sf[loc]
+                    reads.append(loc)
+                if inlet:
+                    g[[1, 5, 8]] = 0.0  # arrive from x = -1: rebuilt by Zou-He, never read
+                return g, solid, reads
+
+            def phase_cells(ring):
+                for r in range(R - ring, R + CH + ring):
+                    for c in range(R - ring, R + CWq + ring):
+                        gx, gy, inside, _, _ = coords(x0, y0, r, c)
+                        comp, inlet = updated(gx, gy, inside, sm[r, c])
+                        if comp:
+                            yield r, c, gx, gy, inlet
+
+            def race_free(reads, writes):
+                owner = {}
+                for i, ws in enumerate(writes):
+                    for loc in ws:
+                        assert loc not in owner, f"LDS {loc} written by two cells"
+                        owner[loc] = i
+                for i, rs in enumerate(reads):
+                    for loc in rs:
+                        assert owner.get(loc, i) == i, f"LDS {loc} read by one cell and written by another"
+
+            # ---- B1: ring <= 2, in place
+            cs_, g9, inl, uin, rds, sol = [], [], [], [], [], []
+            for r, c, gx, gy, inlet in phase_cells(2):
+                g, solid, reads = b_gather(r, c, gx, inlet)
+                cs_.append((r, c, gx))
+                g9.append(g)
+                inl.append(inlet)
+                uin.append(inlet_u[gy] if inlet else 0.0)
+                rds.append(reads)
+                sol.append(solid)
+            wrs = []
+            if cs_:
+                o = col(g9, inl, uin)
+                pend = []
+                for i, (r, c, gx) in enumerate(cs_):
+                    ws = [((0, r, c), o[0, i])]
+                    for q in range(1, 9):
+                        back = sol[i][q] or (not periodic and inl[i] and CX[q] < 0)
+                        ws.append(((OPP[q], r, c) if back else (q, r + CY[q], c + CX[q]), o[q, i]))
+                    if not periodic and gx == W - 2:
+                        ws.append(((3, r, c), o[3, i]))
+                        if not sol[i][2]:
+                            ws.append(((6, r + 1, c), o[6, i]))
+                        if not sol[i][4]:
+                            ws.append(((7, r - 1, c), o[7, i]))
+                    for loc, _ in ws:
+                        assert 0 <= loc[1] < RH and 0 <= loc[2] < RW
+                    wrs.append([loc for loc, _ in ws])
+                    pend.append(ws)
+                race_free(rds, wrs)
+                for ws in pend:
+                    for loc, v in ws:
+                        sf[loc] = v
+            # ---- A2: ring <= 1, in place (local)
+            cs_, g9, inl, uin, rds, wrs = [], [], [], [], [], []
+            for r, c, gx, gy, inlet in phase_cells(1):
+                cs_.append((r, c, gx))
+                g9.append(sf[:, r, c].copy())
+                inl.append(inlet)
+                uin.append(inlet_u[gy] if inlet else 0.0)
+                rds.append([(q, r, c) for q in range(9)])
+                w = [(q, r, c) for q in range(9)]
+                if not periodic and gx == W - 2:
+                    w += [(q, r, c + 1) for q in range(9)]
+                wrs.append(w)
+            if cs_:
+                race_free(rds, wrs)
+                o = col(g9, inl, uin)
+                for i, (r, c, gx) in enumerate(cs_):
+                    sf[OPP, r, c] = o[:, i]
+                    if not periodic and gx == W - 2:
+                        sf[OPP, r, c + 1] = o[:, i]
+            # ---- B2: core -> output
+            cs_, g9, inl, uin, sol = [], [], [], [], []
+            for r in range(R, R + CH):
+                for c in range(R, R + CWq):
+                    gx, gy = x0 + c - R, y0 + r - R
+                    if gx >= W or gy >= H:
+                        continue
+                    comp, inlet = updated(gx, gy, True, sm[r, c])
+                    if not comp:
+                        continue
+                    g, solid, _ = b_gather(r, c, gx, inlet)
+                    cs_.append((gx, gy))
+                    g9.append(g)
+                    inl.append(inlet)
+                    uin.append(inlet_u[gy] if inlet else 0.0)
+                    sol.append(solid)
+            if not cs_:
+                continue
+            o = col(g9, inl, uin)
+
+            def put(q, y, x, v):
+                assert 0 <= y < H and 0 <= x < W
+                out[q, y, x] = v
+                written[q, y, x] += 1
+
+            for i, (gx, gy) in enumerate(cs_):
+                put(0, gy, gx, o[0, i])
+                for q in range(1, 9):
+                    back = sol[i][q] or (not periodic and inl[i] and CX[q] < 0)
+                    if back:
+                        put(OPP[q], gy, gx, o[q, i])
+                    else:
+                        put(q, (gy + CY[q]) % H, (gx + CX[q]) % W, o[q, i])
+                if not periodic and gx == W - 2:
+                    put(3, gy, gx, o[3, i])
+                    if not sol[i][2]:
+                        put(6, gy + 1, gx, o[6, i])
+                    if not sol[i][4]:
+                        put(7, gy - 1, gx, o[7, i])
+    return out, written

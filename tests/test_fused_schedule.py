@@ -85,3 +85,29 @@ def test_pair_schedule_matches_two_single_steps(oracle_lib, periodic, model, W, 
         assert not np.isnan(out[sel]).any(), f"launch {launch}: observable slot not written"
         assert np.abs(out - ref)[sel].max() < 1e-13, f"launch {launch}"
         f = np.where(np.isnan(out), 1e30, out)  # unwritten slots hold garbage: the next launch must not read them
+
+
+@pytest.mark.parametrize("periodic,model,W,H,CH", [(False, O.MRT, 150, 40, 8), (True, O.MRT, 140, 36, 16),
+                                                   (False, O.BGK, 131, 19, 8), (True, O.MRT, 40, 12, 8),
+                                                   (False, O.MRT, 70, 9, 16), (False, O.MRT, 300, 50, 12),
+                                                   (True, O.BGK, 136, 30, 12), (False, O.MRT, 150, 60, 24),
+                                                   (True, O.MRT, 90, 50, 20), (False, O.BGK, 70, 40, 32),
+                                                   (True, O.MRT, 40, 12, 24)])
+def test_quad_schedule_matches_four_single_steps(oracle_lib, periodic, model, W, H, CH):
+    """The four-step tile kernel's phases (tests/fused_emul.py: quad) -- the even step of the 3-ring region,
+    the odd step of ring 2 and the even step of ring 1 in place in LDS, the core's odd step scattered out --
+    reproduce four FP64 oracle AA steps on every observable slot, chained over three launches; the in-place
+    phases are race free and no output slot has two writers."""
+    cs = 0.1 if model == O.MRT else 0.0
+    op = O.params(W, H, 1.7, cs=cs, model=model, bc=O.PERIODIC if periodic else O.CHANNEL)
+    mask, prof, f = _case(W, H, periodic, 0, op)
+    p = dict(tau_inv=F32(1.7), cs=F32(cs), model=model, clamp=1, s_e=F32(1.1), s_eps=F32(1.1), s_q=F32(1.1))
+    ref = f.copy()
+    for launch in range(3):
+        out, written = E.quad(f, mask, prof, p, periodic, CH)
+        assert written.max() <= 1, "an output slot has two writers"
+        O.aa_steps(op, mask, ref, prof, 0, 4)
+        sel = O.observable(op, mask, 0)
+        assert not np.isnan(out[sel]).any(), f"launch {launch}: observable slot not written"
+        assert np.abs(out - ref)[sel].max() < 1e-13, f"launch {launch}"
+        f = np.where(np.isnan(out), 1e30, out)  # unwritten slots hold garbage: the next launch must not read them

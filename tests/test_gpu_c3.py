@@ -95,6 +95,11 @@ def test_shear_layer_two_step_kernel(gpu, storage, rows):
     _equal_launches(storage, 2, (0, rows), 25)
 
 
+@pytest.mark.parametrize("rows", [8, 12, 16, 20, 24, 32])
+def test_shear_layer_four_step_kernel(gpu, rows):
+    _equal_launches("f32", 4, (0, rows), 12)
+
+
 @pytest.mark.parametrize("storage,k,cfg", [("f32", 4, (64, 32)), ("f32", 8, (128, 64)), ("f32", 3, (64, 128)),
                                            ("f16", 4, (128, 64)), ("bf16", 8, (128, 32)), ("f16", 5, (128, 128))])
 def test_shear_layer_strip_kernel(gpu, storage, k, cfg):

@@ -72,7 +72,7 @@ def test_autotuner_picks_a_valid_config(gpu, tmp_path, monkeypatch):
     best, table = autotune.tune_fused(fusedsim, ks=(2, 4))
     assert torch.equal(before, fusedsim.f)  # tuning works on scratch copies
     k, strip, chunk, us = best
-    assert k in (2, 4) and strip in ((0, 64, 128) if k == 2 else (64, 128)) and us > 0
+    assert k in (2, 4) and strip in (0, 64, 128) and us > 0
     sel = _sel(single)
     single.step(2 * k)
     fusedsim.step(2 * k, k=k, cfg=(strip, chunk))
@@ -286,3 +286,131 @@ def test_fused_n_equals_single_launches(gpu, storage, cfg):
     s2 = Simulation(W, H, 1.75, **kw)
     s2.step(3 * k, k=k, cfg=cfg)  # Simulation.step issues the run through the same entry point
     assert int(((a.f != s2.f) & sel).sum()) == 0 and s2.parity == a.parity
+
+
+# ---------------------------------------------------------------------------------------------------------
+# The four-step tile kernel (strip 0, k = 4, FP32 populations): bit for bit against four single steps
+# ---------------------------------------------------------------------------------------------------------
+
+QUAD_CHUNKS = [8, 12, 16, 20, 24, 32]  # 128-wide tiles of 8, 12, 16 rows; 64-wide tiles of 20, 24, 32 rows
+
+
+@pytest.mark.parametrize("chunk", QUAD_CHUNKS)
+@pytest.mark.parametrize("periodic,model", [(False, "mrt"), (True, "mrt"), (False, "bgk"), (True, "bgk")])
+def test_quad_kernel_equals_four_single_steps(gpu, chunk, periodic, model):
+    """Ragged sizes, lattices narrower or lower than one 134 x (CH + 6) region, and periodic wraps."""
+    sizes = [(333, 97), (134, 16), (70, 9)] if not periodic else [(300, 70), (100, 30), (40, 12), (66, 44)]
+    for W, H in sizes:
+        single, fusedsim = _pair(W, H, periodic, model)
+        sel = _sel(single)
+        for launch in range(3):
+            single.step(4)
+            fusedsim.step(4, k=4, cfg=(0, chunk))
+            _assert_equal(single, fusedsim, sel, f"{W}x{H} chunk {chunk} launch {launch}")
+
+
+@pytest.mark.parametrize("chunk", QUAD_CHUNKS)
+def test_quad_kernel_interior_tiles(gpu, chunk):
+    """Interior tiles (region clear of walls, inlet and outlet) with obstacles inside them, across tile edges
+    and on the halo rows, and all-fluid interior tiles (the votes' fast path)."""
+    from lbm.sim import Simulation
+    W, H = 700, 150
+    mask = G.channel_mask(W, H, cylinder=(300, 70, 20))
+    mask[40:60, 380:395] = 1
+    mask[chunk - 2:chunk + 1, 500:520] = 1
+    prof = G.parabolic_profile(H, 0.05)
+    kw = dict(cs=0.12, bc="channel", mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, 11))
+    a, b = Simulation(W, H, 1.75, **kw), Simulation(W, H, 1.75, **kw)
+    sel = _sel(a)
+    for launch in range(4):
+        a.step(4)
+        b.step(4, k=4, cfg=(0, chunk))
+        _assert_equal(a, b, sel, f"chunk {chunk} launch {launch}")
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_quad_kernel_random_configs(gpu, seed):
+    """Seeded random lattices (130..900 x 20..260), random obstacles (some on tile edges and halo rows), either
+    boundary mode, any tile height: the four-step kernel's forms together equal four single steps."""
+    from lbm.sim import Simulation
+    rng = np.random.default_rng(2000 + seed)
+    W, H = int(rng.integers(130, 900)), int(rng.integers(20, 260))
+    periodic = bool(rng.integers(0, 2))
+    chunk = QUAD_CHUNKS[int(rng.integers(0, len(QUAD_CHUNKS)))]
+    mask = G.periodic_mask(W, H) if periodic else G.channel_mask(W, H)
+    for _ in range(int(rng.integers(0, 6))):
+        h, w = int(rng.integers(1, 8)), int(rng.integers(1, 12))
+        y, x = int(rng.integers(1, H - h)), int(rng.integers(2, W - w - 2))
+        mask[y:y + h, x:x + w] = 1
+    if periodic:
+        ux, uy = G.random_phase_velocity(W, H, 4, 0.04, seed=seed)
+        kw = dict(bc="periodic", mask=mask, f0=G.feq(np.ones((H, W)), ux, uy))
+    else:
+        prof = G.parabolic_profile(H, 0.05)
+        kw = dict(bc="channel", mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, seed))
+    kw.update(cs=0.12)
+    a, b = Simulation(W, H, 1.75, **kw), Simulation(W, H, 1.75, **kw)
+    sel = _sel(a)
+    for launch in range(3):
+        a.step(4)
+        b.step(4, k=4, cfg=(0, chunk))
+        _assert_equal(a, b, sel, f"{W}x{H} periodic={periodic} rows {chunk} launch {launch}")
+
+
+@pytest.mark.parametrize("clamp", [True, False])
+def test_quad_kernel_clamp_forms(gpu, clamp):
+    from lbm.sim import Simulation
+    W, H = 300, 40
+    rng = np.random.default_rng(22)
+    lo, hi, um = (0.2, 1.9, 0.6) if clamp else (0.95, 1.05, 0.08)
+    rho = rng.uniform(lo, hi, (H, W))
+    ux, uy = rng.uniform(-um, um, (H, W)), rng.uniform(-um, um, (H, W))
+    mask = G.periodic_mask(W, H)
+    mask[15:20, 150:158] = 1
+    kw = dict(cs=0.15, model="mrt", bc="periodic", mask=mask, f0=G.feq(rho, ux, uy), clamp=clamp)
+    a, b = Simulation(W, H, 1.7, **kw), Simulation(W, H, 1.7, **kw)
+    sel = _sel(a)
+    for launch in range(3):
+        a.step(4)
+        b.step(4, k=4, cfg=(0, 8))
+        _assert_equal(a, b, sel, f"clamp {clamp} launch {launch}")
+
+
+def test_quad_kernel_macros_and_limits(gpu):
+    from lbm import _lib
+    single, fusedsim = _pair(200, 64, False, "mrt")
+    single.step(3)
+    single.step(1, macros=True)
+    fusedsim.step(4, k=4, cfg=(0, 12), macros=True)
+    upd = _sel(single)[0][0]
+    for a, b in ((single.rho, fusedsim.rho), (single.ux, fusedsim.ux), (single.uy, fusedsim.uy)):
+        assert torch.equal(a[upd], b[upd])
+    with pytest.raises(_lib.LbmError):
+        fusedsim.step(4, k=4, cfg=(0, 10))  # not in the four-step shape table
+    fusedsim.step(1)
+    with pytest.raises(_lib.LbmError):
+        fusedsim.step(4, k=4, cfg=(0, 8))  # odd parity
+    h16 = _pair(200, 64, False, "mrt")[0]
+    from lbm.sim import Simulation
+    s16 = Simulation(200, 64, 1.75, cs=0.12, mask=h16.mask_host, inlet_u=G.parabolic_profile(64, 0.05),
+                     storage="f16")
+    with pytest.raises(_lib.LbmError):
+        s16.step(4, k=4, cfg=(0, 8))  # FP32 populations only
+
+
+@pytest.mark.parametrize("chunk", [8, 24])
+def test_quad_kernel_cylinder_large(gpu, chunk):
+    """4096 x 1024 cylinder, the case's full 10000 steps with the four-step kernel (128 x 8 and 64 x 24 tiles):
+    bitwise equal to single steps every 250 launches, the first 10 launches and the last."""
+    from lbm.sim import Simulation
+    case = configs.cylinder_large()
+    a = Simulation(case.W, case.H, case.tau_inv, **case.sim_kwargs())
+    b = Simulation(case.W, case.H, case.tau_inv, **case.sim_kwargs())
+    sel = _sel(a)
+    n = case.steps // 4
+    for launch in range(n):
+        a.step(4)
+        b.step(4, k=4, cfg=(0, chunk))
+        if launch < 10 or launch % 250 == 249 or launch == n - 1:
+            _assert_equal(a, b, sel, f"launch {launch}")
+    assert np.isfinite(b.diagnostics()).all()
