@@ -42,6 +42,9 @@ TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-
 # lines is profiles/k_pair_cylinder_4096x1024.md.  Configurations not listed there report null.
 PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.93e6, ("bf16", 12): 156.97e6, ("f32", 12): 311.19e6}
 PAIR_VALU_PER_CELL = {("f16", 12): 363.8, ("bf16", 12): 363.4, ("f32", 12): 376.9}
+# The same for the four-step tile kernel (FP32, per tile shape); profiles/k_quad_cylinder_4096x1024.md.
+QUAD_TRAFFIC_PER_LAUNCH = {("f32", 24): 345.5e6}
+QUAD_VALU_PER_CELL = {("f32", 24): 799.8, ("f32", 8): 965.7}
 # VALU issue capacity of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each, 2.4 GHz
 VALU_WAVE_INSTS_PER_US = 256 * 4 / 4 * 2400.0
 
@@ -72,18 +75,25 @@ def roofline(name, form, cells, achieved_fp32):
         out.update(kernel=f"k_aa {storage} (single step)", traffic=TRAFFIC_BYTES_PER_STEP.get(storage),
                    traffic_source=TRAFFIC_SOURCE)
         return out
-    pair = form.get("strip") == 0 and k == 2
+    tile = form.get("strip") == 0 and k in (2, 4)
     key = (storage, form.get("chunk"))
-    out["kernel"] = (f"k_pair {storage} {form.get('chunk')}-row tiles (two steps per launch)" if pair else
-                     f"k_fused {storage} k={k} strip={form.get('strip')} chunk={form.get('chunk')}")
-    out["traffic"] = PAIR_TRAFFIC_PER_LAUNCH.get(key) if pair else None
+    ch = form.get("chunk")
+    if tile and k == 2:
+        out["kernel"] = f"k_pair {storage} 128x{ch} tiles (two steps per launch)"
+    elif tile:
+        out["kernel"] = f"k_quad {storage} {128 if ch <= 16 else 64}x{ch} tiles (four steps per launch)"
+    else:
+        out["kernel"] = f"k_fused {storage} k={k} strip={form.get('strip')} chunk={ch}"
+    traffic = (PAIR_TRAFFIC_PER_LAUNCH if k == 2 else QUAD_TRAFFIC_PER_LAUNCH) if tile else {}
+    valu = (PAIR_VALU_PER_CELL if k == 2 else QUAD_VALU_PER_CELL) if tile else {}
+    out["traffic"] = traffic.get(key)
     out["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-calibrated"
-    if pair and key in PAIR_VALU_PER_CELL:
-        floor = PAIR_VALU_PER_CELL[key] * cells / 64 / VALU_WAVE_INSTS_PER_US
-        out["valu"] = {"insts_per_cell_per_launch": PAIR_VALU_PER_CELL[key], "model_floor_us": round(floor, 2),
-                       "launch_us": round(launch_us, 2),
-                       "note": "issue-bound: launches run within a few per cent of this floor (4 cycles per wave64 "
-                               "VALU instruction, 1024 SIMDs, 2.4 GHz; an approximate model)"}
+    if key in valu:
+        floor = valu[key] * cells / 64 / VALU_WAVE_INSTS_PER_US
+        out["valu"] = {"insts_per_cell_per_launch": valu[key], "model_floor_us": round(floor, 2),
+                       "launch_us": round(launch_us, 2), "frac_of_issue_floor": round(floor / launch_us, 4),
+                       "note": "VALU issue floor: the launch's measured VALU wave instructions at one per SIMD per 4 "
+                               "cycles, 1024 SIMDs, 2.4 GHz (an approximate model)"}
     return out
 
 
@@ -493,6 +503,8 @@ def main():
     ap.add_argument("--k", type=int, default=8, help="largest steps per launch the tuner tries")
     ap.add_argument("--fused-cfg", default="", metavar="K,STRIP,CHUNK",
                     help="time the fused form at this configuration instead of autotuning it")
+    ap.add_argument("--fused-storages", default="f32,f16,bf16",
+                    help="storages whose fused form is tuned and timed (comma-separated)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--l2", choices=("full", "short", "off"), default="full",
                     help="L2 errors against FP64: 'full' = the cylinder at 1000/2000/5000/10000 steps and the "
@@ -692,7 +704,7 @@ def main():
     if L.lbm_fused_max_k() >= 2 and args.k >= 2 and args.only != "aa":
         from lbm import autotune
         ks = tuple(k for k in autotune.KS if k <= args.k)
-        for storage in ("f32", "f16", "bf16"):
+        for storage in [v.strip() for v in args.fused_storages.split(",") if v.strip()]:
             sf = sim if storage == "f32" else make_dev(storage)
             if sf.parity:  # the tuner offers the two-step tile kernel from even states only
                 sf.step(1)
@@ -708,7 +720,7 @@ def main():
             sf.step(max(1, fused_warmup // k) * k, k=k, cfg=cfg)
             us = time_steps(lambda n, sf=sf, k=k, cfg=cfg: sf.step(n, k=k, cfg=cfg), nlaunch * k)
             name = f"fused_{storage}"
-            kernel = "k_pair" if (strip == 0 and k == 2) else "k_fused"
+            kernel = ("k_pair" if k == 2 else "k_quad") if strip == 0 else "k_fused"
             forms[name] = {"glups": cells / us / 1e3, "us_per_step": us, "k": k, "strip": strip, "chunk": chunk,
                            "kernel": kernel, "storage": storage,
                            "tuned": {f"k{r[0]}": {"strip": r[1], "chunk": r[2], "us_per_step": round(r[3], 2)}
