@@ -246,7 +246,7 @@ def test_pair_kernel_macros_and_limits(gpu):
         fusedsim.step(2, k=2, cfg=(0, 8))
     fusedsim.step(1)
     with pytest.raises(_lib.LbmError):
-        fusedsim.step(4, k=4, cfg=(0, 8))
+        fusedsim.step(6, k=6, cfg=(0, 8))  # tile kernels exist for k = 2 and k = 4 only
     with pytest.raises(_lib.LbmError):
         fusedsim.step(2, k=2, cfg=(0, 10))
 
