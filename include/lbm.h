@@ -210,6 +210,18 @@ int lbm_diagnostics(const lbm_params *p, const uint8_t *mask, const float *f, co
                     double *work, double *out, lbm_stream_t stream);
 
 /*
+ * lbm_diagnostics without a workspace argument and without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under graph capture, tight serving or health-check loops).
+ * The partial sums live in a static device array of lbm_diagnostics_scratch_bytes() bytes (40 KiB) that each
+ * GPU holds until the process ends; its size does not depend on the lattice, so every lattice the library
+ * accepts fits.  All calls on one GPU share that array: calls that overlap on one GPU (different streams, or
+ * threads) race and must be serialised by the caller.  Results equal lbm_diagnostics' bit for bit.
+ */
+size_t lbm_diagnostics_scratch_bytes(void);
+int lbm_diagnostics_noalloc(const lbm_params *p, const uint8_t *mask, const float *f, const float *inlet_u,
+                            int parity, double *out, lbm_stream_t stream);
+
+/*
  * Batched ensembles (no reference counterpart: the reference advances one lattice per process,
  * lbm_smemaa_main.py:8-10).  An ensemble is a set of independent lattices of any sizes that share model,
  * bc and storage format; lbm_ensemble_step advances every member by one in-place AA step (the update of

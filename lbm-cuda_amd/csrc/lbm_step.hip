@@ -13,6 +13,7 @@
 #include "lbm_aa_cell.h"
 
 #include <cstring>
+#include <mutex>
 
 namespace lbm {
 
@@ -377,6 +378,45 @@ __global__ __launch_bounds__(DIAG_THREADS) void k_diag_final(const double *__res
 
 static dim3 step_grid(const KPar &P) { return dim3((P.W + TILE_X - 1) / TILE_X, (P.H + TILE_Y - 1) / TILE_Y); }
 
+// Scratch of lbm_diagnostics_noalloc: the per-workgroup FP64 partial sums of k_diag_partial's fixed grid
+// (DIAG_BLOCKS x DIAG_N doubles, 40 KiB), held by every GPU the library runs on until the process ends.  Host
+// code must not pass `scratch` itself to a kernel (the name is a host-side stand-in, not a device address):
+// scratch_ptr() looks up each device's own copy once, with hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[DIAG_BLOCKS * DIAG_N];
+constexpr size_t SCRATCH_BYTES = sizeof(double) * DIAG_BLOCKS * DIAG_N;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static double *g_scratch[SCRATCH_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch` (looked up on the device's first call), or NULL with the error set.
+static double *scratch_ptr(int &rc)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        rc = fail((int)e, "lbm_diagnostics_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        rc = fail(LBM_ERR_UNSUPPORTED, "lbm_diagnostics_noalloc: device %d beyond the %d supported", dev,
+                  SCRATCH_MAX_DEVICES);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            rc = fail(e != hipSuccess ? (int)e : LBM_ERR_ARG, "lbm_diagnostics_noalloc: hipGetSymbolAddress: %s",
+                      hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    rc = LBM_OK;
+    return g_scratch[dev];
+}
+
 } // namespace lbm
 
 using namespace lbm;
@@ -592,6 +632,23 @@ int lbm_diagnostics(const lbm_params *p, const uint8_t *mask, const float *f, co
     if (rc) return rc;
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(DIAG_THREADS), 0, s, work, out);
     return check_launch("lbm_diagnostics final");
+}
+
+size_t lbm_diagnostics_scratch_bytes(void) { return SCRATCH_BYTES; }
+
+int lbm_diagnostics_noalloc(const lbm_params *p, const uint8_t *mask, const float *f, const float *inlet_u,
+                            int parity, double *out, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P, 0); // any lattice the library accepts: the partials do not grow with it
+    if (rc) return rc;
+    if (lbm_diagnostics_work_size(p) > SCRATCH_BYTES)
+        return fail(LBM_ERR_SIZE, "lbm_diagnostics_noalloc: %zu bytes of partial sums exceed the %zu-byte scratch",
+                    lbm_diagnostics_work_size(p), SCRATCH_BYTES);
+    double *work = scratch_ptr(rc);
+    if (!work) return rc;
+    // every partial is overwritten by its workgroup in each call, so nothing needs clearing
+    return lbm_diagnostics(p, mask, f, inlet_u, parity, work, out, stream);
 }
 
 } // extern "C"

@@ -85,6 +85,8 @@ SIGNATURES = {
     "lbm_visualize": (_i, [_P, _vp, _vp, _vp, _vp, _f, _vp]),
     "lbm_diagnostics_work_size": (ctypes.c_size_t, [_P]),
     "lbm_diagnostics": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "lbm_diagnostics_scratch_bytes": (ctypes.c_size_t, []),
+    "lbm_diagnostics_noalloc": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp]),
     "lbm_ensemble_create": (_i, [_vp, _i, _i, _vp]),
     "lbm_ensemble_info": (_i, [_vp, _vp, _vp, _vp]),
     "lbm_ensemble_step": (_i, [_vp, _i, _i, _vp]),
