@@ -97,18 +97,22 @@ def roofline(name, form, cells, achieved_fp32):
     return out
 
 
-def bench_ensemble(nsteps, graph_steps=20):
-    """64 periodic lattices of uneven sizes: one persistent launch per step (k = 1) or per two steps (k = 2,
-    the two-step tile kernel), per cache group, replayed from a captured graph of `graph_steps` steps, vs the
-    same steps as 64 separate launches per step (captured step-major and lattice-major, and issued eagerly).
-    The tuner picks k, tile rows, grid and grouping; `gbs_at_72B` prices each step at the single step's
-    72 bytes per cell, whatever k."""
+def bench_ensemble(nsteps, graph_steps=24, l2_steps=1000):
+    """64 periodic lattices of uneven sizes (in size-class order): one persistent launch per step (k = 1) or per
+    two or four steps (k = 2, 4: the tile kernels), per cache group, replayed from a captured graph of
+    `graph_steps` steps, vs the same steps as 64 separate launches per step (captured step-major and
+    lattice-major, and issued eagerly).  Two tunings: one launch shape for the whole mix (tune_ensemble), and
+    one per size class (tune_classes: small / medium / large members, each range with its own k, tile rows, grid
+    and grouping).  `gbs_at_72B` prices each step at the single step's 72 bytes per cell, whatever k.  The FP16
+    ensemble is timed the same way and compared with the FP32 one after `l2_steps` steps (L2 of rho and u over
+    all members' cells)."""
+    import numpy as np
     import torch
 
     from lbm import configs
-    from lbm.ensemble import Ensemble, Graph, tune_ensemble
+    from lbm.ensemble import Ensemble, Graph, sort_by_class, tune_classes, tune_ensemble
 
-    members = configs.ensemble()
+    members = sort_by_class(configs.ensemble())
     t0 = time.perf_counter()
     ens = Ensemble.from_members(members, two_step=True)
     cells = ens.cells
@@ -139,12 +143,22 @@ def bench_ensemble(nsteps, graph_steps=20):
     def rate(us):
         return round(cells / us / 1e3, 3)
 
+    def plan_json(plan):
+        return [{"class": sg.name, "members": sg.m1 - sg.m0, "k": sg.k, "rows": sg.chunk or None,
+                 "blocks": sg.blocks, "group_mb": sg.group_bytes >> 20, "us_per_step": round(sg.us_per_step, 2)}
+                for sg in plan]
+
     g = ens.capture(graph_steps)
     us_p, _ = timed(lambda: ens.replay(g))
     ens.group_bytes = 0
     g1 = ens.capture(graph_steps)
     us_1, _ = timed(lambda: ens.replay(g1))
     ens.group_bytes = group
+    plan = tune_classes(ens, steps=graph_steps, reps=2)
+    gc = ens.capture(graph_steps)
+    us_c, _ = timed(lambda: ens.replay(gc))
+    log(f"ensemble per-class plan: {plan_json(plan)} -> {rate(us_c)} GLUPS")
+    ens.plan = None
     gs = Graph()
     with gs.capture():
         for _ in range(graph_steps):
@@ -157,17 +171,22 @@ def bench_ensemble(nsteps, graph_steps=20):
             m.step(graph_steps)
     us_l, _ = timed(gl.replay)
     us_e, wall_e = timed(lambda: ens.step_separate(graph_steps))
-    for gr in (g, g1, gs, gl):
+    for gr in (g, g1, gc, gs, gl):
         gr.close()
+    best_us = min(us_p, us_c)
     res = {
         "workload": f"ensemble{len(members)}_periodic_mrt_les",
         "members": len(members), "cells": cells, "steps_timed": reps * graph_steps,
         "steps_per_graph": graph_steps,
         "sizes": "W, H each from {256,384,512,768,1024,1536,2048}; nu log-uniform [1e-4, 1e-2]; seed 4",
-        "k": k, "tile_rows": rows if k == 2 else None, "blocks": blocks, "group_mb": group >> 20,
+        "glups": rate(best_us),
+        "k": k, "tile_rows": rows if k > 1 else None, "blocks": blocks, "group_mb": group >> 20,
         "tuner_best_per_k": {f"k{kk}": v for kk, v in sorted(best_by_k.items())},
         "persistent_graph": {"glups": rate(us_p), "us_per_step": round(us_p, 2),
                              "gbs_at_72B": round(BYTES_PER_CELL_FP32 * cells / us_p / 1e3, 1)},
+        "persistent_graph_per_class": {"glups": rate(us_c), "us_per_step": round(us_c, 2),
+                                       "gbs_at_72B": round(BYTES_PER_CELL_FP32 * cells / us_c / 1e3, 1),
+                                       "plan": plan_json(plan)},
         "persistent_graph_ungrouped": {"glups": rate(us_1), "us_per_step": round(us_1, 2),
                                        "gbs_at_72B": round(BYTES_PER_CELL_FP32 * cells / us_1 / 1e3, 1)},
         "separate_graph_step_major": {"glups": rate(us_s), "us_per_step": round(us_s, 2),
@@ -178,8 +197,8 @@ def bench_ensemble(nsteps, graph_steps=20):
                            "us_per_step_events": round(us_e, 2)},
     }
     best_sep = min(us_s, us_l)
-    res["speedup_vs_best_separate_graph"] = round(best_sep / us_p, 3)
-    res["speedup_vs_separate_eager"] = round(wall_e / us_p, 3)
+    res["speedup_vs_best_separate_graph"] = round(best_sep / best_us, 3)
+    res["speedup_vs_separate_eager"] = round(wall_e / best_us, 3)
     ens.close()
     del ens
     torch.cuda.empty_cache()
@@ -189,14 +208,50 @@ def bench_ensemble(nsteps, graph_steps=20):
     g = ens.capture(graph_steps)
     us16, _ = timed(lambda: ens.replay(g))
     g.close()
+    plan16 = tune_classes(ens, steps=graph_steps, reps=2)
+    g = ens.capture(graph_steps)
+    us16c, _ = timed(lambda: ens.replay(g))
+    g.close()
     res["persistent_graph_f16"] = {"glups": rate(us16), "us_per_step": round(us16, 2), "k": k16,
-                                   "tile_rows": rows16 if k16 == 2 else None, "blocks": b16,
+                                   "tile_rows": rows16 if k16 > 1 else None, "blocks": b16,
                                    "group_mb": group16 >> 20}
+    res["persistent_graph_f16_per_class"] = {"glups": rate(us16c), "us_per_step": round(us16c, 2),
+                                             "plan": plan_json(plan16)}
     ens.close()
     del ens
     torch.cuda.empty_cache()
+    # accuracy of the FP16 ensemble: FP16 and FP32 ensembles from the same seeded fields, l2_steps steps each
+    if l2_steps > 0:
+        t1 = time.perf_counter()
+        fields = {}
+        for st in ("f32", "f16"):
+            e = Ensemble.from_members(members, storage=st, two_step=True)
+            if l2_steps % 4 == 0:
+                e.chunk = 12 if st == "f16" else 24
+                e.step(l2_steps, k=2 if st == "f16" else 4)
+            else:
+                e.step(l2_steps, k=1)
+            fields[st] = [tuple(t.double().clone() for t in m.macro()) for m in e.members]
+            e.close()
+            del e
+            torch.cuda.empty_cache()
+        num_u = den_u = num_r = den_r = 0.0
+        worst = 0.0
+        for (r16, u16, v16), (r32, u32, v32) in zip(fields["f16"], fields["f32"]):
+            du = float(((u16 - u32) ** 2 + (v16 - v32) ** 2).sum())
+            nu = float((u32 ** 2 + v32 ** 2).sum())
+            num_u, den_u = num_u + du, den_u + nu
+            num_r += float(((r16 - r32) ** 2).sum())
+            den_r += float((r32 ** 2).sum())
+            worst = max(worst, float(np.sqrt(du / nu)))
+        res["l2_f16_vs_f32"] = {"steps": l2_steps, "u": float(np.sqrt(num_u / den_u)), "rho": float(np.sqrt(num_r / den_r)),
+                                "worst_member_u": worst, "forms": "FP16 two-step vs FP32 four-step ensembles",
+                                "seconds": round(time.perf_counter() - t1, 1)}
+        del fields
+        torch.cuda.empty_cache()
     log("ensemble GLUPS: " + ", ".join(f"{k} {v['glups']}" for k, v in res.items()
-                                       if isinstance(v, dict) and "glups" in v))
+                                       if isinstance(v, dict) and "glups" in v)
+        + f"; L2 FP16 vs FP32: {res.get('l2_f16_vs_f32')}")
     return res
 
 

@@ -264,6 +264,24 @@ int lbm_ensemble_pair_n(lbm_ensemble *e, int npairs, long long group_bytes, int 
 int lbm_ensemble_side(const lbm_ensemble *e);
 /* tiles of the two-step list with `chunk` rows, and its default grid (resident workgroups) */
 int lbm_ensemble_pair_info(lbm_ensemble *e, int chunk, int *total_tiles, int *default_blocks);
+/*
+ * Members [m0, m1) (in creation order) advanced nsteps steps with k steps per launch, so that each size class of
+ * an ensemble can run with its own kernel and tile shape:
+ *   k = 1  single steps in place from `parity` (chunk ignored);
+ *   k = 2  the two-step kernel (chunk = tile rows 8, 12, 16 or 24) from an even state, nsteps a multiple of 2;
+ *   k = 4  the four-step kernel of lbm_step_fused with strip 0 (chunk 8: 128 x 8 tiles, 20 or 24: 64-wide tiles;
+ *          FP32 populations) from an even state, nsteps a multiple of 4.
+ * The members are grouped for the cache as in lbm_ensemble_step_n (group_bytes) and every group runs all its
+ * launches before the next.  The range's members must hold their states on the same side (the buffer
+ * lbm_ensemble_member_side reports); k = 2 and 4 launches move them between f and f_alt.  Every member ends
+ * bit-identical to its solo run.  lbm_ensemble_step, _step_n and _pair_n are this on every member.
+ */
+int lbm_ensemble_run(lbm_ensemble *e, int m0, int m1, int k, int chunk, int nsteps, int parity,
+                     long long group_bytes, int blocks, lbm_stream_t stream);
+/* tiles of the k-step list with `chunk` rows (k = 1: the 128 x 2 list), and its default grid */
+int lbm_ensemble_run_info(lbm_ensemble *e, int k, int chunk, int *total_tiles, int *default_blocks);
+/* which buffer of member i holds its state (0: f, 1: f_alt); lbm_ensemble_side fails when they differ */
+int lbm_ensemble_member_side(const lbm_ensemble *e, int i);
 int lbm_ensemble_destroy(lbm_ensemble *e);
 
 /*

@@ -2,17 +2,22 @@
 // list spanning every member, and stream-capture graphs that replay any sequence of liblbm launches (the
 // reference driver's CuPy graph, lbm_smemaa_main.py:136-151).
 //
-// No reference counterpart: the reference runs one lattice per process (lbm_smemaa_main.py:8-10).  Two work
-// lists are kept per ensemble:
+// No reference counterpart: the reference runs one lattice per process (lbm_smemaa_main.py:8-10).  Work lists
+// kept per ensemble:
 //   k = 1  tiles of 128 x 2 cells, each updated by aa_cell (the single-step kernel's cell update), in place;
 //   k = 2  tiles of 128 x CH cells, each advanced two steps by pair_tile (the two-step tile kernel), from a
-//          member's current buffer into its second one.
+//          member's current buffer into its second one;
+//   k = 4  (FP32 populations) tiles of the four-step kernel's shapes (128 x 8, 64 x 20, 64 x 24), each advanced
+//          four steps by quad_tile, from one buffer into the other.
+// lbm_ensemble_run advances any contiguous range of members with any of these, so a tuner can give each size
+// class of an ensemble its own k and tile shape; each member's buffer side is tracked on its own.
 // Tile t belongs to the member whose first tile is the largest <= t.  Workgroups are dealt round-robin to the
 // 8 XCDs, so XCD k takes the k-th contiguous eighth of the list (neighbouring rows of a member share an L2), and
 // each workgroup strides through its XCD's share.  Every member ends bit-identical to its solo run.
 #include "lbm_aa_cell.h"
 #include "lbm_common.h"
 #include "lbm_pair_tile.h"
+#include "lbm_quad_tile.h"
 
 #include <vector>
 
@@ -20,6 +25,8 @@ namespace lbm {
 
 constexpr int ENS_NCH = 4;
 constexpr int ENS_CH[ENS_NCH] = {8, 12, 16, 24}; // tile rows of the two-step list
+constexpr int ENS_NQ = 3;
+constexpr int ENS_QCH[ENS_NQ] = {8, 20, 24}; // tile rows of the four-step lists (128 x 8, 64 x 20, 64 x 24)
 
 struct EnsDesc {
     KPar P;
@@ -30,6 +37,8 @@ struct EnsDesc {
     int tile0;              // first 128 x 2 tile of this member
     int ptiles_x;           // 128-column tiles of the two-step list
     int ptile0[ENS_NCH];    // first 128 x CH tile of this member, per CH
+    int qtiles_x[ENS_NQ];   // columns of tiles of the four-step lists
+    int qtile0[ENS_NQ];     // first four-step tile of this member, per shape
 };
 
 } // namespace lbm
@@ -37,12 +46,14 @@ struct EnsDesc {
 struct lbm_ensemble {
     int n = 0;
     int fmt = 0, model = 0, bc = 0;
-    int side = 0;           // which of each member's buffers holds the state
+    std::vector<int> side;  // per member: which of its buffers holds the state
     bool has_alt = false;   // every member has a second buffer
     int default_blocks = 0; // single-step workgroups resident at once on the whole device
     int pair_blocks[lbm::ENS_NCH] = {0, 0, 0, 0}; // the same for the two-step kernels (0 = not yet computed)
+    int quad_blocks[lbm::ENS_NQ] = {0, 0, 0};     // and for the four-step kernels
     int total_tiles = 0;
     int total_ptiles[lbm::ENS_NCH] = {0, 0, 0, 0};
+    int total_qtiles[lbm::ENS_NQ] = {0, 0, 0};
     long long total_cells = 0;
     lbm::EnsDesc *d_desc = nullptr;
     std::vector<lbm::EnsDesc> h_desc;
@@ -111,8 +122,50 @@ __global__ __launch_bounds__(pairk::NT) void k_ensemble_pair(const EnsDesc *__re
     }
 }
 
+// Four steps of members [m0, m1) whose tiles are [t0, t1) of the four-step list qi (tile rows ENS_QCH[qi] = CH),
+// from buffer `side` into buffer `side ^ 1`.
+template <int MODEL, bool PERIODIC, int CH>
+__global__ __launch_bounds__(quadk::Shape<CH>::NT) void k_ensemble_quad(const EnsDesc *__restrict__ d, int m0, int m1,
+                                                                        int t0, int t1, int side, int qi)
+{
+    constexpr int CW = quadk::Shape<CH>::CW, NT = quadk::Shape<CH>::NT;
+    __shared__ quadk::QuadLds<FMT_F32, CW, CH, NT> L;
+    const int G = gridDim.x, b = blockIdx.x;
+    const int gper = G / 8;
+    const int xcd = b % 8, l = b / 8;
+    const int per = (t1 - t0 + 7) / 8;
+    const int start = t0 + xcd * per, end = min(start + per, t1);
+    for (int t = start + l; t < end; t += gper) {
+        const EnsDesc &e = d[ens_member(d, m0, m1, t, [qi](const EnsDesc &x) { return x.qtile0[qi]; })];
+        const int lt = t - e.qtile0[qi];
+        const int ty = lt / e.qtiles_x[qi], tx = lt - ty * e.qtiles_x[qi];
+        quadk::quad_tile_any<MODEL, PERIODIC, FMT_F32, CW, CH, NT>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u,
+                                                                    nullptr, nullptr, nullptr, tx, ty, L);
+        __syncthreads(); // the next tile's first phase overwrites the LDS this tile's last phase read
+    }
+}
+
 using EnsKernel = void (*)(const EnsDesc *, int, int, int, int, int);
 using PairKernel = void (*)(const EnsDesc *, int, int, int, int, int, int);
+
+template <int MODEL, bool PERIODIC>
+static PairKernel quad_kernel_c(int qi)
+{
+    switch (qi) {
+    case 0: return k_ensemble_quad<MODEL, PERIODIC, 8>;
+    case 1: return k_ensemble_quad<MODEL, PERIODIC, 20>;
+    default: return k_ensemble_quad<MODEL, PERIODIC, 24>;
+    }
+}
+
+static PairKernel quad_kernel(const lbm_ensemble *e, int qi)
+{
+    const bool periodic = e->bc == LBM_BC_PERIODIC;
+    if (e->model == LBM_BGK) return periodic ? quad_kernel_c<MODEL_BGK, true>(qi) : quad_kernel_c<MODEL_BGK, false>(qi);
+    return periodic ? quad_kernel_c<MODEL_MRT, true>(qi) : quad_kernel_c<MODEL_MRT, false>(qi);
+}
+
+static int quad_threads(int qi) { return ENS_QCH[qi] == 8 ? quadk::Shape<8>::NT : quadk::Shape<24>::NT; }
 
 template <int MODEL, int FMT>
 static EnsKernel ens_kernel_m(int parity, bool periodic)
@@ -185,13 +238,13 @@ static int ens_blocks(int blocks, int dflt)
 
 // The next cache group after member m0: consecutive members whose states total at most group_bytes (at
 // least one member); group_bytes <= 0 puts every member in one group.
-static int next_group(const lbm_ensemble *e, int m0, long long group_bytes)
+static int next_group(const lbm_ensemble *e, int m0, long long group_bytes, int mend)
 {
-    if (group_bytes <= 0) return e->n;
+    if (group_bytes <= 0) return mend;
     const long long elem = e->fmt == LBM_FMT_F32 ? 4 : 2;
     long long bytes = 9LL * elem * e->h_desc[m0].P.plane;
     int m1 = m0 + 1;
-    while (m1 < e->n) {
+    while (m1 < mend) {
         const long long b = 9LL * elem * e->h_desc[m1].P.plane;
         if (bytes + b > group_bytes) break;
         bytes += b;
@@ -219,7 +272,8 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
     e->bc = inst[0].p.bc;
     e->has_alt = true;
     e->h_desc.resize(n);
-    long long tiles = 0, ptiles[ENS_NCH] = {0, 0, 0, 0};
+    e->side.assign(n, 0);
+    long long tiles = 0, ptiles[ENS_NCH] = {0, 0, 0, 0}, qtiles[ENS_NQ] = {0, 0, 0};
     for (int i = 0; i < n; ++i) {
         EnsDesc &d = e->h_desc[i];
         int rc = make_kpar(&inst[i].p, &d.P, fmt == LBM_FMT_F32 ? 4u : 2u);
@@ -256,14 +310,21 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
             d.ptile0[c] = (int)ptiles[c];
             ptiles[c] += (long long)d.ptiles_x * ((d.P.H + ENS_CH[c] - 1) / ENS_CH[c]);
         }
+        for (int c = 0; c < ENS_NQ; ++c) {
+            const int cw = ENS_QCH[c] <= 16 ? 128 : 64;
+            d.qtiles_x[c] = (d.P.W + cw - 1) / cw;
+            d.qtile0[c] = (int)qtiles[c];
+            qtiles[c] += (long long)d.qtiles_x[c] * ((d.P.H + ENS_QCH[c] - 1) / ENS_QCH[c]);
+        }
         e->total_cells += (long long)d.P.plane;
     }
-    if (tiles >= (1LL << 31)) {
+    if (tiles >= (1LL << 31) || qtiles[0] >= (1LL << 31) || ptiles[0] >= (1LL << 31)) {
         delete e;
         return fail(LBM_ERR_SIZE, "lbm_ensemble_create: too many tiles");
     }
     e->total_tiles = (int)tiles;
     for (int c = 0; c < ENS_NCH; ++c) e->total_ptiles[c] = (int)ptiles[c];
+    for (int c = 0; c < ENS_NQ; ++c) e->total_qtiles[c] = (int)qtiles[c];
     // A persistent grid must be resident at once: a workgroup that starts only after others finish would
     // run its whole share of tiles at the end.  Size it by the occupancy of the slower parity's kernel.
     int occ_blocks = 1 << 30;
@@ -303,6 +364,13 @@ static int pair_chi(int chunk)
     return -1;
 }
 
+static int quad_qi(int chunk)
+{
+    for (int c = 0; c < ENS_NQ; ++c)
+        if (ENS_QCH[c] == chunk) return c;
+    return -1;
+}
+
 // resident two-step workgroups for tile rows ENS_CH[chi] (computed on first use)
 static int pair_default(lbm_ensemble *e, int chi, const char *who)
 {
@@ -314,93 +382,156 @@ static int pair_default(lbm_ensemble *e, int chi, const char *who)
     return LBM_OK;
 }
 
+// the same for the four-step kernels
+static int quad_default(lbm_ensemble *e, int qi, const char *who)
+{
+    if (!e->quad_blocks[qi]) {
+        hipError_t he = resident_blocks(reinterpret_cast<const void *>(quad_kernel(e, qi)), quad_threads(qi),
+                                        e->quad_blocks[qi]);
+        if (he != hipSuccess) return fail((int)he, "%s: %s", who, hipGetErrorString(he));
+    }
+    return LBM_OK;
+}
+
 int lbm_ensemble_pair_info(lbm_ensemble *e, int chunk, int *total_tiles, int *default_blocks)
 {
-    if (!e) return fail(LBM_ERR_ARG, "lbm_ensemble_pair_info: NULL ensemble");
-    const int chi = pair_chi(chunk);
-    if (chi < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_pair_info: chunk must be 8, 12, 16 or 24");
-    int rc = pair_default(e, chi, "lbm_ensemble_pair_info");
-    if (rc) return rc;
-    if (total_tiles) *total_tiles = e->total_ptiles[chi];
-    if (default_blocks) *default_blocks = e->pair_blocks[chi];
+    return lbm_ensemble_run_info(e, 2, chunk, total_tiles, default_blocks);
+}
+
+int lbm_ensemble_run_info(lbm_ensemble *e, int k, int chunk, int *total_tiles, int *default_blocks)
+{
+    if (!e) return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: NULL ensemble");
+    int tiles = 0, blocks = 0;
+    if (k == 1) {
+        tiles = e->total_tiles;
+        blocks = e->default_blocks;
+    } else if (k == 2) {
+        const int chi = pair_chi(chunk);
+        if (chi < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: k = 2 needs chunk 8, 12, 16 or 24");
+        int rc = pair_default(e, chi, "lbm_ensemble_run_info");
+        if (rc) return rc;
+        tiles = e->total_ptiles[chi];
+        blocks = e->pair_blocks[chi];
+    } else if (k == 4) {
+        const int qi = quad_qi(chunk);
+        if (qi < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: k = 4 needs chunk 8, 20 or 24");
+        if (e->fmt != LBM_FMT_F32)
+            return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_run_info: k = 4 takes FP32 populations only");
+        int rc = quad_default(e, qi, "lbm_ensemble_run_info");
+        if (rc) return rc;
+        tiles = e->total_qtiles[qi];
+        blocks = e->quad_blocks[qi];
+    } else {
+        return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: k must be 1, 2 or 4");
+    }
+    if (total_tiles) *total_tiles = tiles;
+    if (default_blocks) *default_blocks = blocks;
     return LBM_OK;
 }
 
 int lbm_ensemble_side(const lbm_ensemble *e)
 {
     if (!e) return fail(LBM_ERR_ARG, "lbm_ensemble_side: NULL ensemble");
-    return e->side;
+    for (int i = 1; i < e->n; ++i)
+        if (e->side[i] != e->side[0])
+            return fail(LBM_ERR_ARG, "lbm_ensemble_side: members hold their states on different sides "
+                                     "(lbm_ensemble_member_side)");
+    return e->side[0];
 }
 
-static int ens_launch(const lbm_ensemble *e, int parity, int blocks, int m0, int m1, hipStream_t s)
+int lbm_ensemble_member_side(const lbm_ensemble *e, int i)
+{
+    if (!e || i < 0 || i >= e->n) return fail(LBM_ERR_ARG, "lbm_ensemble_member_side: no member %d", i);
+    return e->side[i];
+}
+
+// one launch of the k-step kernel (list `li`: chunk index) over members [m0, m1), tiles [t0, t1), from `side`
+static int ens_launch_k(const lbm_ensemble *e, int k, int li, int parity, int blocks, int m0, int m1, int side,
+                        hipStream_t s)
 {
     const EnsDesc *d = e->d_desc;
-    int t0 = e->h_desc[m0].tile0;
-    int t1 = m1 < e->n ? e->h_desc[m1].tile0 : e->total_tiles;
-    int side = e->side;
-    void *args[] = {&d, &m0, &m1, &t0, &t1, &side};
-    hipError_t he = hipLaunchKernel(reinterpret_cast<const void *>(ens_kernel(e, parity)), dim3(blocks),
-                                    dim3(TILE_X, TILE_Y), args, 0, s);
-    if (he != hipSuccess) return fail((int)he, "lbm_ensemble_step: %s", hipGetErrorString(he));
+    hipError_t he;
+    if (k == 1) {
+        int t0 = e->h_desc[m0].tile0;
+        int t1 = m1 < e->n ? e->h_desc[m1].tile0 : e->total_tiles;
+        void *args[] = {&d, &m0, &m1, &t0, &t1, &side};
+        he = hipLaunchKernel(reinterpret_cast<const void *>(ens_kernel(e, parity)), dim3(blocks),
+                             dim3(TILE_X, TILE_Y), args, 0, s);
+    } else if (k == 2) {
+        int t0 = e->h_desc[m0].ptile0[li];
+        int t1 = m1 < e->n ? e->h_desc[m1].ptile0[li] : e->total_ptiles[li];
+        void *args[] = {&d, &m0, &m1, &t0, &t1, &side, &li};
+        he = hipLaunchKernel(reinterpret_cast<const void *>(pair_kernel(e, li)), dim3(blocks), dim3(pairk::NT), args, 0,
+                             s);
+    } else {
+        int t0 = e->h_desc[m0].qtile0[li];
+        int t1 = m1 < e->n ? e->h_desc[m1].qtile0[li] : e->total_qtiles[li];
+        void *args[] = {&d, &m0, &m1, &t0, &t1, &side, &li};
+        he = hipLaunchKernel(reinterpret_cast<const void *>(quad_kernel(e, li)), dim3(blocks), dim3(quad_threads(li)),
+                             args, 0, s);
+    }
+    if (he != hipSuccess) return fail((int)he, "lbm_ensemble_run: %s", hipGetErrorString(he));
     return LBM_OK;
+}
+
+int lbm_ensemble_run(lbm_ensemble *e, int m0, int m1, int k, int chunk, int nsteps, int parity,
+                     long long group_bytes, int blocks, lbm_stream_t stream)
+{
+    if (!e) return fail(LBM_ERR_ARG, "lbm_ensemble_run: NULL ensemble");
+    if (m0 < 0 || m1 > e->n || m0 >= m1) return fail(LBM_ERR_ARG, "lbm_ensemble_run: members [%d, %d) of %d", m0, m1, e->n);
+    if (nsteps < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_run: nsteps < 0");
+    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_ensemble_run: parity must be 0 or 1");
+    for (int i = m0 + 1; i < m1; ++i)
+        if (e->side[i] != e->side[m0])
+            return fail(LBM_ERR_ARG, "lbm_ensemble_run: members %d and %d hold their states on different sides", m0, i);
+    int li = 0, dflt = e->default_blocks;
+    if (k == 2 || k == 4) {
+        if (!e->has_alt) return fail(LBM_ERR_ARG, "lbm_ensemble_run: k = %d needs a second buffer (f_alt) per member", k);
+        if (parity != 0) return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_run: k = %d starts from an even state", k);
+        if (nsteps % k) return fail(LBM_ERR_ARG, "lbm_ensemble_run: nsteps %d is not a multiple of k = %d", nsteps, k);
+        int rc = lbm_ensemble_run_info(e, k, chunk, nullptr, &dflt);
+        if (rc) return rc;
+        li = k == 2 ? pair_chi(chunk) : quad_qi(chunk);
+    } else if (k != 1) {
+        return fail(LBM_ERR_ARG, "lbm_ensemble_run: k must be 1, 2 or 4");
+    }
+    blocks = ens_blocks(blocks, dflt);
+    hipStream_t s = to_hip(stream);
+    const int nl = k == 1 ? nsteps : nsteps / k;
+    const int side0 = e->side[m0];
+    for (int g0 = m0; g0 < m1;) {
+        const int g1 = next_group(e, g0, group_bytes, m1);
+        for (int n = 0; n < nl; ++n) {
+            const int side = k == 1 ? side0 : side0 ^ (n & 1);
+            int rc = ens_launch_k(e, k, li, (parity + n) & 1, blocks, g0, g1, side, s);
+            if (rc) return rc;
+        }
+        g0 = g1;
+    }
+    if (k != 1)
+        for (int i = m0; i < m1; ++i) e->side[i] ^= nl & 1;
+    return check_launch("lbm_ensemble_run");
 }
 
 int lbm_ensemble_step(lbm_ensemble *e, int parity, int blocks, lbm_stream_t stream)
 {
     if (!e) return fail(LBM_ERR_ARG, "lbm_ensemble_step: NULL ensemble");
-    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_ensemble_step: parity must be 0 or 1");
-    int rc = ens_launch(e, parity, ens_blocks(blocks, e->default_blocks), 0, e->n, to_hip(stream));
-    return rc ? rc : check_launch("lbm_ensemble_step");
+    return lbm_ensemble_run(e, 0, e->n, 1, 0, 1, parity, 0, blocks, stream);
 }
 
 int lbm_ensemble_step_n(lbm_ensemble *e, int parity, int nsteps, long long group_bytes, int blocks,
                         lbm_stream_t stream)
 {
     if (!e) return fail(LBM_ERR_ARG, "lbm_ensemble_step_n: NULL ensemble");
-    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_ensemble_step_n: parity must be 0 or 1");
-    if (nsteps < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_step_n: nsteps < 0");
-    blocks = ens_blocks(blocks, e->default_blocks);
-    hipStream_t s = to_hip(stream);
-    for (int m0 = 0; m0 < e->n;) {
-        const int m1 = next_group(e, m0, group_bytes);
-        for (int n = 0; n < nsteps; ++n) {
-            int rc = ens_launch(e, (parity + n) & 1, blocks, m0, m1, s);
-            if (rc) return rc;
-        }
-        m0 = m1;
-    }
-    return check_launch("lbm_ensemble_step_n");
+    return lbm_ensemble_run(e, 0, e->n, 1, 0, nsteps, parity, group_bytes, blocks, stream);
 }
 
 int lbm_ensemble_pair_n(lbm_ensemble *e, int npairs, long long group_bytes, int chunk, int blocks,
                         lbm_stream_t stream)
 {
     if (!e) return fail(LBM_ERR_ARG, "lbm_ensemble_pair_n: NULL ensemble");
-    if (!e->has_alt) return fail(LBM_ERR_ARG, "lbm_ensemble_pair_n: every member needs a second buffer (f_alt)");
     if (npairs < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_pair_n: npairs < 0");
-    const int chi = pair_chi(chunk);
-    if (chi < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_pair_n: chunk must be 8, 12, 16 or 24");
-    const void *kern = reinterpret_cast<const void *>(pair_kernel(e, chi));
-    int rc = pair_default(e, chi, "lbm_ensemble_pair_n");
-    if (rc) return rc;
-    blocks = ens_blocks(blocks, e->pair_blocks[chi]);
-    hipStream_t s = to_hip(stream);
-    const EnsDesc *d = e->d_desc;
-    for (int m0 = 0; m0 < e->n;) {
-        int m1 = next_group(e, m0, group_bytes);
-        int t0 = e->h_desc[m0].ptile0[chi];
-        int t1 = m1 < e->n ? e->h_desc[m1].ptile0[chi] : e->total_ptiles[chi];
-        for (int n = 0; n < npairs; ++n) {
-            int side = e->side ^ (n & 1);
-            int a0 = m0, c = chi;
-            void *args[] = {&d, &a0, &m1, &t0, &t1, &side, &c};
-            hipError_t he = hipLaunchKernel(kern, dim3(blocks), dim3(pairk::NT), args, 0, s);
-            if (he != hipSuccess) return fail((int)he, "lbm_ensemble_pair_n: %s", hipGetErrorString(he));
-        }
-        m0 = m1;
-    }
-    e->side ^= npairs & 1;
-    return check_launch("lbm_ensemble_pair_n");
+    return lbm_ensemble_run(e, 0, e->n, 2, chunk, 2 * npairs, 0, group_bytes, blocks, stream);
 }
 
 int lbm_ensemble_destroy(lbm_ensemble *e)

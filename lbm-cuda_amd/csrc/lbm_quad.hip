@@ -16,11 +16,6 @@
 namespace lbm {
 namespace quadk {
 
-template <int CH> struct Shape {
-    static constexpr int CW = CH <= 16 ? 128 : 64;
-    static constexpr int NT = (CH == 8 || CH == 20 || CH == 24) ? 512 : 1024;
-};
-
 template <int MODEL, bool PERIODIC, int FMT, int CH>
 __global__ __launch_bounds__(Shape<CH>::NT) void k_quad(KPar P, const uint8_t *__restrict__ mask,
                                                         const void *__restrict__ fin, void *__restrict__ fout,

@@ -28,6 +28,13 @@ namespace quadk {
 
 constexpr int R = 3; // halo rings
 
+// Tile shapes by core rows: 8, 12 and 16 rows of 128-wide tiles, 20, 24 and 32 rows of 64-wide ones; 512-thread
+// workgroups where two fit a CU's LDS with FP32 populations (8, 20, 24 rows), 1024 threads otherwise.
+template <int CH> struct Shape {
+    static constexpr int CW = CH <= 16 ? 128 : 64;
+    static constexpr int NT = (CH == 8 || CH == 20 || CH == 24) ? 512 : 1024;
+};
+
 // The LDS image of a CW x CH tile's region, (CW + 6) x (CH + 6) cells.  The row stride is the region width, so
 // the 64 consecutive cells a wave handles are 64 consecutive LDS words even when they span two rows.
 template <int FMT, int CW, int CH, int NT> struct QuadLds {

@@ -95,6 +95,9 @@ SIGNATURES = {
     "lbm_ensemble_pair_n": (_i, [_vp, _i, ctypes.c_longlong, _i, _i, _vp]),
     "lbm_ensemble_side": (_i, [_vp]),
     "lbm_ensemble_pair_info": (_i, [_vp, _i, _vp, _vp]),
+    "lbm_ensemble_run": (_i, [_vp, _i, _i, _i, _i, _i, _i, ctypes.c_longlong, _i, _vp]),
+    "lbm_ensemble_run_info": (_i, [_vp, _i, _i, _vp, _vp]),
+    "lbm_ensemble_member_side": (_i, [_vp, _i]),
     "lbm_selftest": (_i, [_i, _vp, _vp, _vp]),
     "lbm_step_fused_n": (_i, [_P, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
     "lbm_step_fused_h16_n": (_i, [_P, _vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp, _vp]),

@@ -6,16 +6,19 @@ storage format and parity), and ``lbm_ensemble_step`` advances all of them by on
 launch whose workgroups walk a tile list spanning every member; ``lbm_ensemble_step_n`` advances groups of
 members that fit in the Infinity Cache several steps at a time (members are independent, so the order in
 which groups advance changes nothing).  With ``two_step=True`` every member also holds a second buffer and
-``step(n, k=2)`` advances the ensemble two steps per launch with the two-step tile kernel
-(``lbm_ensemble_pair_n``), moving each member's state between its two buffers.  ``Graph`` records any
-sequence of liblbm launches on a side stream (``capture``) and replays it; ``Ensemble.capture(n, k)`` records
-n ensemble steps.  Every member's state is bit-identical to what its own ``Simulation.step`` would produce.
+``step(n, k=2)`` advances the ensemble two steps per launch with the two-step tile kernel, ``k=4`` four steps
+with the four-step one (FP32 populations), moving each member's state between its two buffers.  A plan
+(``tune_classes``) gives each size class -- a contiguous range of members, ``sort_by_class`` -- its own k, tile
+rows, grid and cache grouping (``lbm_ensemble_run``).  ``Graph`` records any sequence of liblbm launches on a
+side stream (``capture``) and replays it; ``Ensemble.capture(n, k)`` records n ensemble steps.  Every member's
+state is bit-identical to what its own ``Simulation.step`` would produce.
 """
 from __future__ import annotations
 
 import ctypes
 import json
 import os
+from dataclasses import dataclass
 
 import torch
 
@@ -28,6 +31,38 @@ class LbmInstance(ctypes.Structure):
 
     _fields_ = [("p", _lib.LbmParams), ("mask", ctypes.c_void_p), ("f", ctypes.c_void_p),
                 ("inlet_u", ctypes.c_void_p), ("f_alt", ctypes.c_void_p)]
+
+
+# Size classes of ensemble members, by cell count: the tuner gives each its own launch shape.  Small members
+# (fewer than 2^19 cells) have few interior tiles, large ones (2^21 and more) fill the GPU on their own.
+CLASS_BOUNDS = (1 << 19, 1 << 21)
+CLASS_NAMES = ("small", "medium", "large")
+
+
+def size_class(W, H):
+    """Size class name of a W x H member (CLASS_BOUNDS on its cell count)."""
+    cells = int(W) * int(H)
+    return CLASS_NAMES[sum(cells >= b for b in CLASS_BOUNDS)]
+
+
+def sort_by_class(members):
+    """Members (anything with W and H) in size-class order, stable within a class, so that each class is a
+    contiguous range of the ensemble."""
+    return sorted(members, key=lambda m: CLASS_NAMES.index(size_class(m.W, m.H)))
+
+
+@dataclass
+class Segment:
+    """One planned range of an ensemble: members [m0, m1) advanced with k steps per launch, `chunk` tile rows,
+    `blocks` persistent workgroups (0 = the kernel's resident grid) and cache groups of `group_bytes`."""
+    name: str
+    m0: int
+    m1: int
+    k: int = 1
+    chunk: int = 0
+    blocks: int = 0
+    group_bytes: int = 0
+    us_per_step: float = 0.0
 
 
 class Graph:
@@ -91,8 +126,13 @@ class Ensemble:
     single-step path updates them in place, so ``Simulation.step(k=1)`` keeps them valid too).
 
     ``two_step=True`` gives every member a second buffer (``Simulation.f_alt``, allocated if missing) for
-    ``step(n, k=2)``; the ensemble then swaps each member's ``f`` and ``f_alt`` so that ``f`` always holds
-    the state.  ``k`` and ``chunk`` (tile rows, 8/12/16/24) are the defaults of ``step``."""
+    ``step(n, k=2)`` and ``step(n, k=4)``; the ensemble then swaps each member's ``f`` and ``f_alt`` so that
+    ``f`` always holds the state.  ``k`` and ``chunk`` (tile rows: 8/12/16/24 for k = 2; 8/20/24 for k = 4, FP32
+    populations) are the defaults of ``step``.
+
+    ``plan`` (set by ``tune_classes``) gives contiguous member ranges -- the size classes, see
+    ``sort_by_class`` -- their own launch shape: a list of ``Segment``; ``step(n)`` without an explicit k then
+    advances each range with its own k, tile rows, grid and cache grouping (lbm_ensemble_run)."""
 
     def __init__(self, members, blocks=0, group_bytes=0, two_step=False, k=1, chunk=16):
         self.lib = _lib.load()
@@ -109,6 +149,7 @@ class Ensemble:
         self.group_bytes = int(group_bytes)
         self.two_step = bool(two_step)
         self.k, self.chunk = int(k), int(chunk)
+        self.plan = None
         if self.two_step:
             for m in self.members:
                 if m.f_alt is None:
@@ -119,7 +160,7 @@ class Ensemble:
                                          m.f_alt.data_ptr() if self.two_step else None)
         # the buffers as registered: buffer 0 holds the state at side 0, buffer 1 at side 1
         self._bufs = [(m.f.data_ptr(), m.f_alt.data_ptr() if self.two_step else None) for m in self.members]
-        self.side = 0
+        self.sides = [0] * len(self.members)
         self.handle = ctypes.c_void_p()
         _lib.check(self.lib.lbm_ensemble_create(self._table, len(self.members), self.fmt, ctypes.byref(self.handle)),
                    "lbm_ensemble_create")
@@ -128,13 +169,22 @@ class Ensemble:
                                               ctypes.byref(blk)), "lbm_ensemble_info")
         self.total_tiles, self.cells, self.default_blocks = tiles.value, cells.value, blk.value
 
+    @property
+    def side(self):
+        """The buffer holding every member's state (0: f, 1: f_alt), or -1 when members differ."""
+        return self.sides[0] if all(v == self.sides[0] for v in self.sides) else -1
+
     @classmethod
-    def from_members(cls, members, storage="f32", device="cuda", blocks=0, workers=8, two_step=False):
-        """Build the Simulations of ``configs.ensemble`` members (periodic, MRT + LES) and group them.  The
-        initial states are computed on `workers` host threads, at most `workers` of them held at once."""
+    def from_members(cls, members, storage="f32", device="cuda", blocks=0, workers=8, two_step=False,
+                     by_class=False):
+        """Build the Simulations of ``configs.ensemble`` members (periodic, MRT + LES) and group them (in size-class
+        order with ``by_class``, see ``sort_by_class``).  The initial states are computed on `workers` host
+        threads, at most `workers` of them held at once."""
         import concurrent.futures as cf
         import numpy as np
         members = list(members)
+        if by_class:
+            members = sort_by_class(members)
         sims = []
         with cf.ThreadPoolExecutor(max_workers=workers) as ex:
             for i in range(0, len(members), workers):
@@ -144,49 +194,62 @@ class Ensemble:
                                            device=device, storage=storage))
         return cls(sims, blocks=blocks, two_step=two_step)
 
-    def pair_blocks(self, chunk):
-        """Workgroups of the two-step kernel with `chunk` tile rows resident at once (its default grid)."""
+    def pair_blocks(self, chunk, k=2):
+        """Workgroups of the k-step kernel (k = 2 or 4) with `chunk` tile rows resident at once (its default
+        grid)."""
         tiles, blk = ctypes.c_int(), ctypes.c_int()
-        _lib.check(self.lib.lbm_ensemble_pair_info(self.handle, int(chunk), ctypes.byref(tiles), ctypes.byref(blk)),
-                   "lbm_ensemble_pair_info")
+        _lib.check(self.lib.lbm_ensemble_run_info(self.handle, int(k), int(chunk), ctypes.byref(tiles),
+                                                  ctypes.byref(blk)), "lbm_ensemble_run_info")
         return blk.value
 
     def _check_buffers(self):
-        for m, b in zip(self.members, self._bufs):
-            if m.f.data_ptr() != b[self.side] or (self.two_step and m.f_alt.data_ptr() != b[self.side ^ 1]):
+        for m, b, sd in zip(self.members, self._bufs, self.sides):
+            if m.f.data_ptr() != b[sd] or (self.two_step and m.f_alt.data_ptr() != b[sd ^ 1]):
                 raise RuntimeError("a member's state buffer moved (its own fused steps swap buffers): "
                                    "rebuild the ensemble")
 
-    def _check_k(self, n, k):
-        if k not in (1, 2):
-            raise ValueError("ensemble steps take k = 1 or 2")
-        if k == 2:
-            if not self.two_step:
-                raise ValueError("k = 2 needs an ensemble built with two_step=True")
-            if self.parity != 0 or n % 2:
-                raise ValueError("k = 2 advances an even number of steps from an even state")
+    def _segments(self, k):
+        """The (m0, m1, k, chunk, blocks, group_bytes) launches of one step call: the plan when no k is forced,
+        else every member with k and the ensemble's defaults."""
+        if k is None and self.plan is not None:
+            return [(sg.m0, sg.m1, sg.k, sg.chunk, sg.blocks, sg.group_bytes) for sg in self.plan]
+        k = self.k if k is None else int(k)
+        return [(0, len(self.members), k, self.chunk, self.blocks, self.group_bytes)]
 
-    def _launch(self, parity, n, stream, k=1):
-        if k == 1:
-            _lib.check(self.lib.lbm_ensemble_step_n(self.handle, parity, n, self.group_bytes, self.blocks, stream),
-                       "lbm_ensemble_step_n")
-            return
-        _lib.check(self.lib.lbm_ensemble_pair_n(self.handle, n // 2, self.group_bytes, self.chunk, self.blocks,
-                                                stream), "lbm_ensemble_pair_n")
-        side = self.lib.lbm_ensemble_side(self.handle)
-        if side != self.side:
-            for m in self.members:
+    def _check_k(self, n, k):
+        if k not in (1, 2, 4):
+            raise ValueError("ensemble steps take k = 1, 2 or 4")
+        if k > 1:
+            if not self.two_step:
+                raise ValueError(f"k = {k} needs an ensemble built with two_step=True")
+            if self.parity != 0 or n % k:
+                raise ValueError(f"k = {k} advances a multiple of {k} steps from an even state")
+            if k == 4 and self.fmt != _lib.FMT_F32:
+                raise ValueError("k = 4 takes FP32 populations only")
+
+    def _run(self, m0, m1, k, chunk, n, blocks, group_bytes, stream):
+        _lib.check(self.lib.lbm_ensemble_run(self.handle, m0, m1, k, chunk if k > 1 else 0, n, self.parity,
+                                             group_bytes, blocks, stream), "lbm_ensemble_run")
+        if k > 1 and (n // k) & 1:
+            for i in range(m0, m1):
+                m = self.members[i]
                 m.f, m.f_alt = m.f_alt, m.f
-            self.side = side
+                self.sides[i] ^= 1
+
+    def _launch(self, parity, n, stream, k=None):
+        assert parity == self.parity
+        for m0, m1, kk, ch, blocks, gb in self._segments(k):
+            self._run(m0, m1, kk, ch, n, blocks, gb, stream)
 
     def step(self, n=1, k=None):
         """n AA steps of every member: members grouped by ``group_bytes`` (0 = all in one group), each
-        group advanced n steps (one persistent launch per step, or per two steps with k = 2) before the
-        next."""
+        group advanced n steps (one persistent launch per step, or per k steps with k = 2 or 4) before the
+        next; with a plan and no k, each planned range with its own shape."""
         self._check_buffers()
-        n, k = int(n), self.k if k is None else int(k)
+        n = int(n)
         if n > 0:
-            self._check_k(n, k)
+            for _, _, kk, _, _, _ in self._segments(k):
+                self._check_k(n, kk)
             self._launch(self.parity, n, _stream(), k)
             self.parity = (self.parity + n) & 1
             self._sync_members(n)
@@ -221,24 +284,29 @@ class Ensemble:
             m.steps += n
 
     def capture(self, n=2, k=None):
-        """A Graph that performs n ensemble steps from the current parity and side.  Each replay must return
-        to both, so n is even, and with k = 2 a multiple of 4; replay it with ``replay(g)``."""
-        n, k = int(n), self.k if k is None else int(k)
-        if n < 2 or n % (2 * k):
-            raise ValueError("capture an even number of steps (a multiple of 4 with k = 2)")
-        self._check_k(n, k)
+        """A Graph that performs n ensemble steps from the current parity and sides.  Each replay must return
+        to both, so n is a multiple of 2k for every k in use; replay it with ``replay(g)``."""
+        n = int(n)
+        segs = self._segments(k)
+        for _, _, kk, _, _, _ in segs:
+            if n < 2 or n % (2 * kk):
+                raise ValueError(f"capture a multiple of {2 * kk} steps with k = {kk}")
+            self._check_k(n, kk)
         self._check_buffers()
         g = Graph()
+        sides = list(self.sides)
         with g.capture():
             self._launch(self.parity, n, ctypes.c_void_p(g.stream.cuda_stream), k)
-        g.steps, g.at = n, (self.parity, self.side)
+        assert self.sides == sides  # an even number of launches per member
+        g.steps, g.at = n, (self.parity, tuple(self.sides))
         return g
 
     def replay(self, g, times=1):
         """Replay a graph from ``capture``; the ensemble must be at the parity and side it was captured at
         (the launches' buffers and parities are fixed in the graph)."""
-        if (self.parity, self.side) != g.at:
-            raise RuntimeError(f"graph captured at (parity, side) {g.at}, ensemble at {(self.parity, self.side)}")
+        if (self.parity, tuple(self.sides)) != g.at:
+            raise RuntimeError(f"graph captured at (parity, sides) {g.at[0], g.at[1][:4]}..., ensemble at "
+                               f"{(self.parity, tuple(self.sides)[:4])}...")
         self._check_buffers()
         for _ in range(int(times)):
             g.replay()
@@ -296,16 +364,24 @@ GROUPS_MB = (0, 32, 64, 128, 192)
 PAIR_CHUNKS = (8, 12, 16, 24)
 
 
+QUAD_CHUNKS = (8, 20, 24)
+
+
 def tune_candidates(ens, steps):
     """(k, chunk, blocks) launch shapes to time: the single-step kernel at 2, 4, 6 or 8 workgroups per CU
-    (never more than are resident at once), and, for a two-step ensemble at an even state with an even
-    `steps`, the two-step kernel at each tile height on its resident grid (blocks 0) and half of it."""
+    (never more than are resident at once), and, for a two-step ensemble at an even state, the two-step kernel
+    (even `steps`) and the four-step kernel (FP32, `steps` a multiple of 4) at each tile shape on its resident
+    grid (blocks 0) and half of it."""
     full = ens.default_blocks
     cands = [(1, 0, b) for b in sorted({max(8, full * w // 8 // 8 * 8) for w in (2, 4, 6, 8)})]
     if ens.two_step and ens.parity == 0 and steps % 2 == 0:
         for ch in PAIR_CHUNKS:
             pb = ens.pair_blocks(ch)
             cands += [(2, ch, b) for b in sorted({max(8, pb // 2 // 8 * 8), pb})]
+    if ens.two_step and ens.parity == 0 and steps % 4 == 0 and ens.fmt == _lib.FMT_F32:
+        for ch in QUAD_CHUNKS:
+            pb = ens.pair_blocks(ch, k=4)
+            cands += [(4, ch, b) for b in sorted({max(8, pb // 2 // 8 * 8), pb})]
     return cands
 
 
@@ -316,7 +392,7 @@ def tune_ensemble(ens, use_cache=True, steps=20, reps=2, verbose=False):
     dev = torch.cuda.get_device_name(ens.members[0].device)
     m0 = ens.members[0]
     two = ens.two_step and ens.parity == 0 and steps % 2 == 0
-    key = f"{dev}|fmt{ens.fmt}|model{m0.model}|bc{m0.bc}|steps{steps}|k{2 if two else 1}|{size_classes(ens)}"
+    key = f"{dev}|fmt{ens.fmt}|model{m0.model}|bc{m0.bc}|steps{steps}|k{2 if two else 1}|v2|{size_classes(ens)}"
     cache = {}
     if use_cache:
         try:
@@ -348,3 +424,61 @@ def tune_ensemble(ens, use_cache=True, steps=20, reps=2, verbose=False):
                 pass
     ens.blocks, ens.group_bytes, ens.k, ens.chunk = int(best[0]), int(best[1]), int(best[3]), int(best[4])
     return best
+
+
+def class_segments(ens):
+    """The ensemble's contiguous runs of members of one size class, as Segments (an ensemble built from
+    ``sort_by_class`` members has one per class present)."""
+    segs, m0 = [], 0
+    names = [size_class(m.W, m.H) for m in ens.members]
+    for i in range(1, len(names) + 1):
+        if i == len(names) or names[i] != names[m0]:
+            segs.append(Segment(names[m0], m0, i))
+            m0 = i
+    return segs
+
+
+def _time_segment(ens, sg, k, chunk, blocks, group_bytes, steps, reps):
+    """Mean microseconds per step of the members of `sg` alone with the given shape.  `steps` is a multiple of
+    8, so every run makes an even number of launches for any k and leaves each member's state in the buffer it
+    started in, at the same parity."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    st = _stream()
+    ens._run(sg.m0, sg.m1, k, chunk, steps, blocks, group_bytes, st)
+    e0.record()
+    for _ in range(reps):
+        ens._run(sg.m0, sg.m1, k, chunk, steps, blocks, group_bytes, st)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / (steps * reps)
+
+
+def tune_classes(ens, steps=24, reps=2, verbose=False):
+    """A per-size-class plan: for each class range, the fastest (k, tile rows, grid, cache grouping) over
+    ``tune_candidates`` x GROUPS_MB, timed on that range alone.  Sets ``ens.plan`` and returns it.  Timing
+    advances the classes by different numbers of steps, so the states are copied first and put back afterwards:
+    the ensemble ends where it started."""
+    if ens.parity != 0 or steps % 8:
+        raise ValueError("tune classes from an even state with steps a multiple of 8")
+    ens._check_buffers()
+    saved = [m.f.clone() for m in ens.members]
+    sides = list(ens.sides)
+    plan = []
+    try:
+        for sg in class_segments(ens):
+            best = None
+            for gmb in GROUPS_MB:
+                for k, ch, b in tune_candidates(ens, steps):
+                    us = _time_segment(ens, sg, k, ch, b, gmb << 20, steps, reps)
+                    if verbose:
+                        print(f"  {sg.name}: k={k} rows={ch} blocks={b} group={gmb} MB: {us:.1f} us/step", flush=True)
+                    if best is None or us < best.us_per_step:
+                        best = Segment(sg.name, sg.m0, sg.m1, k, ch, b, gmb << 20, us)
+            plan.append(best)
+    finally:
+        assert ens.sides == sides  # even launch counts: every state is back in its own buffer
+        for m, f in zip(ens.members, saved):
+            m.f.copy_(f)
+        torch.cuda.synchronize()
+    ens.plan = plan
+    return plan

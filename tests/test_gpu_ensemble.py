@@ -309,9 +309,126 @@ def test_ensemble_tuner_two_step(gpu, tmp_path, monkeypatch):
     a, b = _periodic_pair(SIZES[:6], seed=91)
     ens = Ensemble(a, two_step=True)
     best = tune_ensemble(ens, steps=4, reps=1)
-    assert best[3] in (1, 2) and best[0] % 8 == 0 and best[2] > 0
+    assert best[3] in (1, 2, 4) and best[0] % 8 == 0 and best[2] > 0
     assert (ens.blocks, ens.group_bytes, ens.k, ens.chunk) == (best[0], best[1], best[3], best[4])
     assert tuple(tune_ensemble(ens, steps=4, reps=1)) == tuple(best)
     n = ens.members[0].steps
     _solo_steps(b, n)
     _assert_same_observable(a, b, "after tuning")
+
+
+# ---------------------------------------------------------------------------------------------------------
+# four-step launches, per-size-class plans and the 64-member workload
+# ---------------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("chunk", [8, 20, 24])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_ensemble_four_step_bitwise(gpu, chunk, periodic):
+    """k = 4 launches (the four-step tile kernel over every member, FP32) equal single steps of each member
+    alone, on ragged sizes including lattices narrower and lower than one tile region; mixed with k = 2 and
+    k = 1 ensemble steps."""
+    from lbm.ensemble import Ensemble
+    a, b = _periodic_pair(SIZES, seed=91) if periodic else _channel_pair(SIZES, seed=92)
+    ens = Ensemble(a, two_step=True, k=4, chunk=chunk)
+    for n in (4, 8, 4):
+        ens.step(n)
+        _solo_steps(b, n)
+        _assert_same_observable(a, b, f"k=4 rows {chunk} after +{n} (side {ens.side})")
+    ens.chunk = 12
+    ens.step(2, k=2)
+    _solo_steps(b, 2)
+    ens.chunk = chunk
+    ens.step(1, k=1)
+    _solo_steps(b, 1)
+    with pytest.raises(ValueError):
+        ens.step(4, k=4)  # odd parity
+    ens.step(3, k=1)
+    _solo_steps(b, 3)
+    ens.step(4, k=4)
+    _solo_steps(b, 4)
+    _assert_same_observable(a, b, f"k=4 rows {chunk} mixed")
+
+
+def test_ensemble_four_step_rejects_16bit(gpu):
+    from lbm.ensemble import Ensemble
+    a, _ = _periodic_pair(SIZES[:3], storage="f16", seed=93)
+    ens = Ensemble(a, two_step=True)
+    with pytest.raises(ValueError):
+        ens.step(4, k=4)
+
+
+def test_ensemble_class_plan_bitwise(gpu):
+    """A plan giving each member range its own k and tile shape (k = 1 in place, k = 2, k = 4): members end on
+    different buffer sides and every one equals its solo run; graphs of planned steps replay."""
+    from lbm import _lib
+    from lbm.ensemble import Ensemble, Segment
+    sizes = SIZES + [(300, 200), (520, 300)]
+    a, b = _periodic_pair(sizes, seed=94)
+    ens = Ensemble(a, two_step=True)
+    n = len(sizes)
+    ens.plan = [Segment("x", 0, 3, 1, 0, 0, 0), Segment("y", 3, 6, 2, 16, 0, 1 << 20),
+                Segment("z", 6, n, 4, 24, 64, 0)]
+    ens.step(4)
+    _solo_steps(b, 4)
+    _assert_same_observable(a, b, "plan +4")
+    assert ens.sides[:3] == [0] * 3 and ens.sides[3:6] == [0] * 3 and ens.sides[6:] == [1] * (n - 6)
+    assert ens.side == -1
+    with pytest.raises(_lib.LbmError):
+        _lib.check(ens.lib.lbm_ensemble_side(ens.handle), "lbm_ensemble_side")  # the sides differ
+    ens.step(8)
+    _solo_steps(b, 8)
+    _assert_same_observable(a, b, "plan +8")
+    with pytest.raises(ValueError):
+        ens.capture(4)  # k = 4 ranges need a multiple of 8 steps per replay
+    g = ens.capture(8)
+    for rep in range(3):
+        ens.replay(g)
+        _solo_steps(b, 8)
+        _assert_same_observable(a, b, f"plan replay {rep}")
+        ens.step(8)  # eager work between replays
+        _solo_steps(b, 8)
+    g.close()
+
+
+def test_ensemble_class_tuner(gpu):
+    """tune_classes gives every size class of a sorted member list its own shape, leaves the states where
+    they were, and the planned steps stay bitwise equal to solo runs."""
+    from lbm.ensemble import Ensemble, class_segments, sort_by_class, tune_classes
+    sizes = sort_by_class([type("M", (), dict(W=w, H=h))() for w, h in
+                           [(256, 256), (1024, 768), (384, 256), (512, 512), (1536, 1536), (768, 384)]])
+    a, b = _periodic_pair([(m.W, m.H) for m in sizes], seed=95)
+    ens = Ensemble(a, two_step=True)
+    names = [sg.name for sg in class_segments(ens)]
+    assert names == ["small", "medium", "large"]
+    before = [m.f.clone() for m in a]
+    plan = tune_classes(ens, steps=8, reps=1)
+    assert [sg.name for sg in plan] == names and all(sg.k in (1, 2, 4) for sg in plan)
+    for m, f in zip(a, before):
+        assert torch.equal(m.f, f)
+    ens.step(8)
+    _solo_steps(b, 8)
+    _assert_same_observable(a, b, "tuned plan")
+
+
+def test_ensemble_workload_64_members(gpu):
+    """The 64-member batched workload (configs.ensemble: sizes 256..2048 in both directions, nu log-uniform,
+    random-phase fields): the persistent ensemble with k = 1, 2 and 4 launches equals every member's solo run
+    bit for bit after 16 steps."""
+    from lbm import configs
+    from lbm.ensemble import Ensemble
+    from lbm.sim import Simulation
+    members = configs.ensemble()
+    ens = Ensemble.from_members(members, two_step=True)
+    solo = [Simulation(m.W, m.H, m.tau_inv, cs=m.cs, model="mrt", bc="periodic", f0=s.f.cpu().numpy())
+            for m, s in zip(members, ens.members)]
+    ens.chunk = 20
+    ens.step(4, k=4)
+    ens.chunk = 8
+    ens.step(4, k=2)
+    ens.step(4, k=1)
+    ens.chunk = 24
+    ens.step(4, k=4)
+    for s in solo:
+        s.step(16)
+    _assert_same_observable(ens.members, solo, "64 members after 16 steps")
+    ens.close()
