@@ -2,11 +2,12 @@
 //
 // No reference counterpart: the reference's only closure is the Smagorinsky model of lbm_core4.cu:171-177,
 // whose inputs m7neq / m8neq (lbm_core4.cu:172-173) are the network's features here, taken over a 3 x 3
-// stencil.  One wave evaluates 64 samples with v_mfma_f32_16x16x16_bf16 (BF16 operands, FP32
-// accumulation): a layer is a [64 x K] x [K x 64] product, tiled 4 x 4 over 16 x 16 output tiles.
+// stencil.  One wave evaluates 64 samples with gfx950's v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32
+// accumulation, 32 of K per instruction): a layer is a [64 x K] x [K x 64] product, tiled 4 x 4 over 16 x 16
+// output tiles.
 //
-// MFMA operand layout (16 x 16 x 16, lane l, r = l % 16, g = l / 16, item j = 0..3):
-//   A[i][k]: lane holds A[r][4g + j]      B[k][n]: lane holds B[4g + j][r]      D[i][n]: lane holds D[4g + j][r]
+// MFMA operand layout (16 x 16 x 32, lane l, r = l % 16, g = l / 16; A and B items j = 0..7, D items j = 0..3):
+//   A[i][k]: lane holds A[r][8g + j]      B[k][n]: lane holds B[8g + j][r]      D[i][n]: lane holds D[4g + j][r]
 //
 // Parameters, FP32, in this order (5441 values):
 //   W1 [18][64] (input-major) | b1 [64] | W2 [64][64] | b2 [64] | W3 [64] | b3
@@ -33,7 +34,7 @@ constexpr int XS = 40; // inputs [64][XS], columns 0..31 used
 constexpr int HS = 72; // activations [64][HS], columns 0..63 used
 constexpr int WAVE_LDS = 64 * XS + 2 * 64 * HS; // bf16 elements per wave
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float x)
@@ -44,18 +45,17 @@ __device__ __forceinline__ uint16_t f2bf(float x)
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
 
-__device__ __forceinline__ f4 mma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mma(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
-__device__ __forceinline__ s4 ld4(const uint16_t *p) { return *reinterpret_cast<const s4 *>(p); }
+// eight consecutive BF16 values (16-byte aligned: every row stride and offset used is a multiple of 8 elements)
+__device__ __forceinline__ b8 ld8(const uint16_t *p) { return *reinterpret_cast<const b8 *>(p); }
 
-// four values of one column at rows row0 .. row0 + 3
-__device__ __forceinline__ s4 ld4col(const uint16_t *p, int row0, int stride, int col)
+// eight values of one column at rows row0 .. row0 + 7
+__device__ __forceinline__ b8 ld8col(const uint16_t *p, int row0, int stride, int col)
 {
-    s4 v;
-    v[0] = (short)p[(row0 + 0) * stride + col];
-    v[1] = (short)p[(row0 + 1) * stride + col];
-    v[2] = (short)p[(row0 + 2) * stride + col];
-    v[3] = (short)p[(row0 + 3) * stride + col];
+    b8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = __builtin_bit_cast(__bf16, p[(row0 + j) * stride + col]);
     return v;
 }
 
@@ -78,13 +78,11 @@ __device__ __forceinline__ void wave_forward(const uint16_t *xs, uint16_t *h1s, 
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         const float bias = prm[OFF_B1 + 16 * nt + r];
-        const s4 b0 = ld4(wp + PK_W1T + (16 * nt + r) * KIN + 4 * g);
-        const s4 b1 = ld4(wp + PK_W1T + (16 * nt + r) * KIN + 16 + 4 * g);
+        const b8 b0 = ld8(wp + PK_W1T + (16 * nt + r) * KIN + 8 * g);
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mma(ld4(xs + (16 * mt + r) * XS + 4 * g), b0, acc);
-            acc = mma(ld4(xs + (16 * mt + r) * XS + 16 + 4 * g), b1, acc);
+            acc = mma(ld8(xs + (16 * mt + r) * XS + 8 * g), b0, acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float z = acc[j] + bias;
@@ -97,14 +95,14 @@ __device__ __forceinline__ void wave_forward(const uint16_t *xs, uint16_t *h1s, 
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         const float bias = prm[OFF_B2 + 16 * nt + r];
-        s4 b[4];
+        b8 b[2];
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) b[ks] = ld4(wp + PK_W2T + (16 * nt + r) * NH + 16 * ks + 4 * g);
+        for (int ks = 0; ks < 2; ++ks) b[ks] = ld8(wp + PK_W2T + (16 * nt + r) * NH + 32 * ks + 8 * g);
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) acc = mma(ld4(h1s + (16 * mt + r) * HS + 16 * ks + 4 * g), b[ks], acc);
+            for (int ks = 0; ks < 2; ++ks) acc = mma(ld8(h1s + (16 * mt + r) * HS + 32 * ks + 8 * g), b[ks], acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float z = acc[j] + bias;
@@ -115,14 +113,14 @@ __device__ __forceinline__ void wave_forward(const uint16_t *xs, uint16_t *h1s, 
     }
     __syncthreads();
     const float b3 = prm[OFF_B3];
-    s4 b[4];
+    b8 b[2];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) b[ks] = ld4(wp + PK_W3T + r * NH + 16 * ks + 4 * g);
+    for (int ks = 0; ks < 2; ++ks) b[ks] = ld8(wp + PK_W3T + r * NH + 32 * ks + 8 * g);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) acc = mma(ld4(h2s + (16 * mt + r) * HS + 16 * ks + 4 * g), b[ks], acc);
+        for (int ks = 0; ks < 2; ++ks) acc = mma(ld8(h2s + (16 * mt + r) * HS + 32 * ks + 8 * g), b[ks], acc);
         if (r == 0) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) ys[16 * mt + 4 * g + j] = acc[j] + b3;

@@ -143,8 +143,8 @@ __global__ __launch_bounds__(256) void k_grad(const uint16_t *__restrict__ pk, c
         for (int nt = 0; nt < 4; ++nt) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-                acc = mma(ld4col(h1s, 16 * ks + 4 * g, HS, 16 * mt + r), ld4col(h2s, 16 * ks + 4 * g, HS, 16 * nt + r),
+            for (int ks = 0; ks < 2; ++ks)
+                acc = mma(ld8col(h1s, 32 * ks + 8 * g, HS, 16 * mt + r), ld8col(h2s, 32 * ks + 8 * g, HS, 16 * nt + r),
                           acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) out[OFF_W2 + (16 * mt + 4 * g + j) * NH + 16 * nt + r] = acc[j];
@@ -154,15 +154,15 @@ __global__ __launch_bounds__(256) void k_grad(const uint16_t *__restrict__ pk, c
     f4 dz1[4][4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        s4 b[4];
+        b8 b[2];
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) b[ks] = ld4(wl + PK_W2N + (16 * nt + r) * NH + 16 * ks + 4 * g);
+        for (int ks = 0; ks < 2; ++ks) b[ks] = ld8(wl + PK_W2N + (16 * nt + r) * NH + 32 * ks + 8 * g);
         float acc_b = 0.0f;
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) acc = mma(ld4(h2s + (16 * mt + r) * HS + 16 * ks + 4 * g), b[ks], acc);
+            for (int ks = 0; ks < 2; ++ks) acc = mma(ld8(h2s + (16 * mt + r) * HS + 32 * ks + 8 * g), b[ks], acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 dz1[mt][nt][j] = acc[j] * gelu_d(z1[mt][nt][j]);
@@ -192,8 +192,8 @@ __global__ __launch_bounds__(256) void k_grad(const uint16_t *__restrict__ pk, c
         for (int nt = 0; nt < 4; ++nt) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-                acc = mma(ld4col(xs, 16 * ks + 4 * g, XS, 16 * mt + r), ld4col(h1s, 16 * ks + 4 * g, HS, 16 * nt + r),
+            for (int ks = 0; ks < 2; ++ks)
+                acc = mma(ld8col(xs, 32 * ks + 8 * g, XS, 16 * mt + r), ld8col(h1s, 32 * ks + 8 * g, HS, 16 * nt + r),
                           acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {

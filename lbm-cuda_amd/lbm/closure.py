@@ -256,12 +256,28 @@ class StudyConfig:
     snapshots: int = 8         # training snapshots (the last one is held out for validation)
     interval: int = 400        # fine steps between snapshots
     warm: int = 10             # coarse Smagorinsky steps before the features are taken
-    window: int = 5            # neighbourhood of the least-squares eddy-viscosity target
-    eval_steps: int = 400      # coarse steps of the a-posteriori comparison
-    eval_every: int = 100
+    window: int = 33           # neighbourhood of the least-squares eddy-viscosity target (coarse cells)
+    clip_target: bool = False  # train on max(nu*, 0) (the step clamps negative outputs to 0 anyway)
+    augment: bool = True       # train on the 8 rotations / reflections of every sample (the D4 symmetry)
+    eval_steps: int = 20000    # coarse steps of the a-posteriori comparison
+    eval_every: int = 2000
     epochs: int = 6
     batch: int = 8192
     lr: float = 2e-3
+
+
+def _step_fast(sim, n):
+    """n steps of an FP32 periodic lattice, four per launch with the four-step kernel where the parity allows
+    (bit-identical to single steps)."""
+    n = int(n)
+    if sim.parity:
+        sim.step(1)
+        n -= 1
+    q = (n // 4) * 4
+    if q:
+        sim.step(q, k=4, cfg=(0, 24))
+    if n - q:
+        sim.step(n - q)
 
 
 def _coarse_sim(cfg: StudyConfig, rho, u, v, cs, device):
@@ -297,27 +313,55 @@ def _coarse_uv(sim):
     return rho, ux, uy
 
 
-def run_study(cfg: StudyConfig = StudyConfig(), device="cuda", log=print):
-    """Build the training set from the fine run, train the MLP on the GPU, and compare the closure with
-    plain Smagorinsky a posteriori.  Returns a report dict."""
+def d4_augment(X, T, Wt):
+    """The 8 images of every sample under the symmetries of the square lattice.  Feature 2 s + c holds moment c
+    (c = 0: m7neq ~ S_xx - S_yy, c = 1: m8neq ~ S_xy) at stencil offset s = 3 (dy + 1) + (dx + 1).  A rotation
+    by 90 degrees maps the offset (dx, dy) to (-dy, dx) and flips the signs of both moments; the reflection
+    x -> -x maps (dx, dy) to (-dx, dy) and flips the sign of m8neq.  The target (a scalar viscosity) and the
+    weight are invariant."""
+    def perm(fn):
+        out = np.empty(18, np.int64)
+        for s in range(9):
+            dx, dy = s % 3 - 1, s // 3 - 1
+            ndx, ndy = fn(dx, dy)
+            t = 3 * (ndy + 1) + (ndx + 1)
+            out[2 * t], out[2 * t + 1] = 2 * s, 2 * s + 1
+        return out
+    rot = perm(lambda dx, dy: (-dy, dx))
+    ref = perm(lambda dx, dy: (-dx, dy))
+    s_rot = np.tile(np.array([-1.0, -1.0], np.float32), 9)
+    s_ref = np.tile(np.array([1.0, -1.0], np.float32), 9)
+    xs = []
+    base = X
+    for _ in range(4):
+        xs.append(base)
+        xs.append(base[:, ref] * s_ref[None, :])
+        base = base[:, rot] * s_rot[None, :]
+    return np.concatenate(xs), np.tile(T, 8), np.tile(Wt, 8)
+
+
+def build_dataset(cfg: StudyConfig, device="cuda", log=print):
+    """The fine run (spin-up, then `snapshots` snapshots `interval` steps apart) and, per snapshot, the coarse
+    features (3 x 3 (m7neq, m8neq) stencils of a coarse lattice started from the filtered state after `warm`
+    Smagorinsky steps), the windowed least-squares eddy-viscosity target of the filtered fine subgrid stress,
+    its weight, and the Smagorinsky viscosity of the same coarse state.  Returns (fine, dataset dict)."""
     import time
     from . import geometry as geo
     t_start = time.perf_counter()
     N, fct = cfg.N, cfg.factor
-    n = N // fct
     ux, uy = geo.random_phase_velocity(N, N, k0=cfg.k0, u_rms=cfg.u_rms, seed=cfg.seed)
     fine = Simulation(N, N, float(np.float32(geo.nu_tau_inv(cfg.nu_fine))), cs=cfg.cs_fine, model="mrt",
                       bc="periodic", f0=geo.feq(np.ones((N, N)), ux, uy), device=device)
     del ux, uy
     log(f"closure study: fine {N}^2 built in {time.perf_counter() - t_start:.1f} s")
-    fine.step(cfg.spinup)
+    _step_fast(fine, cfg.spinup)
     X, T, Wt, smag_nu = [], [], [], []
     for s in range(cfg.snapshots):
         rb, u, v, _ = filtered_fields(fine, fct)
         coarse = _coarse_sim(cfg, rb, u, v, cfg.cs_coarse, device)
         coarse.step(cfg.warm)
         nq = neq(coarse).cpu().numpy()
-        fine.step(cfg.warm * fct)
+        _step_fast(fine, cfg.warm * fct)
         rb2, u2, v2, tau2 = filtered_fields(fine, fct)
         nu_star, ss = sgs_target(u2, v2, tau2, cfg.window)
         X.append(stencil_features(nq).astype(np.float32))
@@ -326,26 +370,42 @@ def run_study(cfg: StudyConfig = StudyConfig(), device="cuda", log=print):
         rho_c = coarse.macro()[0].cpu().numpy().astype(np.float64)
         smag_nu.append(smagorinsky_nu(nq, rho_c, 1.0 / coarse.params.tau_inv, cfg.cs_coarse).reshape(-1))
         del coarse
-        fine.step(cfg.interval)
+        _step_fast(fine, cfg.interval)
     log(f"closure study: {cfg.snapshots} snapshots in {time.perf_counter() - t_start:.1f} s")
-    # normalisation from the training snapshots
+    return fine, {"X": X, "T": T, "W": Wt, "smag": smag_nu}
+
+
+def train_closure(cfg: StudyConfig, data, device="cuda", log=print):
+    """Normalise the training snapshots (all but the last), train the MLP with the GPU kernels (Adam, cosine
+    learning rate, weighted squared error), and score it a priori on the held-out snapshot.  Returns
+    (mlp, closure config, a-priori report, training info)."""
+    import time
+    X, T, Wt, smag_nu = data["X"], data["T"], data["W"], data["smag"]
     Xtr = np.concatenate(X[:-1])
     s7 = 1.0 / Xtr[:, 0::2].std()
     s8 = 1.0 / Xtr[:, 1::2].std()
-    in_scale = np.tile(np.array([s7, s8], np.float32), 9)
+    s78 = 1.0 / np.sqrt(0.5 * (Xtr[:, 0::2].var() + Xtr[:, 1::2].var())) if cfg.augment else None
+    in_scale = np.tile(np.array([s78, s78] if cfg.augment else [s7, s8], np.float32), 9)
     Ttr = np.concatenate(T[:-1])
     Wtr = np.concatenate(Wt[:-1])
     nu_scale = float(np.sqrt(np.sum(Wtr * Ttr ** 2) / np.sum(Wtr)))
     ccfg = ClosureConfig(in_scale=in_scale, nu_scale=nu_scale, nu_max=0.05, feat_max=20.0)
 
-    def prep(Xs, Ts, Ws):
-        x = np.concatenate(Xs) * in_scale[None, :]
-        t = np.concatenate(Ts) / nu_scale
+    def prep(Xs, Ts, Ws, aug):
+        x = np.concatenate(Xs)
+        t = np.concatenate(Ts)
         w = np.concatenate(Ws)
+        if cfg.clip_target:
+            t = np.maximum(t, 0.0)
+        if aug:
+            x, t, w = d4_augment(x, t, w)
+        x = x * in_scale[None, :]
+        t = t / nu_scale
         return x.astype(np.float32), t.astype(np.float32), (w / w.mean()).astype(np.float32)
 
-    xtr, ttr, wtr = prep(X[:-1], T[:-1], Wt[:-1])
-    xva, tva, wva = prep(X[-1:], T[-1:], Wt[-1:])
+    xtr, ttr, wtr = prep(X[:-1], T[:-1], Wt[:-1], cfg.augment)
+    xva, tva, wva = prep(X[-1:], T[-1:], Wt[-1:], False)
+    tva_raw = np.concatenate(T[-1:]) / nu_scale  # the unclipped target, for the a-priori scores
     mlp = MLP(seed=cfg.seed, device=device)
     y_init = mlp.forward(torch.from_numpy(xva).to(device)).cpu().numpy()
     rng = np.random.default_rng(cfg.seed)
@@ -364,6 +424,7 @@ def run_study(cfg: StudyConfig = StudyConfig(), device="cuda", log=print):
             mlp.train_step(xe[sl], te[sl], we[sl], lr=lr, loss=loss)
             steps += 1
         log(f"  epoch {ep}: lr {lr:.2e}, last batch loss {loss.item():.4f}")
+        del xe, te, we
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t_train
 
@@ -372,53 +433,133 @@ def run_study(cfg: StudyConfig = StudyConfig(), device="cuda", log=print):
 
     y_va = mlp.forward(torch.from_numpy(xva).to(device)).cpu().numpy()
     smag_va = smag_nu[-1] / nu_scale
-    apriori = {"val_wmse_mlp": wmse(y_va, tva, wva), "val_wmse_untrained": wmse(y_init, tva, wva),
-               "val_wmse_smagorinsky": wmse(smag_va, tva, wva),
-               "val_wmse_zero": wmse(np.zeros_like(tva), tva, wva),
-               "corr_mlp": float(np.corrcoef(y_va, tva)[0, 1]), "corr_smagorinsky": float(np.corrcoef(smag_va, tva)[0, 1])}
+    y_used = np.maximum(y_va, 0.0)  # what the step applies
+    apriori = {"val_wmse_mlp": wmse(y_va, tva_raw, wva), "val_wmse_mlp_clamped": wmse(y_used, tva_raw, wva),
+               "val_wmse_untrained": wmse(y_init, tva_raw, wva),
+               "val_wmse_smagorinsky": wmse(smag_va, tva_raw, wva),
+               "val_wmse_zero": wmse(np.zeros_like(tva_raw), tva_raw, wva),
+               "corr_mlp": float(np.corrcoef(y_va, tva_raw)[0, 1]),
+               "corr_smagorinsky": float(np.corrcoef(smag_va, tva_raw)[0, 1]),
+               "mean_nu_ratio_mlp": float(np.sum(wva * y_used) / np.sum(wva * np.maximum(tva_raw, 0.0))),
+               "mean_nu_ratio_smagorinsky": float(np.sum(wva * smag_va) / np.sum(wva * np.maximum(tva_raw, 0.0)))}
     log(f"closure study: trained {steps} steps in {train_s:.1f} s; a-priori {apriori}")
+    info = {"samples_train": int(len(xtr)), "samples_val": int(len(xva)), "train_steps": steps,
+            "train_seconds": round(train_s, 2), "nu_scale": nu_scale}
+    return mlp, ccfg, apriori, info
 
-    # a posteriori: coarse Smagorinsky vs coarse learned closure from the same filtered state
+
+def reference_track(cfg: StudyConfig, fine: Simulation, log=print):
+    """The filtered fine run from now on: its filtered start state (rho, u, v) for the coarse runs and, every
+    `eval_every` coarse steps up to `eval_steps`, the filtered field's energy spectrum, energy and enstrophy."""
+    fct = cfg.factor
     rb, u, v, _ = filtered_fields(fine, fct)
-    smag = _coarse_sim(cfg, rb, u, v, cfg.cs_coarse, device)
-    learned = ClosureSim(_coarse_sim(cfg, rb, u, v, cfg.cs_coarse, device), mlp, ccfg)
-    rows = []
-    kres = np.arange(1, n // 8 + 1)  # shells well inside the filter (k <= n / 8)
+    start = (rb, u, v)
+    track = []
     for k in range(0, cfg.eval_steps + 1, cfg.eval_every):
         if k:
-            fine.step(cfg.eval_every * fct)
-            smag.step(cfg.eval_every)
-            learned.step(cfg.eval_every)
+            _step_fast(fine, cfg.eval_every * fct)
         _, uf, vf, _ = filtered_fields(fine, fct)
-        ref_e, ref_z = spectrum(uf, vf), enstrophy(uf, vf)
-        row = {"coarse_step": k, "enstrophy_ref": ref_z}
-        for name, sim in (("smagorinsky", smag), ("learned", learned.sim)):
+        e = spectrum(uf, vf)
+        track.append({"coarse_step": k, "spectrum": e, "energy": float(e[1:].sum()), "enstrophy": enstrophy(uf, vf)})
+    log(f"closure study: reference track of {len(track)} filtered checkpoints")
+    return start, track
+
+
+def evaluate(cfg: StudyConfig, start, track, runners, device="cuda", log=print, spectra=None):
+    """Coarse runs from the filtered start, scored against the reference track: mean |log E / E_ref| over the
+    shells 1 .. n / 8 (well inside the filter), the relative energy and enstrophy errors.  `runners` maps a name
+    to a function (steps) -> coarse Simulation after those steps (each keeps its own state).  `spectra` (a dict,
+    optional) receives each runner's spectra and enstrophies at the checkpoints."""
+    n = start[0].shape[0]
+    kres = np.arange(1, n // 8 + 1)
+    rows = []
+    done = 0
+    for ref in track:
+        k = ref["coarse_step"]
+        row = {"coarse_step": k, "enstrophy_ref": ref["enstrophy"]}
+        for name, run in runners.items():
+            sim = run(k - done)
             _, uc, vc = _coarse_uv(sim)
             e = spectrum(uc, vc)
-            row[f"spectrum_err_{name}"] = float(np.mean(np.abs(np.log(e[kres] / ref_e[kres]))))
-            row[f"energy_err_{name}"] = abs(float(e[1:].sum() / ref_e[1:].sum()) - 1.0)
-            row[f"enstrophy_{name}"] = enstrophy(uc, vc)
-            row[f"enstrophy_err_{name}"] = abs(row[f"enstrophy_{name}"] - ref_z) / ref_z
+            row[f"spectrum_err_{name}"] = float(np.mean(np.abs(np.log(e[kres] / ref["spectrum"][kres]))))
+            row[f"energy_err_{name}"] = abs(float(e[1:].sum()) / ref["energy"] - 1.0)
+            z = enstrophy(uc, vc)
+            if spectra is not None:
+                spectra.setdefault(name, []).append((e, z))
+            row[f"enstrophy_{name}"] = z
+            row[f"enstrophy_err_{name}"] = abs(z - ref["enstrophy"]) / ref["enstrophy"]
+        done = k
         rows.append(row)
-        log(f"  coarse step {k}: spectrum err smag {row['spectrum_err_smagorinsky']:.4f} learned "
-            f"{row['spectrum_err_learned']:.4f}; enstrophy err smag {row['enstrophy_err_smagorinsky']:.4f} "
-            f"learned {row['enstrophy_err_learned']:.4f}")
+        log("  coarse step %d: " % k + ", ".join(f"{nm} spec {row[f'spectrum_err_{nm}']:.4f} ens "
+                                                  f"{row[f'enstrophy_err_{nm}']:.4f}" for nm in runners))
+    return rows
+
+
+def time_averaged_errors(track, spectra, n):
+    """Errors of the spectrum and the enstrophy averaged over the checkpoints after the start (the statistics of a
+    chaotic flow, less sensitive to where its vortices happen to be than any one instant): mean over the shells
+    1 .. n / 8 of |log <E> / <E_ref>|, its signed mean over the shells above n / 16 (negative: too little energy
+    at small scales, i.e. too much dissipation), and |<Z> - <Z_ref>| / <Z_ref>."""
+    kres = np.arange(1, n // 8 + 1)
+    hi = kres >= n // 16
+    R = np.mean([t["spectrum"] for t in track[1:]], 0)[kres]
+    Z = float(np.mean([t["enstrophy"] for t in track[1:]]))
+    out = {}
+    for name, vals in spectra.items():
+        M = np.mean([e for e, _ in vals[1:]], 0)[kres]
+        out[f"spectrum_err_timeavg_{name}"] = float(np.mean(np.abs(np.log(M / R))))
+        out[f"spectrum_log_bias_high_k_{name}"] = float(np.mean(np.log(M[hi] / R[hi])))
+        out[f"enstrophy_err_timeavg_{name}"] = abs(float(np.mean([z for _, z in vals[1:]])) - Z) / Z
+    return out
+
+
+def summarize(rows, names):
+    """Means over the checkpoints after the start, and the final checkpoint's errors, per runner."""
+    out = {}
+    for nm in names:
+        out[f"spectrum_err_mean_{nm}"] = float(np.mean([r[f"spectrum_err_{nm}"] for r in rows[1:]]))
+        out[f"enstrophy_err_mean_{nm}"] = float(np.mean([r[f"enstrophy_err_{nm}"] for r in rows[1:]]))
+        out[f"energy_err_mean_{nm}"] = float(np.mean([r[f"energy_err_{nm}"] for r in rows[1:]]))
+        out[f"spectrum_err_final_{nm}"] = rows[-1][f"spectrum_err_{nm}"]
+        out[f"enstrophy_err_final_{nm}"] = rows[-1][f"enstrophy_err_{nm}"]
+        out[f"energy_err_final_{nm}"] = rows[-1][f"energy_err_{nm}"]
+    return out
+
+
+def run_study(cfg: StudyConfig = StudyConfig(), device="cuda", log=print):
+    """Build the training set from the fine run, train the MLP on the GPU, and compare the closure with
+    plain Smagorinsky a posteriori over `eval_steps` coarse steps.  Returns (report, mlp, closure config)."""
+    import time
+    t_start = time.perf_counter()
+    fine, data = build_dataset(cfg, device, log)
+    mlp, ccfg, apriori, info = train_closure(cfg, data, device, log)
+    del data
+    start, track = reference_track(cfg, fine, log)
+    del fine
+    torch.cuda.empty_cache()
+    rb, u, v = start
+    smag = _coarse_sim(cfg, rb, u, v, cfg.cs_coarse, device)
+    learned = ClosureSim(_coarse_sim(cfg, rb, u, v, cfg.cs_coarse, device), mlp, ccfg)
+
+    def run_smag(k):
+        _step_fast(smag, k)
+        return smag
+
+    def run_learned(k):
+        learned.step(k)
+        return learned.sim
+
+    spectra = {}
+    rows = evaluate(cfg, start, track, {"smagorinsky": run_smag, "learned": run_learned}, device, log, spectra)
+    n = rb.shape[0]
     fb = int(learned.fallbacks.item())
     report = {
         "config": {k: getattr(cfg, k) for k in cfg.__dataclass_fields__},
-        "coarse": n, "samples_train": int(len(xtr)), "samples_val": int(len(xva)), "train_steps": steps,
-        "train_seconds": round(train_s, 2), "nu_scale": nu_scale, "apriori": apriori, "aposteriori": rows,
-        "fallback_fraction": fb / float(cfg.eval_steps * n * n),
+        "coarse": n, **info, "apriori": apriori, "aposteriori": rows,
+        "fallback_fraction": fb / float(max(1, cfg.eval_steps) * n * n),
         "seconds": round(time.perf_counter() - t_start, 1),
     }
-    final = rows[-1]
-    report["summary"] = {
-        "spectrum_shells": f"1..{n // 8}",
-        "spectrum_err_mean_smagorinsky": float(np.mean([r["spectrum_err_smagorinsky"] for r in rows[1:]])),
-        "spectrum_err_mean_learned": float(np.mean([r["spectrum_err_learned"] for r in rows[1:]])),
-        "energy_err_final_smagorinsky": final["energy_err_smagorinsky"],
-        "energy_err_final_learned": final["energy_err_learned"],
-        "enstrophy_err_final_smagorinsky": final["enstrophy_err_smagorinsky"],
-        "enstrophy_err_final_learned": final["enstrophy_err_learned"],
-    }
+    report["summary"] = {"spectrum_shells": f"1..{n // 8}", **summarize(rows, ("smagorinsky", "learned")),
+                         **time_averaged_errors(track, spectra, n)}
+    log(f"closure study: {report['summary']}")
     return report, mlp, ccfg

@@ -112,3 +112,34 @@ def test_regularized_state_moments():
     pxx_pyy = ((G.CX[:, None, None] ** 2 - G.CY[:, None, None] ** 2) * fn).sum(0)
     assert np.allclose(pxy, -(2 / 3) * rho * tau * sxy, rtol=1e-10, atol=1e-16)
     assert np.allclose(pxx_pyy, -(2 / 3) * rho * tau * (dudx - dvdy), rtol=1e-10, atol=1e-16)
+
+
+def test_d4_augmentation_matches_transformed_fields():
+    """The closure's symmetry augmentation is the stencil features of the rotated / reflected field.  A
+    traceless symmetric tensor field (m7 ~ S_xx - S_yy, m8 ~ S_xy) rotated by 90 degrees, (x, y) -> (-y, x),
+    becomes m7'(r) = -m7(R^T r), m8'(r) = -m8(R^T r); reflected, x -> -x, it becomes m7'(r) = m7(Mr),
+    m8'(r) = -m8(Mr).  The features of the transformed field at r must be the augmented features of the original
+    at the pre-image of r (periodic N x N grid, arrays indexed [y, x])."""
+    from lbm import closure as C
+    N = 12
+    rng = np.random.default_rng(3)
+    m7, m8 = rng.standard_normal((N, N)), rng.standard_normal((N, N))
+    X = C.stencil_features(np.stack([m7, m8]))  # [N*N, 18], row y * N + x
+    T = np.zeros(N * N)
+    Xa, _, _ = C.d4_augment(X.astype(np.float32), T, T)
+    images = Xa.reshape(8, N * N, 18)
+    yy, xx = np.mgrid[0:N, 0:N]
+    # rotation: the value at r = (x, y) comes from R^T r = (y, -x)
+    src_y, src_x = (-xx) % N, yy
+    rot = C.stencil_features(np.stack([-m7[src_y, src_x], -m8[src_y, src_x]]))
+    pre = (src_y * N + src_x).reshape(-1)
+    # reflection: the value at (x, y) comes from (-x, y)
+    ry, rx = yy, (-xx) % N
+    refl = C.stencil_features(np.stack([m7[ry, rx], -m8[ry, rx]]))
+    pre_r = (ry * N + rx).reshape(-1)
+    # d4_augment's order: identity, reflection, rotation, rotation then reflection, ...
+    assert np.allclose(images[0], X, atol=1e-6)
+    assert np.allclose(images[2][pre], rot, atol=1e-6)
+    assert np.allclose(images[1][pre_r], refl, atol=1e-6)
+    # the 8 images are distinct group elements (no duplicate)
+    assert len({img.tobytes() for img in images}) == 8
