@@ -74,8 +74,9 @@ typedef struct lbm_params {
     float cs;       /* Smagorinsky constant; 0 = plain MRT (no LES) */
     float s_e, s_eps, s_q; /* MRT rates of e, eps and q_x/q_y (reference: 1.1) */
     int32_t sums;   /* 0: plain FP32 sums of rho - 1 and j in the MRT collision; 1: compensated (Neumaier) sums of
-                       both, for the precision study.  1 is supported by the single-step entry points (lbm_step_aa*)
-                       only; the others return LBM_ERR_UNSUPPORTED */
+                       both, for the precision study.  Every stepping entry point takes 1 (single, pull, fused,
+                       ensemble; each temporally blocked form bit-identical to its single steps); lbm_neq and the
+                       closure step return LBM_ERR_UNSUPPORTED */
 } lbm_params;
 
 /* ABI version of this header (bumped on any signature change). */

@@ -14,38 +14,13 @@
 // Tile t belongs to the member whose first tile is the largest <= t.  Workgroups are dealt round-robin to the
 // 8 XCDs, so XCD k takes the k-th contiguous eighth of the list (neighbouring rows of a member share an L2), and
 // each workgroup strides through its XCD's share.  Every member ends bit-identical to its solo run.
-#include "lbm_aa_cell.h"
-#include "lbm_common.h"
-#include "lbm_pair_tile.h"
-#include "lbm_quad_tile.h"
+#include "lbm_ensemble_impl.h"
 
 #include <vector>
 
-namespace lbm {
-
-constexpr int ENS_NCH = 4;
-constexpr int ENS_CH[ENS_NCH] = {8, 12, 16, 24}; // tile rows of the two-step list
-constexpr int ENS_NQ = 3;
-constexpr int ENS_QCH[ENS_NQ] = {8, 20, 24}; // tile rows of the four-step lists (128 x 8, 64 x 20, 64 x 24)
-
-struct EnsDesc {
-    KPar P;
-    const uint8_t *mask;
-    void *f[2]; // the state lives in f[side]; f[1] is the second buffer of the two-step path (may be NULL)
-    const float *inlet_u;
-    int tiles_x;
-    int tile0;              // first 128 x 2 tile of this member
-    int ptiles_x;           // 128-column tiles of the two-step list
-    int ptile0[ENS_NCH];    // first 128 x CH tile of this member, per CH
-    int qtiles_x[ENS_NQ];   // columns of tiles of the four-step lists
-    int qtile0[ENS_NQ];     // first four-step tile of this member, per shape
-};
-
-} // namespace lbm
-
 struct lbm_ensemble {
     int n = 0;
-    int fmt = 0, model = 0, bc = 0;
+    int fmt = 0, model = 0, bc = 0, sums = 0;
     std::vector<int> side;  // per member: which of its buffers holds the state
     bool has_alt = false;   // every member has a second buffer
     int default_blocks = 0; // single-step workgroups resident at once on the whole device
@@ -66,114 +41,15 @@ struct lbm_graph {
 
 namespace lbm {
 
-// the member of list tile t among [m0, m1): uniform binary search over the members' first tiles
-template <typename First>
-__device__ __forceinline__ int ens_member(const EnsDesc *__restrict__ d, int m0, int m1, int t, First first)
-{
-    int lo = m0, hi = m1 - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (first(d[mid]) <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
-// One single step of members [m0, m1) whose tiles are [t0, t1) of the k = 1 list, in their buffer `side`.
-template <int MODEL, int FMT, int PARITY, bool PERIODIC>
-__global__ __launch_bounds__(TILE_X *TILE_Y) void k_ensemble(const EnsDesc *__restrict__ d, int m0, int m1, int t0,
-                                                              int t1, int side)
-{
-    const int G = gridDim.x, b = blockIdx.x;
-    const int gper = G / 8;
-    const int xcd = b % 8, l = b / 8;
-    const int per = (t1 - t0 + 7) / 8;
-    const int start = t0 + xcd * per, end = min(start + per, t1);
-    for (int t = start + l; t < end; t += gper) {
-        const EnsDesc &e = d[ens_member(d, m0, m1, t, [](const EnsDesc &x) { return x.tile0; })];
-        const int lt = t - e.tile0;
-        const int ty = lt / e.tiles_x, tx = lt - ty * e.tiles_x;
-        aa_tile_cell<MODEL, FMT, PARITY, PERIODIC, false>(e.P, e.mask, e.f[side], e.inlet_u, nullptr, nullptr, nullptr,
-                                                          tx, ty);
-    }
-}
-
-// Two steps (even then odd) of members [m0, m1) whose tiles are [t0, t1) of the k = 2 list (tile rows
-// ENS_CH[chi] = CH), from buffer `side` into buffer `side ^ 1`.
-template <int MODEL, int FMT, bool PERIODIC, int CH>
-__global__ __launch_bounds__(pairk::NT) void k_ensemble_pair(const EnsDesc *__restrict__ d, int m0, int m1, int t0,
-                                                             int t1, int side, int chi)
-{
-    __shared__ pairk::TileLds<FMT, CH> L;
-    const int G = gridDim.x, b = blockIdx.x;
-    const int gper = G / 8;
-    const int xcd = b % 8, l = b / 8;
-    const int per = (t1 - t0 + 7) / 8;
-    const int start = t0 + xcd * per, end = min(start + per, t1);
-    for (int t = start + l; t < end; t += gper) {
-        const EnsDesc &e = d[ens_member(d, m0, m1, t, [chi](const EnsDesc &x) { return x.ptile0[chi]; })];
-        const int lt = t - e.ptile0[chi];
-        const int ty = lt / e.ptiles_x, tx = lt - ty * e.ptiles_x;
-        pairk::pair_tile_any<MODEL, PERIODIC, FMT, CH>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u, nullptr, nullptr,
-                                                   nullptr, tx, ty, L);
-        __syncthreads(); // the next tile's phase A overwrites the LDS this tile's phase B read
-    }
-}
-
-// Four steps of members [m0, m1) whose tiles are [t0, t1) of the four-step list qi (tile rows ENS_QCH[qi] = CH),
-// from buffer `side` into buffer `side ^ 1`.
-template <int MODEL, bool PERIODIC, int CH>
-__global__ __launch_bounds__(quadk::Shape<CH>::NT) void k_ensemble_quad(const EnsDesc *__restrict__ d, int m0, int m1,
-                                                                        int t0, int t1, int side, int qi)
-{
-    constexpr int CW = quadk::Shape<CH>::CW, NT = quadk::Shape<CH>::NT;
-    __shared__ quadk::QuadLds<FMT_F32, CW, CH, NT> L;
-    const int G = gridDim.x, b = blockIdx.x;
-    const int gper = G / 8;
-    const int xcd = b % 8, l = b / 8;
-    const int per = (t1 - t0 + 7) / 8;
-    const int start = t0 + xcd * per, end = min(start + per, t1);
-    for (int t = start + l; t < end; t += gper) {
-        const EnsDesc &e = d[ens_member(d, m0, m1, t, [qi](const EnsDesc &x) { return x.qtile0[qi]; })];
-        const int lt = t - e.qtile0[qi];
-        const int ty = lt / e.qtiles_x[qi], tx = lt - ty * e.qtiles_x[qi];
-        quadk::quad_tile_any<MODEL, PERIODIC, FMT_F32, CW, CH, NT>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u,
-                                                                    nullptr, nullptr, nullptr, tx, ty, L);
-        __syncthreads(); // the next tile's first phase overwrites the LDS this tile's last phase read
-    }
-}
-
-using EnsKernel = void (*)(const EnsDesc *, int, int, int, int, int);
-using PairKernel = void (*)(const EnsDesc *, int, int, int, int, int, int);
-
-template <int MODEL, bool PERIODIC>
-static PairKernel quad_kernel_c(int qi)
-{
-    switch (qi) {
-    case 0: return k_ensemble_quad<MODEL, PERIODIC, 8>;
-    case 1: return k_ensemble_quad<MODEL, PERIODIC, 20>;
-    default: return k_ensemble_quad<MODEL, PERIODIC, 24>;
-    }
-}
-
 static PairKernel quad_kernel(const lbm_ensemble *e, int qi)
 {
     const bool periodic = e->bc == LBM_BC_PERIODIC;
+    if (e->sums) return quad_kernel_sums(periodic, qi);
     if (e->model == LBM_BGK) return periodic ? quad_kernel_c<MODEL_BGK, true>(qi) : quad_kernel_c<MODEL_BGK, false>(qi);
     return periodic ? quad_kernel_c<MODEL_MRT, true>(qi) : quad_kernel_c<MODEL_MRT, false>(qi);
 }
 
 static int quad_threads(int qi) { return ENS_QCH[qi] == 8 ? quadk::Shape<8>::NT : quadk::Shape<24>::NT; }
-
-template <int MODEL, int FMT>
-static EnsKernel ens_kernel_m(int parity, bool periodic)
-{
-    if (parity)
-        return periodic ? k_ensemble<MODEL, FMT, 1, true> : k_ensemble<MODEL, FMT, 1, false>;
-    return periodic ? k_ensemble<MODEL, FMT, 0, true> : k_ensemble<MODEL, FMT, 0, false>;
-}
 
 template <int FMT>
 static EnsKernel ens_kernel_f(int model, int parity, bool periodic)
@@ -185,20 +61,10 @@ static EnsKernel ens_kernel_f(int model, int parity, bool periodic)
 static EnsKernel ens_kernel(const lbm_ensemble *e, int parity)
 {
     const bool periodic = e->bc == LBM_BC_PERIODIC;
+    if (e->sums) return ens_kernel_sums(e->fmt, parity, periodic);
     if (e->fmt == LBM_FMT_F32) return ens_kernel_f<FMT_F32>(e->model, parity, periodic);
     if (e->fmt == LBM_FMT_F16) return ens_kernel_f<FMT_F16>(e->model, parity, periodic);
     return ens_kernel_f<FMT_BF16>(e->model, parity, periodic);
-}
-
-template <int MODEL, int FMT, bool PERIODIC>
-static PairKernel pair_kernel_c(int chi)
-{
-    switch (chi) {
-    case 0: return k_ensemble_pair<MODEL, FMT, PERIODIC, 8>;
-    case 1: return k_ensemble_pair<MODEL, FMT, PERIODIC, 12>;
-    case 2: return k_ensemble_pair<MODEL, FMT, PERIODIC, 16>;
-    default: return k_ensemble_pair<MODEL, FMT, PERIODIC, 24>;
-    }
 }
 
 template <int FMT>
@@ -212,6 +78,7 @@ static PairKernel pair_kernel_f(int model, bool periodic, int chi)
 static PairKernel pair_kernel(const lbm_ensemble *e, int chi)
 {
     const bool periodic = e->bc == LBM_BC_PERIODIC;
+    if (e->sums) return pair_kernel_sums(e->fmt, periodic, chi);
     if (e->fmt == LBM_FMT_F32) return pair_kernel_f<FMT_F32>(e->model, periodic, chi);
     if (e->fmt == LBM_FMT_F16) return pair_kernel_f<FMT_F16>(e->model, periodic, chi);
     return pair_kernel_f<FMT_BF16>(e->model, periodic, chi);
@@ -281,13 +148,14 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
             delete e;
             return rc;
         }
-        if (d.P.sums) {
+        if (i == 0) e->sums = d.P.sums;
+        if (d.P.sums && e->model != LBM_MRT_LES) {
             delete e;
-            return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_create: compensated sums are single-step (AA) only");
+            return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_create: compensated sums need the MRT model");
         }
-        if (inst[i].p.model != e->model || inst[i].p.bc != e->bc) {
+        if (inst[i].p.model != e->model || inst[i].p.bc != e->bc || d.P.sums != e->sums) {
             delete e;
-            return fail(LBM_ERR_ARG, "lbm_ensemble_create: instance %d differs in model or bc", i);
+            return fail(LBM_ERR_ARG, "lbm_ensemble_create: instance %d differs in model, bc or sums", i);
         }
         if (!inst[i].mask || !inst[i].f || (e->bc == LBM_BC_CHANNEL && !inst[i].inlet_u)) {
             delete e;

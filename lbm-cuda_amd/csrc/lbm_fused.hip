@@ -57,7 +57,6 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
 {
     int rc = make_kpar(p, &P, bytes);
     if (rc) return rc;
-    if (P.sums) return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: compensated sums are single-step (AA) only");
     if (!mask || !f_now || !f_out) return fail(LBM_ERR_ARG, "lbm_step_fused: mask, f_now, f_out must be non-NULL");
     {
         const char *a = static_cast<const char *>(f_now), *b = static_cast<const char *>(f_out);

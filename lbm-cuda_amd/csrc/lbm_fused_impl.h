@@ -325,6 +325,22 @@ static int launch_fused_k(const KPar &P, int k, int lw, int parity, hipStream_t 
     return LBM_OK;
 }
 
+// Launch of the k-step kernel with compensated sums (MODEL_MRT_SUMS; instantiated in lbm_fused_sums.hip).
+template <int FMT>
+int launch_fused_sums_fmt(const KPar &P, int bc, int k, int strip, int parity, hipStream_t st, const uint8_t *mask,
+                          const void *fin, void *fout, const float *inlet_u, int CH, float *ux, float *uy, float *rho)
+#ifdef LBM_FUSED_SUMS_TU
+{
+    return bc == LBM_BC_PERIODIC
+               ? launch_fused_k<MODEL_MRT_SUMS, true, FMT>(P, k, strip, parity, st, mask, fin, fout, inlet_u, CH, ux, uy,
+                                                           rho)
+               : launch_fused_k<MODEL_MRT_SUMS, false, FMT>(P, k, strip, parity, st, mask, fin, fout, inlet_u, CH, ux,
+                                                            uy, rho);
+}
+#else
+    ;
+#endif
+
 // Launch of the k-step kernel for one storage format (instantiated in lbm_fused_<fmt>.hip).
 template <int FMT>
 int launch_fused_fmt(const KPar &P, int model, int bc, int k, int strip, int parity, hipStream_t st,
@@ -337,6 +353,7 @@ int launch_fused_fmt(const KPar &P, int model, int bc, int k, int strip, int par
                                                           rho)
                    : launch_fused_k<MODEL_BGK, false, FMT>(P, k, strip, parity, st, mask, fin, fout, inlet_u, CH, ux,
                                                            uy, rho);
+    if (P.sums) return launch_fused_sums_fmt<FMT>(P, bc, k, strip, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho);
     return per ? launch_fused_k<MODEL_MRT, true, FMT>(P, k, strip, parity, st, mask, fin, fout, inlet_u, CH, ux, uy, rho)
                : launch_fused_k<MODEL_MRT, false, FMT>(P, k, strip, parity, st, mask, fin, fout, inlet_u, CH, ux, uy,
                                                        rho);

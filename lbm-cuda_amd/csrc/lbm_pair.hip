@@ -67,6 +67,11 @@ int launch_pair_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, co
             launch_ch<MODEL_BGK, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
         else
             launch_ch<MODEL_BGK, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+    } else if (P.sums) { // compensated sums read the clamp setting at run time
+        if (per)
+            launch_ch<MODEL_MRT_SUMS, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+        else
+            launch_ch<MODEL_MRT_SUMS, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
     } else if (P.clamp) { // the clamp setting as a template argument: no run-time select per cell
         if (per)
             launch_ch<MODEL_MRT_CLAMP1, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);

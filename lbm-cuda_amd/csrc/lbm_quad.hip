@@ -65,6 +65,11 @@ int launch_quad_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, co
             launch_ch<MODEL_BGK, true, FMT>(P, CH, st, mask, fin, fout, inlet_u, ux, uy, rho);
         else
             launch_ch<MODEL_BGK, false, FMT>(P, CH, st, mask, fin, fout, inlet_u, ux, uy, rho);
+    } else if (P.sums) {
+        if (per)
+            launch_ch<MODEL_MRT_SUMS, true, FMT>(P, CH, st, mask, fin, fout, inlet_u, ux, uy, rho);
+        else
+            launch_ch<MODEL_MRT_SUMS, false, FMT>(P, CH, st, mask, fin, fout, inlet_u, ux, uy, rho);
     } else if (P.clamp) {
         if (per)
             launch_ch<MODEL_MRT_CLAMP1, true, FMT>(P, CH, st, mask, fin, fout, inlet_u, ux, uy, rho);

@@ -466,7 +466,6 @@ int lbm_step_pull(const lbm_params *p, const uint8_t *mask, const float *f_now, 
     KPar P;
     int rc = make_kpar(p, &P, 0);
     if (rc) return rc;
-    if (P.sums) return fail(LBM_ERR_UNSUPPORTED, "lbm_step_pull: compensated sums are single-step (AA) only");
     if (!mask || !f_now || !f_out || !ux || !uy || !rho)
         return fail(LBM_ERR_ARG, "lbm_step_pull: mask, f_now, f_out, ux, uy, rho must be non-NULL");
     if (f_now == f_out) return fail(LBM_ERR_ARG, "lbm_step_pull: f_now and f_out must differ");
@@ -479,6 +478,13 @@ int lbm_step_pull(const lbm_params *p, const uint8_t *mask, const float *f_now, 
             hipLaunchKernelGGL((k_pull<MODEL_BGK, true>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy, rho);
         else
             hipLaunchKernelGGL((k_pull<MODEL_BGK, false>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy, rho);
+    } else if (P.sums) {
+        if (per)
+            hipLaunchKernelGGL((k_pull<MODEL_MRT_SUMS, true>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy,
+                               rho);
+        else
+            hipLaunchKernelGGL((k_pull<MODEL_MRT_SUMS, false>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy,
+                               rho);
     } else {
         if (per)
             hipLaunchKernelGGL((k_pull<MODEL_MRT, true>), grid, block, 0, s, P, mask, f_now, f_out, inlet_u, ux, uy, rho);

@@ -53,7 +53,7 @@ class Simulation:
         self.W, self.H = int(W), int(H)
         self.model = _MODELS[model] if isinstance(model, str) else int(model)
         self.bc = _BCS[bc] if isinstance(bc, str) else int(bc)
-        # sums="neumaier": compensated sums of rho - 1 and j in the MRT collision (single steps only)
+        # sums="neumaier": compensated sums of rho - 1 and j in the MRT collision (every kernel)
         self.params = _lib.params(self.W, self.H, tau_inv, cs, self.model, self.bc, int(bool(clamp)), s_e, s_eps,
                                   s_q, {"plain": 0, "neumaier": 1}[sums])
         self.device = torch.device(device)

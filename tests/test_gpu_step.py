@@ -295,13 +295,3 @@ def test_aa_step_compensated_sums_bitwise(gpu, periodic):
         _check_steps_bitwise(gpu, comp, 4)
         plain.step(4)
         assert not np.array_equal(plain.f.cpu().numpy(), comp.f.cpu().numpy())
-
-
-def test_compensated_sums_single_step_only(gpu):
-    from lbm import _lib
-    sim = _small_channel(sums="neumaier")
-    with pytest.raises(_lib.LbmError):
-        sim.step(2, k=2)
-    from lbm.ensemble import Ensemble
-    with pytest.raises(_lib.LbmError):
-        Ensemble([_small_periodic(sums="neumaier")])
