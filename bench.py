@@ -405,6 +405,73 @@ def copy_ceiling(nbytes, dev, reps=20):
     return out
 
 
+def rmw_ceiling(W, H, dev, reps=20):
+    """The single step's own memory pattern without the collision (lbm_rmw_probe: same tiles, same XCD order,
+    same 4-byte accesses to the 9 planes, in place): even-step pattern (each cell's own slots) and odd-step
+    pattern (the planes shifted by their links), GB/s of 72 B per cell, best of three batches; `pair` is the
+    rate of one even plus one odd sweep."""
+    import torch
+    from lbm import _lib
+    L = _lib.load()
+    buf = torch.ones(9 * W * H, dtype=torch.float32, device=dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    us = {}
+    for name, shift in (("even", 0), ("odd", 1)):
+        def fn(shift=shift):
+            _lib.check(L.lbm_rmw_probe(ctypes.c_void_p(buf.data_ptr()), W, H, 1, shift, ctypes.c_float(1.0), st),
+                       "lbm_rmw_probe")
+        fn()
+        best = None
+        for _ in range(3):
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1) * 1e3 / reps
+            best = t if best is None else min(best, t)
+        us[name] = best
+    del buf
+    torch.cuda.empty_cache()
+    nb = BYTES_PER_CELL_FP32 * W * H
+    return {"W": W, "H": H, "even_gbs": round(nb / us["even"] / 1e3, 1), "odd_gbs": round(nb / us["odd"] / 1e3, 1),
+            "pair_gbs": round(2 * nb / (us["even"] + us["odd"]) / 1e3, 1)}
+
+
+def plain_beyond_mall(dev, W=8192, H=2048, pairs=30):
+    """The plain FP32 single step on a lattice whose 604 MB state exceeds the 256 MB MALL (the cylinder case
+    rescaled to W x H): 2 x pairs steps timed as one batch with HIP events, best of three batches, against
+    rmw_ceiling at the same size (timed the same way) and torch's in-place multiply of the same bytes."""
+    import torch
+    from lbm import configs
+    from lbm.sim import Simulation
+    case = configs.cylinder_scaled(W, H)
+    sim = Simulation(W, H, case.tau_inv, **case.sim_kwargs(), device=dev)
+    sim.step(200)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = None
+    for _ in range(3):
+        torch.cuda.synchronize()
+        e0.record()
+        sim.step(2 * pairs)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / (2 * pairs)
+        best = us if best is None else min(best, us)
+    del sim
+    torch.cuda.empty_cache()
+    cells = W * H
+    ceil = rmw_ceiling(W, H, dev)
+    gbs = BYTES_PER_CELL_FP32 * cells / best / 1e3
+    return {"W": W, "H": H, "state_mb": round(BYTES_PER_CELL_FP32 / 2 * cells / 1e6, 1),
+            "glups": round(cells / best / 1e3, 2), "us_per_step": round(best, 2), "achieved_gbs": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "ceiling": ceil,
+            "frac_of_ceiling": round(gbs / ceil["pair_gbs"], 4),
+            "torch_inplace_rmw_gbs": copy_ceiling(BYTES_PER_CELL_FP32 // 2 * cells, dev)["inplace_rmw_gbs"]}
+
+
 # C2 checkpoints at which every storage is compared with the FP64 oracle (steps from the seeded start; all
 # multiples of 8, so every k of the tuned forms divides them) and the C3 full-size checkpoints
 L2_MARKS_C2 = (1000, 2000, 5000, 10000)
@@ -566,7 +633,7 @@ def main():
                          "8192x2048 shear layer at 1000/2000 steps (FP64 oracle in a background thread), "
                          "'short' = the cylinder at 1000 steps only")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=("all", "aa", "fused", "ensemble", "closure", "precision", "forces"),
+    ap.add_argument("--only", choices=("all", "aa", "fused", "ensemble", "closure", "precision", "forces", "mall"),
                     default="all", help="run only one part (for profiling)")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="timed steps of the 64-lattice batched ensemble (0 = skip)")
@@ -576,6 +643,8 @@ def main():
                     help="fine lattice size of the learned-closure study (coarse = n / 8; 0 = skip)")
     ap.add_argument("--force-steps", type=int, default=100000,
                     help="length of the extended drag/lift record beyond the case's 10000 steps (0 = the case only)")
+    ap.add_argument("--no-beyond-mall", dest="beyond_mall", action="store_false",
+                    help="skip the plain FP32 step at 8192x2048 (state beyond the 256 MB MALL) and its ceiling")
     ap.add_argument("--copy-probe", default="", metavar="MIB[,MIB...]",
                     help="first copy device buffers of these many MiB (known-size kernels for counter calibration)")
     args = ap.parse_args()
@@ -612,6 +681,9 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: {mib} MiB read + {mib} MiB written")
         del src, dst
+    if args.only == "mall":
+        print(json.dumps({"plain_fp32_beyond_mall": plain_beyond_mall(dev, pairs=args.steps // 2 or 30)}), flush=True)
+        return
     case = configs.cylinder_large()
     W, H = case.W, case.H
     cells = W * H
@@ -712,6 +784,14 @@ def main():
                          "storage": "f32", "k": 1}}
     fluid = int(np.count_nonzero(updated_cells(sim)))
     ceiling = copy_ceiling(BYTES_PER_CELL_FP32 // 2 * cells, dev) if args.only != "fused" else None
+    rmw = rmw_ceiling(W, H, dev) if args.only != "fused" else None
+    if rmw:
+        log(f"single-step pattern ceiling {W}x{H}: even {rmw['even_gbs']:.0f}, odd {rmw['odd_gbs']:.0f}, "
+            f"pair {rmw['pair_gbs']:.0f} GB/s; plain step at {achieved / rmw['pair_gbs']:.1%} of it")
+    beyond = None
+    if args.only in ("all", "aa") and args.beyond_mall:
+        beyond = plain_beyond_mall(dev)
+        log(f"plain fp32 beyond the MALL: {beyond}")
 
     # ---------------- the other forms: 16-bit deviation storage, and the fused kernels (autotuned)
     def time_steps(fn, nsteps):
@@ -864,7 +944,10 @@ def main():
         "plain_fp32": {"glups": round(glups, 3), "us_per_step": round(kern_us, 2), "glups_wall": round(glups_wall, 3),
                        "wall_ms_per_step": round(wall * 1e3 / args.steps, 5),
                        "fluid_glups": round(glups * fluid / cells, 3),
-                       "frac_of_copy": (round(achieved / ceiling["inplace_rmw_gbs"], 4) if ceiling else None)},
+                       "frac_of_copy": (round(achieved / ceiling["inplace_rmw_gbs"], 4) if ceiling else None),
+                       "frac_of_rmw_ceiling": (round(achieved / rmw["pair_gbs"], 4) if rmw else None)},
+        "rmw_ceiling": rmw,
+        "plain_fp32_beyond_mall": beyond,
         "x_plain": round(ch["glups"] / glups, 4),
         "copy_ceiling": ceiling,
         "fluid_cells": fluid,

@@ -359,6 +359,15 @@ int lbm_step_closure(const lbm_params *p, const uint8_t *mask, float *f, int par
 int lbm_selftest(int what, uint32_t *bad, uint32_t *first, lbm_stream_t stream);
 int lbm_selftest_threads(void);
 
+/*
+ * Bandwidth ceiling of the in-place single step's memory pattern (no reference counterpart: a measurement
+ * kernel).  `buf` holds 9 planes of W * H FP32 values.  In the single step's tiles and XCD order, every thread
+ * reads `width` (1, 2 or 4) consecutive values of each plane through one buffer resource, multiplies them by s
+ * and writes them back in place.  shift != 0 moves plane q's window by the lattice link (c_qx + c_qy * W), the
+ * odd step's neighbour pattern (unaligned for width > 1).  W must be a multiple of width; 36 * W * H < 2^32.
+ */
+int lbm_rmw_probe(float *buf, int W, int H, int width, int shift, float s, lbm_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,7 @@ struct KPar {
     int clamp; // MRT: velocity and density clamps
     float tau_inv, tau0, tau0sq, c18cs2, s_e, s_eps, s_q;
     int sums; // lbm_params.sums: compensated sums (MODEL_MRT_SUMS instantiations)
+    int rowmap; // single step: the state (9 W H population bytes) exceeds the 256 MB MALL -> xcd_row_tile order
 };
 
 // The reference's FP32 literals (lbm_core4.cu:15-16,55-57; lbm_core_smemaa.cu:4-7)

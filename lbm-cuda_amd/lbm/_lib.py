@@ -102,6 +102,7 @@ SIGNATURES = {
     "lbm_step_fused_n": (_i, [_P, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
     "lbm_step_fused_h16_n": (_i, [_P, _vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp, _vp]),
     "lbm_selftest_threads": (_i, []),
+    "lbm_rmw_probe": (_i, [_vp, _i, _i, _i, _i, ctypes.c_float, _vp]),
     "lbm_graph_begin": (_i, [_vp]),
     "lbm_graph_end": (_i, [_vp, _vp]),
     "lbm_graph_launch": (_i, [_vp, _vp]),

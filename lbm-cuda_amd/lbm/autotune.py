@@ -3,9 +3,9 @@
 The reference fixes its tile (SQUARE_A = 24) and T at compile time and mirrors them by hand in the driver
 (lbm_smemaa_main.py:93-95) -- its compile-time traversal-order table (lbm_core_smemaa_policy.cu:12-24) is
 the only tuning knob.  Here every candidate is timed on the GPU with HIP events on scratch copies of the
-state (the simulation itself is not advanced) and the winner is cached per lattice size class.  For k = 2
-from an even state the candidates include the two-step tile kernel (strip 0), and for k = 4 with FP32 populations
-the four-step tile kernel.
+state (the simulation itself is not advanced) and the winner is cached per lattice size class.  From an even
+state the candidates are the tile kernels (strip 0): the two-step kernel for k = 2 and, for FP32 populations, the
+four-step kernel for k = 4; the strip sweep is offered from odd states only.
 """
 from __future__ import annotations
 
@@ -95,14 +95,18 @@ def time_config(sim, k, cfg, reps=10, bufs=None):
 
 
 def candidates(sim, k):
-    """(strip, chunk) pairs to time: strip-sweep shapes, plus the tile kernels (strip 0, chunk = tile rows) from
-    an even state: the two-step kernel for k = 2, the four-step kernel for k = 4 (FP32 populations)."""
+    """(strip, chunk) pairs to time.  From an even state: the tile kernels only (strip 0, chunk = tile rows) --
+    the two-step kernel for k = 2, the four-step kernel for k = 4 (FP32 populations) -- and nothing for other k:
+    on 4096 x 1024 the strip sweep ran 61-85 us per step against 20-36 us for the tile kernels and 34-45 us for
+    single steps, so it cannot win there.  From an odd state (no tile kernel applies): strip-sweep shapes."""
     H = sim.params.H
+    if sim.parity == 0:
+        if k == 2:
+            return [(0, ch) for ch in (8, 12, 16, 24)]
+        if k == 4 and sim.fmt == _lib.FMT_F32:
+            return [(0, ch) for ch in (8, 12, 16, 20, 24, 32)]
+        return []
     out = []
-    if sim.parity == 0 and k == 2:
-        out = [(0, ch) for ch in (8, 12, 16, 24)]
-    elif sim.parity == 0 and k == 4 and sim.fmt == _lib.FMT_F32:
-        out = [(0, ch) for ch in (8, 12, 16, 20, 24, 32)]
     for strip in (STRIPS if sim.fmt == _lib.FMT_F32 else (128,)):
         if strip - 2 * (k + 1) < 16:
             continue
@@ -125,7 +129,10 @@ def tune_fused(sim, ks=KS, use_cache=True, reps=10, verbose=False):
             table.append((k, strip, chunk, us))
             continue
         best = None
-        for cfg in candidates(sim, k):
+        cands = candidates(sim, k)
+        if not cands:
+            continue
+        for cfg in cands:
             us = time_config(sim, k, cfg, reps=reps, bufs=bufs)
             if verbose:
                 print(f"  k={k} strip={cfg[0]} chunk={cfg[1]}: {us:.2f} us/step", flush=True)
@@ -135,6 +142,8 @@ def tune_fused(sim, ks=KS, use_cache=True, reps=10, verbose=False):
         table.append((k,) + tuple(best))
     if use_cache:
         save_cache(cache)
+    if not table:
+        raise ValueError(f"no fused launch shape for k in {tuple(ks)} from parity {sim.parity}")
     best = min(table, key=lambda r: r[3])
     del bufs
     return best, table

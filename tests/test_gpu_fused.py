@@ -81,6 +81,21 @@ def test_autotuner_picks_a_valid_config(gpu, tmp_path, monkeypatch):
     assert best2 == best  # served from the cache
 
 
+def test_autotuner_candidates_by_parity(gpu):
+    """Even states: tile kernels only (k = 2; k = 4 for FP32), none for other k; odd states: strip shapes."""
+    from lbm import autotune
+    _, sim = _pair(512, 256, True, "mrt")
+    assert sim.parity == 0
+    assert {s for s, _ in autotune.candidates(sim, 2)} == {0}
+    assert {s for s, _ in autotune.candidates(sim, 4)} == {0}
+    assert autotune.candidates(sim, 6) == []
+    sim.step(1)
+    odd = autotune.candidates(sim, 4)
+    assert odd and all(s in (64, 128) for s, _ in odd)
+    best, table = autotune.tune_fused(sim, ks=(2, 4), use_cache=False, reps=2)
+    assert best[1] in (64, 128) and len(table) == 2
+
+
 def test_fused_macros_match_single(gpu):
     single, fusedsim = _pair(200, 64, False, "mrt")
     single.step(3)
