@@ -7,8 +7,9 @@ Workload: channel flow past a cylinder, 4096 x 1024, R = 64 at (1024, 512), Re =
 parabolic inlet u_max = 0.05 (synthetic initial state: feq at rho = 1, u = inlet profile, seeded noise).
 A "step" is one collide-stream update of all W*H cells with the state resident in HBM.  The flow is first
 developed by --spinup untimed FP32 steps (default 3000) and every form is timed from that state.  Timed
-region: K steps bracketed by torch.cuda.synchronize(); HIP events on the launch stream give the per-launch
-kernel time used for the roofline (72 B/cell/step algorithmic traffic for FP32 single steps).
+region: exactly K steps, recorded beforehand as a graph and replayed once, bracketed by torch.cuda.synchronize();
+HIP events on the replay stream give the per-step kernel time used for the roofline (72 B/cell/step algorithmic
+traffic for FP32 single steps), the host clock around the same replay the wall time.
 The CPU baseline is the oracle's FP32 mirror (C, OpenMP over rows) on a bounded number of steps of the
 same lattice, timed on this host.  The "ensemble" object reports the batched workload of 64 periodic
 lattices of uneven sizes (one persistent launch per step vs 64 separate launches per step).
@@ -476,8 +477,9 @@ def plain_beyond_mall(dev, W=8192, H=2048, pairs=30):
 # multiples of 8, so every k of the tuned forms divides them) and the C3 full-size checkpoints
 L2_MARKS_C2 = (1000, 2000, 5000, 10000)
 L2_MARKS_C3 = (1000, 2000)
-# untimed launches of each form right after it is chosen and before its warm-up and timed run, whatever
-# --warmup is: the clocks take tens of milliseconds to settle after the tuner's heavy candidates
+# untimed steps of each fused form, with its --warmup steps, right before its timed run: the tuner leaves the
+# clocks unsettled for tens of milliseconds, and a GPU that idles before the timed region runs its first
+# milliseconds slower
 SETTLE_STEPS = 2000
 
 
@@ -764,17 +766,42 @@ def main():
     fused_steps, fused_warmup = args.steps, args.warmup
     if args.only == "fused":  # the plain step only as the reference rate: a short run
         args.warmup, args.steps = 2, 20
-    # ---------------- plain FP32 single AA steps: HIP events on the launch stream around exactly K steps
-    run_aa(args.warmup)
-    torch.cuda.synchronize()
+    from lbm.ensemble import Graph
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record()
-    run_aa(args.steps)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_us = e0.elapsed_time(e1) * 1e3 / args.steps
+
+    def timed_replay(lat, fn, nsteps, pre_steps, unit):
+        """Exactly `nsteps` steps of lattice `lat` (advanced by fn(n)) timed as one replay of a graph that
+        recorded fn(nsteps) beforehand, HIP events on the replay stream around the replay, bracketed by
+        synchronizations: the Python and ctypes work of issuing the launches stays out of the timed region, as
+        in the reference driver's replayed capture (lbm_smemaa_main.py:136-151,173-174).  The recording runs
+        nothing; it comes first, then `pre_steps` untimed steps (rounded up to a whole even number of launches
+        of `unit` steps, so that the state is back in the buffer and at the parity the graph starts from) keep
+        the GPU busy up to the timed region.  Returns (us per step by events, us per step by the host clock)."""
+        pre = -(-max(pre_steps, 1) // (2 * unit)) * 2 * unit
+        saved = (lat.f, lat.f_alt, lat.parity, lat.steps)
+        g = Graph()
+        with g.capture():
+            fn(nsteps)
+        after = (lat.f, lat.f_alt, lat.parity, lat.steps)
+        lat.f, lat.f_alt, lat.parity, lat.steps = saved
+        fn(pre)
+        if lat.f is not saved[0] or lat.parity != saved[2]:
+            raise RuntimeError("untimed steps did not return the state to the recorded buffer and parity")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        wall_us = (time.perf_counter() - t0) * 1e6 / nsteps
+        g.close()
+        lat.f, lat.f_alt, lat.parity = after[:3]
+        lat.steps = after[3] + pre
+        return e0.elapsed_time(e1) * 1e3 / nsteps, wall_us
+
+    # ---------------- plain FP32 single AA steps: exactly K steps, one graph replay
+    kern_us, wall_us = timed_replay(sim, run_aa, args.steps, args.warmup, 1)
+    wall = wall_us * args.steps / 1e6
     glups = cells / kern_us / 1e3
     glups_wall = cells / (wall * 1e6 / args.steps) / 1e3
     achieved = BYTES_PER_CELL_FP32 * cells / kern_us / 1e3
@@ -794,14 +821,6 @@ def main():
         log(f"plain fp32 beyond the MALL: {beyond}")
 
     # ---------------- the other forms: 16-bit deviation storage, and the fused kernels (autotuned)
-    def time_steps(fn, nsteps):
-        torch.cuda.synchronize()
-        e0.record()
-        fn(nsteps)
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) * 1e3 / nsteps
-
     def make(storage):
         return Simulation(W, H, case.tau_inv, **case.sim_kwargs(), device=dev, storage=storage)
 
@@ -827,8 +846,7 @@ def main():
                 _lib.check(rc, "lbm_step_aa_h16_n")
                 s16.parity = (s16.parity + n) & 1
 
-            run16(args.warmup)
-            us = time_steps(run16, args.steps)
+            us = timed_replay(s16, run16, args.steps, args.warmup, 1)[0]
             forms[f"aa_{storage}"] = {"glups": cells / us / 1e3, "us_per_step": us, "bytes_per_cell": 36,
                                       "achieved_gbs": 36 * cells / us / 1e3,
                                       "frac": 36 * cells / us / 1e3 / HBM_PEAK_GBS,
@@ -851,9 +869,8 @@ def main():
                 k, strip, chunk, _ = best
             cfg = (strip, chunk)
             nlaunch = max(1, fused_steps // k)
-            sf.step((SETTLE_STEPS // k) * k, k=k, cfg=cfg)
-            sf.step(max(1, fused_warmup // k) * k, k=k, cfg=cfg)
-            us = time_steps(lambda n, sf=sf, k=k, cfg=cfg: sf.step(n, k=k, cfg=cfg), nlaunch * k)
+            us = timed_replay(sf, lambda n, sf=sf, k=k, cfg=cfg: sf.step(n, k=k, cfg=cfg), nlaunch * k,
+                              SETTLE_STEPS + fused_warmup, k)[0]
             name = f"fused_{storage}"
             kernel = ("k_pair" if k == 2 else "k_quad") if strip == 0 else "k_fused"
             forms[name] = {"glups": cells / us / 1e3, "us_per_step": us, "k": k, "strip": strip, "chunk": chunk,
