@@ -270,7 +270,7 @@ int lbm_ensemble_pair_info(lbm_ensemble *e, int chunk, int *total_tiles, int *de
  * an ensemble can run with its own kernel and tile shape:
  *   k = 1  single steps in place from `parity` (chunk ignored);
  *   k = 2  the two-step kernel (chunk = tile rows 8, 12, 16 or 24) from an even state, nsteps a multiple of 2;
- *   k = 4  the four-step kernel of lbm_step_fused with strip 0 (chunk 8: 128 x 8 tiles, 20 or 24: 64-wide tiles;
+ *   k = 4  the four-step kernel of lbm_step_fused with strip 0 (chunk 8: 128 x 8 tiles, 12, 20 or 24: 64-wide tiles;
  *          FP32 populations) from an even state, nsteps a multiple of 4.
  * The members are grouped for the cache as in lbm_ensemble_step_n (group_bytes) and every group runs all its
  * launches before the next.  The range's members must hold their states on the same side (the buffer

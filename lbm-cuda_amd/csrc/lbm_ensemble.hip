@@ -25,10 +25,10 @@ struct lbm_ensemble {
     bool has_alt = false;   // every member has a second buffer
     int default_blocks = 0; // single-step workgroups resident at once on the whole device
     int pair_blocks[lbm::ENS_NCH] = {0, 0, 0, 0}; // the same for the two-step kernels (0 = not yet computed)
-    int quad_blocks[lbm::ENS_NQ] = {0, 0, 0};     // and for the four-step kernels
+    int quad_blocks[lbm::ENS_NQ] = {0, 0, 0, 0};     // and for the four-step kernels
     int total_tiles = 0;
     int total_ptiles[lbm::ENS_NCH] = {0, 0, 0, 0};
-    int total_qtiles[lbm::ENS_NQ] = {0, 0, 0};
+    int total_qtiles[lbm::ENS_NQ] = {0, 0, 0, 0};
     long long total_cells = 0;
     lbm::EnsDesc *d_desc = nullptr;
     std::vector<lbm::EnsDesc> h_desc;
@@ -49,7 +49,7 @@ static PairKernel quad_kernel(const lbm_ensemble *e, int qi)
     return periodic ? quad_kernel_c<MODEL_MRT, true>(qi) : quad_kernel_c<MODEL_MRT, false>(qi);
 }
 
-static int quad_threads(int qi) { return ENS_QCH[qi] == 8 ? quadk::Shape<8>::NT : quadk::Shape<24>::NT; }
+static int quad_threads(int qi) { return quadk::quad_nt(ENS_QCH[qi]); }
 
 template <int FMT>
 static EnsKernel ens_kernel_f(int model, int parity, bool periodic)
@@ -140,7 +140,7 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
     e->has_alt = true;
     e->h_desc.resize(n);
     e->side.assign(n, 0);
-    long long tiles = 0, ptiles[ENS_NCH] = {0, 0, 0, 0}, qtiles[ENS_NQ] = {0, 0, 0};
+    long long tiles = 0, ptiles[ENS_NCH] = {0, 0, 0, 0}, qtiles[ENS_NQ] = {0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         EnsDesc &d = e->h_desc[i];
         int rc = make_kpar(&inst[i].p, &d.P, fmt == LBM_FMT_F32 ? 4u : 2u);
@@ -179,14 +179,17 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
             ptiles[c] += (long long)d.ptiles_x * ((d.P.H + ENS_CH[c] - 1) / ENS_CH[c]);
         }
         for (int c = 0; c < ENS_NQ; ++c) {
-            const int cw = ENS_QCH[c] <= 16 ? 128 : 64;
+            const int cw = quadk::quad_cw(ENS_QCH[c]);
             d.qtiles_x[c] = (d.P.W + cw - 1) / cw;
             d.qtile0[c] = (int)qtiles[c];
             qtiles[c] += (long long)d.qtiles_x[c] * ((d.P.H + ENS_QCH[c] - 1) / ENS_QCH[c]);
         }
         e->total_cells += (long long)d.P.plane;
     }
-    if (tiles >= (1LL << 31) || qtiles[0] >= (1LL << 31) || ptiles[0] >= (1LL << 31)) {
+    bool too_many = tiles >= (1LL << 31);
+    for (int c = 0; c < ENS_NCH; ++c) too_many = too_many || ptiles[c] >= (1LL << 31);
+    for (int c = 0; c < ENS_NQ; ++c) too_many = too_many || qtiles[c] >= (1LL << 31);
+    if (too_many) {
         delete e;
         return fail(LBM_ERR_SIZE, "lbm_ensemble_create: too many tiles");
     }
@@ -282,7 +285,7 @@ int lbm_ensemble_run_info(lbm_ensemble *e, int k, int chunk, int *total_tiles, i
         blocks = e->pair_blocks[chi];
     } else if (k == 4) {
         const int qi = quad_qi(chunk);
-        if (qi < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: k = 4 needs chunk 8, 20 or 24");
+        if (qi < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: k = 4 needs chunk 8, 12, 20 or 24");
         if (e->fmt != LBM_FMT_F32)
             return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_run_info: k = 4 takes FP32 populations only");
         int rc = quad_default(e, qi, "lbm_ensemble_run_info");

@@ -12,8 +12,8 @@ namespace lbm {
 
 constexpr int ENS_NCH = 4;
 constexpr int ENS_CH[ENS_NCH] = {8, 12, 16, 24}; // tile rows of the two-step list
-constexpr int ENS_NQ = 3;
-constexpr int ENS_QCH[ENS_NQ] = {8, 20, 24}; // tile rows of the four-step lists (128 x 8, 64 x 20, 64 x 24)
+constexpr int ENS_NQ = 4;
+constexpr int ENS_QCH[ENS_NQ] = {8, 12, 20, 24}; // tile rows of the four-step lists (128 x 8, 64 x 12, 64 x 20, 64 x 24)
 
 struct EnsDesc {
     KPar P;
@@ -87,26 +87,38 @@ __global__ __launch_bounds__(pairk::NT) void k_ensemble_pair(const EnsDesc *__re
 
 // Four steps of members [m0, m1) whose tiles are [t0, t1) of the four-step list qi (tile rows ENS_QCH[qi] = CH),
 // from buffer `side` into buffer `side ^ 1`.
+#define LBM_ENS_QUAD_BODY                                                                                          \
+    constexpr int CW = quadk::Shape<CH>::CW, NT = quadk::Shape<CH>::NT;                                           \
+    __shared__ quadk::QuadLds<FMT_F32, CW, CH, NT> L;                                                             \
+    const int G = gridDim.x, b = blockIdx.x;                                                                      \
+    const int gper = G / 8;                                                                                       \
+    const int xcd = b % 8, l = b / 8;                                                                             \
+    const int per = (t1 - t0 + 7) / 8;                                                                            \
+    const int start = t0 + xcd * per, end = min(start + per, t1);                                                 \
+    for (int t = start + l; t < end; t += gper) {                                                                 \
+        const EnsDesc &e = d[ens_member(d, m0, m1, t, [qi](const EnsDesc &x) { return x.qtile0[qi]; })];         \
+        const int lt = t - e.qtile0[qi];                                                                          \
+        const int ty = lt / e.qtiles_x[qi], tx = lt - ty * e.qtiles_x[qi];                                        \
+        quadk::quad_tile_any<MODEL, PERIODIC, FMT_F32, CW, CH, NT>(e.P, e.mask, e.f[side], e.f[side ^ 1],          \
+                                                                    e.inlet_u, nullptr, nullptr, nullptr, tx, ty, L); \
+        __syncthreads(); /* the next tile's first phase overwrites the LDS this tile's last phase read */         \
+    }
+
 template <int MODEL, bool PERIODIC, int CH>
 __global__ __launch_bounds__(quadk::Shape<CH>::NT) void k_ensemble_quad(const EnsDesc *__restrict__ d, int m0, int m1,
                                                                         int t0, int t1, int side, int qi)
 {
-    constexpr int CW = quadk::Shape<CH>::CW, NT = quadk::Shape<CH>::NT;
-    __shared__ quadk::QuadLds<FMT_F32, CW, CH, NT> L;
-    const int G = gridDim.x, b = blockIdx.x;
-    const int gper = G / 8;
-    const int xcd = b % 8, l = b / 8;
-    const int per = (t1 - t0 + 7) / 8;
-    const int start = t0 + xcd * per, end = min(start + per, t1);
-    for (int t = start + l; t < end; t += gper) {
-        const EnsDesc &e = d[ens_member(d, m0, m1, t, [qi](const EnsDesc &x) { return x.qtile0[qi]; })];
-        const int lt = t - e.qtile0[qi];
-        const int ty = lt / e.qtiles_x[qi], tx = lt - ty * e.qtiles_x[qi];
-        quadk::quad_tile_any<MODEL, PERIODIC, FMT_F32, CW, CH, NT>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u,
-                                                                    nullptr, nullptr, nullptr, tx, ty, L);
-        __syncthreads(); // the next tile's first phase overwrites the LDS this tile's last phase read
-    }
+    LBM_ENS_QUAD_BODY
 }
+
+// the shapes whose LDS fits three workgroups per CU: registers for six waves per SIMD
+template <int MODEL, bool PERIODIC, int CH>
+__global__ __launch_bounds__(quadk::Shape<CH>::NT) __attribute__((amdgpu_waves_per_eu(6))) void k_ensemble_quad3(
+    const EnsDesc *__restrict__ d, int m0, int m1, int t0, int t1, int side, int qi)
+{
+    LBM_ENS_QUAD_BODY
+}
+#undef LBM_ENS_QUAD_BODY
 
 using EnsKernel = void (*)(const EnsDesc *, int, int, int, int, int);
 using PairKernel = void (*)(const EnsDesc *, int, int, int, int, int, int);
@@ -114,9 +126,11 @@ using PairKernel = void (*)(const EnsDesc *, int, int, int, int, int, int);
 template <int MODEL, bool PERIODIC>
 inline PairKernel quad_kernel_c(int qi)
 {
+    static_assert(ENS_QCH[1] == 12 && quadk::Shape<12>::THREE, "four-step list 1 is the three-workgroup shape");
     switch (qi) {
     case 0: return k_ensemble_quad<MODEL, PERIODIC, 8>;
-    case 1: return k_ensemble_quad<MODEL, PERIODIC, 20>;
+    case 1: return k_ensemble_quad3<MODEL, PERIODIC, 12>;
+    case 2: return k_ensemble_quad<MODEL, PERIODIC, 20>;
     default: return k_ensemble_quad<MODEL, PERIODIC, 24>;
     }
 }

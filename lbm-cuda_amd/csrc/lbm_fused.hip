@@ -20,8 +20,8 @@ int launch_quad_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, co
                     void *fout, const float *inlet_u, float *ux, float *uy, float *rho);
 
 // strip = 0 selects a tile kernel from an even state -- the two-step kernel for k = 2 (chunk = 8, 12, 16 or 24
-// tile rows), the four-step kernel for k = 4 (chunk = 8, 12 or 16 rows of 128-wide tiles, 20, 24 or 32 rows of
-// 64-wide ones; FP32 populations); otherwise the strip-sweep kernel
+// tile rows), the four-step kernel for k = 4 (chunk = 8 or 16 rows of 128-wide tiles, 12, 14, 20, 24 or 32 rows
+// of 64-wide ones; FP32 populations); otherwise the strip-sweep kernel
 template <int FMT>
 static int launch_any(const KPar &P, const lbm_params *p, int k, const lbm_fused_cfg &c, int parity, hipStream_t st,
                       const uint8_t *mask, const void *fin, void *fout, const float *inlet_u, float *ux, float *uy,
@@ -78,9 +78,9 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
                         "lbm_step_fused: the tile kernels (strip 0) need k = 2 or 4 and parity 0");
         if (k == 2 && c.chunk != 8 && c.chunk != 12 && c.chunk != 16 && c.chunk != 24)
             return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 with k = 2 needs chunk 8, 12, 16 or 24");
-        if (k == 4 && c.chunk != 8 && c.chunk != 12 && c.chunk != 16 && c.chunk != 20 && c.chunk != 24 &&
-            c.chunk != 32)
-            return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 with k = 4 needs chunk 8, 12, 16, 20, 24 or 32");
+        if (k == 4 && c.chunk != 8 && c.chunk != 12 && c.chunk != 14 && c.chunk != 16 && c.chunk != 20 &&
+            c.chunk != 24 && c.chunk != 32)
+            return fail(LBM_ERR_ARG, "lbm_step_fused: strip 0 with k = 4 needs chunk 8, 12, 14, 16, 20, 24 or 32");
         if (k == 4 && bytes != 4)
             return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: the four-step tile kernel stores FP32 populations only");
         return LBM_OK;

@@ -28,11 +28,17 @@ namespace quadk {
 
 constexpr int R = 3; // halo rings
 
-// Tile shapes by core rows: 8, 12 and 16 rows of 128-wide tiles, 20, 24 and 32 rows of 64-wide ones; 512-thread
-// workgroups where two fit a CU's LDS with FP32 populations (8, 20, 24 rows), 1024 threads otherwise.
+// Tile shapes by core rows CH: 128 x CH for 8 and 16 rows, 64 x CH for 12, 14, 20, 24 and 32; 1024-thread
+// workgroups for 16 and 32 rows, 512 otherwise.  With FP32 populations 12 and 14 rows (46.6 and 51.8 KB of LDS)
+// fit three workgroups per CU, 8, 20 and 24 rows two, 16 and 32 rows one.  Host code maps a run-time chunk with
+// the same functions.
+__host__ __device__ constexpr int quad_cw(int ch) { return ch == 8 || ch == 16 ? 128 : 64; }
+__host__ __device__ constexpr int quad_nt(int ch) { return ch == 16 || ch == 32 ? 1024 : 512; }
+__host__ __device__ constexpr bool quad_three(int ch) { return ch == 12 || ch == 14; }
 template <int CH> struct Shape {
-    static constexpr int CW = CH <= 16 ? 128 : 64;
-    static constexpr int NT = (CH == 8 || CH == 20 || CH == 24) ? 512 : 1024;
+    static constexpr int CW = quad_cw(CH);
+    static constexpr int NT = quad_nt(CH);
+    static constexpr bool THREE = quad_three(CH); // three workgroups per CU: at most 85 VGPRs (6 waves per SIMD)
 };
 
 // The LDS image of a CW x CH tile's region, (CW + 6) x (CH + 6) cells.  The row stride is the region width, so

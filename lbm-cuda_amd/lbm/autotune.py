@@ -104,7 +104,7 @@ def candidates(sim, k):
         if k == 2:
             return [(0, ch) for ch in (8, 12, 16, 24)]
         if k == 4 and sim.fmt == _lib.FMT_F32:
-            return [(0, ch) for ch in (8, 12, 16, 20, 24, 32)]
+            return [(0, ch) for ch in (8, 12, 14, 16, 20, 24, 32)]
         return []
     out = []
     for strip in (STRIPS if sim.fmt == _lib.FMT_F32 else (128,)):

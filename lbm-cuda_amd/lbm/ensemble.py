@@ -127,7 +127,7 @@ class Ensemble:
 
     ``two_step=True`` gives every member a second buffer (``Simulation.f_alt``, allocated if missing) for
     ``step(n, k=2)`` and ``step(n, k=4)``; the ensemble then swaps each member's ``f`` and ``f_alt`` so that
-    ``f`` always holds the state.  ``k`` and ``chunk`` (tile rows: 8/12/16/24 for k = 2; 8/20/24 for k = 4, FP32
+    ``f`` always holds the state.  ``k`` and ``chunk`` (tile rows: 8/12/16/24 for k = 2; 8/12/20/24 for k = 4, FP32
     populations) are the defaults of ``step``.
 
     ``plan`` (set by ``tune_classes``) gives contiguous member ranges -- the size classes, see
@@ -364,7 +364,7 @@ GROUPS_MB = (0, 32, 64, 128, 192)
 PAIR_CHUNKS = (8, 12, 16, 24)
 
 
-QUAD_CHUNKS = (8, 20, 24)
+QUAD_CHUNKS = (8, 12, 20, 24)
 
 
 def tune_candidates(ens, steps):

@@ -334,7 +334,7 @@ def quad(f_in, mask, inlet_u, p, periodic, CH=8):
     in-place phase is checked race free: no LDS location is written by two cells, and no cell reads a
     location that another cell of the same phase writes."""
     _, H, W = f_in.shape
-    CWq, R = (128 if CH <= 16 else 64), 3  # the kernel's shape table: 128 x {8, 12, 16}, 64 x {20, 24, 32}
+    CWq, R = (128 if CH in (8, 16) else 64), 3  # the kernel's shape table: 128 x {8, 16}, 64 x {12, 14, 20, 24, 32}
     RW, RH = CWq + 2 * R, CH + 2 * R
     out = np.full_like(f_in, np.nan)
     written = np.zeros(f_in.shape, np.int32)

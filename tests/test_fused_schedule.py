@@ -92,7 +92,8 @@ def test_pair_schedule_matches_two_single_steps(oracle_lib, periodic, model, W, 
                                                    (False, O.MRT, 70, 9, 16), (False, O.MRT, 300, 50, 12),
                                                    (True, O.BGK, 136, 30, 12), (False, O.MRT, 150, 60, 24),
                                                    (True, O.MRT, 90, 50, 20), (False, O.BGK, 70, 40, 32),
-                                                   (True, O.MRT, 40, 12, 24)])
+                                                   (True, O.MRT, 40, 12, 24), (False, O.MRT, 150, 44, 14),
+                                                   (True, O.MRT, 70, 30, 14)])
 def test_quad_schedule_matches_four_single_steps(oracle_lib, periodic, model, W, H, CH):
     """The four-step tile kernel's phases (tests/fused_emul.py: quad) -- the even step of the 3-ring region,
     the odd step of ring 2 and the even step of ring 1 in place in LDS, the core's odd step scattered out --

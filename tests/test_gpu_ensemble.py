@@ -321,7 +321,7 @@ def test_ensemble_tuner_two_step(gpu, tmp_path, monkeypatch):
 # four-step launches, per-size-class plans and the 64-member workload
 # ---------------------------------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("chunk", [8, 20, 24])
+@pytest.mark.parametrize("chunk", [8, 12, 20, 24])
 @pytest.mark.parametrize("periodic", [True, False])
 def test_ensemble_four_step_bitwise(gpu, chunk, periodic):
     """k = 4 launches (the four-step tile kernel over every member, FP32) equal single steps of each member

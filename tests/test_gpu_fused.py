@@ -307,7 +307,7 @@ def test_fused_n_equals_single_launches(gpu, storage, cfg):
 # The four-step tile kernel (strip 0, k = 4, FP32 populations): bit for bit against four single steps
 # ---------------------------------------------------------------------------------------------------------
 
-QUAD_CHUNKS = [8, 12, 16, 20, 24, 32]  # 128-wide tiles of 8, 12, 16 rows; 64-wide tiles of 20, 24, 32 rows
+QUAD_CHUNKS = [8, 12, 14, 16, 20, 24, 32]  # 128-wide tiles of 8 and 16 rows; 64-wide tiles of 12, 14, 20, 24, 32 rows
 
 
 @pytest.mark.parametrize("chunk", QUAD_CHUNKS)
