@@ -44,8 +44,8 @@ TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-
 PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.93e6, ("bf16", 12): 156.97e6, ("f32", 12): 311.19e6}
 PAIR_VALU_PER_CELL = {("f16", 12): 363.8, ("bf16", 12): 363.4, ("f32", 12): 376.9}
 # The same for the four-step tile kernel (FP32, per tile shape); profiles/k_quad_cylinder_4096x1024.md.
-QUAD_TRAFFIC_PER_LAUNCH = {("f32", 24): 345.5e6}
-QUAD_VALU_PER_CELL = {("f32", 24): 799.8, ("f32", 8): 965.7}
+QUAD_TRAFFIC_PER_LAUNCH = {("f32", 12): 327.2e6, ("f32", 14): 336.1e6, ("f32", 24): 345.5e6}
+QUAD_VALU_PER_CELL = {("f32", 12): 898.9, ("f32", 14): 873.6, ("f32", 24): 799.8, ("f32", 8): 965.7}
 # VALU issue capacity of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each, 2.4 GHz
 VALU_WAVE_INSTS_PER_US = 256 * 4 / 4 * 2400.0
 
@@ -82,7 +82,7 @@ def roofline(name, form, cells, achieved_fp32):
     if tile and k == 2:
         out["kernel"] = f"k_pair {storage} 128x{ch} tiles (two steps per launch)"
     elif tile:
-        out["kernel"] = f"k_quad {storage} {128 if ch <= 16 else 64}x{ch} tiles (four steps per launch)"
+        out["kernel"] = f"k_quad {storage} {128 if ch in (8, 16) else 64}x{ch} tiles (four steps per launch)"
     else:
         out["kernel"] = f"k_fused {storage} k={k} strip={form.get('strip')} chunk={ch}"
     traffic = (PAIR_TRAFFIC_PER_LAUNCH if k == 2 else QUAD_TRAFFIC_PER_LAUNCH) if tile else {}

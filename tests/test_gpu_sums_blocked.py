@@ -84,7 +84,7 @@ def test_pair_kernel_sums_equals_two_single_steps(gpu, storage, chunk, periodic)
             _assert_equal(a, b, sel, f"{storage} {W}x{H} chunk {chunk} launch {launch}")
 
 
-@pytest.mark.parametrize("chunk", [8, 16, 20, 24, 32])
+@pytest.mark.parametrize("chunk", [8, 12, 14, 16, 20, 24, 32])
 @pytest.mark.parametrize("periodic", [True, False])
 def test_quad_kernel_sums_equals_four_single_steps(gpu, chunk, periodic):
     for W, H in ([(300, 70), (40, 12)] if periodic else [(333, 97), (70, 9)]):
