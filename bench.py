@@ -12,7 +12,7 @@ HIP events on the replay stream give the per-step kernel time used for the roofl
 traffic for FP32 single steps), the host clock around the same replay the wall time.
 The CPU baseline is the oracle's FP32 mirror (C, OpenMP over rows) on a bounded number of steps of the
 same lattice, timed on this host.  The "ensemble" object reports the batched workload of 64 periodic
-lattices of uneven sizes (one persistent launch per step vs 64 separate launches per step).
+lattices of uneven sizes (one batched launch per step, or per two or four steps, vs 64 separate launches per step).
 """
 from __future__ import annotations
 
@@ -99,8 +99,9 @@ def roofline(name, form, cells, achieved_fp32):
 
 
 def bench_ensemble(nsteps, graph_steps=24, l2_steps=1000):
-    """64 periodic lattices of uneven sizes (in size-class order): one persistent launch per step (k = 1) or per
-    two or four steps (k = 2, 4: the tile kernels), per cache group, replayed from a captured graph of
+    """64 periodic lattices of uneven sizes (in size-class order): one launch over all members per step (k = 1: a
+    persistent grid) or per two or four steps (k = 2, 4: the tile kernels, one workgroup per tile), per cache
+    group, replayed from a captured graph of
     `graph_steps` steps, vs the same steps as 64 separate launches per step (captured step-major and
     lattice-major, and issued eagerly).  Two tunings: one launch shape for the whole mix (tune_ensemble), and
     one per size class (tune_classes: small / medium / large members, each range with its own k, tile rows, grid
@@ -183,12 +184,12 @@ def bench_ensemble(nsteps, graph_steps=24, l2_steps=1000):
         "glups": rate(best_us),
         "k": k, "tile_rows": rows if k > 1 else None, "blocks": blocks, "group_mb": group >> 20,
         "tuner_best_per_k": {f"k{kk}": v for kk, v in sorted(best_by_k.items())},
-        "persistent_graph": {"glups": rate(us_p), "us_per_step": round(us_p, 2),
+        "batched_graph": {"glups": rate(us_p), "us_per_step": round(us_p, 2),
                              "gbs_at_72B": round(BYTES_PER_CELL_FP32 * cells / us_p / 1e3, 1)},
-        "persistent_graph_per_class": {"glups": rate(us_c), "us_per_step": round(us_c, 2),
+        "batched_graph_per_class": {"glups": rate(us_c), "us_per_step": round(us_c, 2),
                                        "gbs_at_72B": round(BYTES_PER_CELL_FP32 * cells / us_c / 1e3, 1),
                                        "plan": plan_json(plan)},
-        "persistent_graph_ungrouped": {"glups": rate(us_1), "us_per_step": round(us_1, 2),
+        "batched_graph_ungrouped": {"glups": rate(us_1), "us_per_step": round(us_1, 2),
                                        "gbs_at_72B": round(BYTES_PER_CELL_FP32 * cells / us_1 / 1e3, 1)},
         "separate_graph_step_major": {"glups": rate(us_s), "us_per_step": round(us_s, 2),
                                       "launches_per_step": len(members)},
@@ -213,10 +214,10 @@ def bench_ensemble(nsteps, graph_steps=24, l2_steps=1000):
     g = ens.capture(graph_steps)
     us16c, _ = timed(lambda: ens.replay(g))
     g.close()
-    res["persistent_graph_f16"] = {"glups": rate(us16), "us_per_step": round(us16, 2), "k": k16,
+    res["batched_graph_f16"] = {"glups": rate(us16), "us_per_step": round(us16, 2), "k": k16,
                                    "tile_rows": rows16 if k16 > 1 else None, "blocks": b16,
                                    "group_mb": group16 >> 20}
-    res["persistent_graph_f16_per_class"] = {"glups": rate(us16c), "us_per_step": round(us16c, 2),
+    res["batched_graph_f16_per_class"] = {"glups": rate(us16c), "us_per_step": round(us16c, 2),
                                              "plan": plan_json(plan16)}
     ens.close()
     del ens

@@ -263,7 +263,7 @@ int lbm_ensemble_step_n(lbm_ensemble *e, int parity, int nsteps, long long group
 int lbm_ensemble_pair_n(lbm_ensemble *e, int npairs, long long group_bytes, int chunk, int blocks,
                         lbm_stream_t stream);
 int lbm_ensemble_side(const lbm_ensemble *e);
-/* tiles of the two-step list with `chunk` rows, and its default grid (resident workgroups) */
+/* tiles of the two-step list with `chunk` rows, and its grid (one workgroup per tile) */
 int lbm_ensemble_pair_info(lbm_ensemble *e, int chunk, int *total_tiles, int *default_blocks);
 /*
  * Members [m0, m1) (in creation order) advanced nsteps steps with k steps per launch, so that each size class of
@@ -273,13 +273,16 @@ int lbm_ensemble_pair_info(lbm_ensemble *e, int chunk, int *total_tiles, int *de
  *   k = 4  the four-step kernel of lbm_step_fused with strip 0 (chunk 8: 128 x 8 tiles, 12, 20 or 24: 64-wide tiles;
  *          FP32 populations) from an even state, nsteps a multiple of 4.
  * The members are grouped for the cache as in lbm_ensemble_step_n (group_bytes) and every group runs all its
- * launches before the next.  The range's members must hold their states on the same side (the buffer
+ * launches before the next.  `blocks` sets the persistent grid of the k = 1 kernel (<= 0: as many workgroups as
+ * are resident at once); the k = 2 and 4 kernels launch one workgroup per tile and ignore it.  The range's
+ * members must hold their states on the same side (the buffer
  * lbm_ensemble_member_side reports); k = 2 and 4 launches move them between f and f_alt.  Every member ends
  * bit-identical to its solo run.  lbm_ensemble_step, _step_n and _pair_n are this on every member.
  */
 int lbm_ensemble_run(lbm_ensemble *e, int m0, int m1, int k, int chunk, int nsteps, int parity,
                      long long group_bytes, int blocks, lbm_stream_t stream);
-/* tiles of the k-step list with `chunk` rows (k = 1: the 128 x 2 list), and its default grid */
+/* tiles of the k-step list with `chunk` rows (k = 1: the 128 x 2 list), and its default grid (k = 2, 4: the
+ * number of tiles, one workgroup each) */
 int lbm_ensemble_run_info(lbm_ensemble *e, int k, int chunk, int *total_tiles, int *default_blocks);
 /* which buffer of member i holds its state (0: f, 1: f_alt); lbm_ensemble_side fails when they differ */
 int lbm_ensemble_member_side(const lbm_ensemble *e, int i);

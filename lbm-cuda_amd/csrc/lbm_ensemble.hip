@@ -1,13 +1,14 @@
-// Batched ensembles of independent lattices of uneven sizes, advanced by persistent kernels that walk a work
-// list spanning every member, and stream-capture graphs that replay any sequence of liblbm launches (the
-// reference driver's CuPy graph, lbm_smemaa_main.py:136-151).
+// Batched ensembles of independent lattices of uneven sizes, advanced by kernels over a work list spanning every
+// member (a persistent grid for single steps, one workgroup per tile for the two- and four-step tiles), and
+// stream-capture graphs that replay any sequence of liblbm launches (the reference driver's CuPy graph,
+// lbm_smemaa_main.py:136-151).
 //
 // No reference counterpart: the reference runs one lattice per process (lbm_smemaa_main.py:8-10).  Work lists
 // kept per ensemble:
 //   k = 1  tiles of 128 x 2 cells, each updated by aa_cell (the single-step kernel's cell update), in place;
 //   k = 2  tiles of 128 x CH cells, each advanced two steps by pair_tile (the two-step tile kernel), from a
 //          member's current buffer into its second one;
-//   k = 4  (FP32 populations) tiles of the four-step kernel's shapes (128 x 8, 64 x 20, 64 x 24), each advanced
+//   k = 4  (FP32 populations) tiles of the four-step kernel's shapes (128 x 8, 64 x 12, 64 x 20, 64 x 24), each advanced
 //          four steps by quad_tile, from one buffer into the other.
 // lbm_ensemble_run advances any contiguous range of members with any of these, so a tuner can give each size
 // class of an ensemble its own k and tile shape; each member's buffer side is tracked on its own.
@@ -24,8 +25,6 @@ struct lbm_ensemble {
     std::vector<int> side;  // per member: which of its buffers holds the state
     bool has_alt = false;   // every member has a second buffer
     int default_blocks = 0; // single-step workgroups resident at once on the whole device
-    int pair_blocks[lbm::ENS_NCH] = {0, 0, 0, 0}; // the same for the two-step kernels (0 = not yet computed)
-    int quad_blocks[lbm::ENS_NQ] = {0, 0, 0, 0};     // and for the four-step kernels
     int total_tiles = 0;
     int total_ptiles[lbm::ENS_NCH] = {0, 0, 0, 0};
     int total_qtiles[lbm::ENS_NQ] = {0, 0, 0, 0};
@@ -242,28 +241,6 @@ static int quad_qi(int chunk)
     return -1;
 }
 
-// resident two-step workgroups for tile rows ENS_CH[chi] (computed on first use)
-static int pair_default(lbm_ensemble *e, int chi, const char *who)
-{
-    if (!e->pair_blocks[chi]) {
-        hipError_t he = resident_blocks(reinterpret_cast<const void *>(pair_kernel(e, chi)), pairk::NT,
-                                        e->pair_blocks[chi]);
-        if (he != hipSuccess) return fail((int)he, "%s: %s", who, hipGetErrorString(he));
-    }
-    return LBM_OK;
-}
-
-// the same for the four-step kernels
-static int quad_default(lbm_ensemble *e, int qi, const char *who)
-{
-    if (!e->quad_blocks[qi]) {
-        hipError_t he = resident_blocks(reinterpret_cast<const void *>(quad_kernel(e, qi)), quad_threads(qi),
-                                        e->quad_blocks[qi]);
-        if (he != hipSuccess) return fail((int)he, "%s: %s", who, hipGetErrorString(he));
-    }
-    return LBM_OK;
-}
-
 int lbm_ensemble_pair_info(lbm_ensemble *e, int chunk, int *total_tiles, int *default_blocks)
 {
     return lbm_ensemble_run_info(e, 2, chunk, total_tiles, default_blocks);
@@ -279,19 +256,15 @@ int lbm_ensemble_run_info(lbm_ensemble *e, int k, int chunk, int *total_tiles, i
     } else if (k == 2) {
         const int chi = pair_chi(chunk);
         if (chi < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: k = 2 needs chunk 8, 12, 16 or 24");
-        int rc = pair_default(e, chi, "lbm_ensemble_run_info");
-        if (rc) return rc;
         tiles = e->total_ptiles[chi];
-        blocks = e->pair_blocks[chi];
+        blocks = tiles; // one workgroup per tile
     } else if (k == 4) {
         const int qi = quad_qi(chunk);
         if (qi < 0) return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: k = 4 needs chunk 8, 12, 20 or 24");
         if (e->fmt != LBM_FMT_F32)
             return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_run_info: k = 4 takes FP32 populations only");
-        int rc = quad_default(e, qi, "lbm_ensemble_run_info");
-        if (rc) return rc;
         tiles = e->total_qtiles[qi];
-        blocks = e->quad_blocks[qi];
+        blocks = tiles; // one workgroup per tile
     } else {
         return fail(LBM_ERR_ARG, "lbm_ensemble_run_info: k must be 1, 2 or 4");
     }
@@ -331,14 +304,16 @@ static int ens_launch_k(const lbm_ensemble *e, int k, int li, int parity, int bl
     } else if (k == 2) {
         int t0 = e->h_desc[m0].ptile0[li];
         int t1 = m1 < e->n ? e->h_desc[m1].ptile0[li] : e->total_ptiles[li];
+        if (t1 <= t0) return LBM_OK;
         void *args[] = {&d, &m0, &m1, &t0, &t1, &side, &li};
-        he = hipLaunchKernel(reinterpret_cast<const void *>(pair_kernel(e, li)), dim3(blocks), dim3(pairk::NT), args, 0,
-                             s);
+        he = hipLaunchKernel(reinterpret_cast<const void *>(pair_kernel(e, li)), dim3(t1 - t0), dim3(pairk::NT), args,
+                             0, s);
     } else {
         int t0 = e->h_desc[m0].qtile0[li];
         int t1 = m1 < e->n ? e->h_desc[m1].qtile0[li] : e->total_qtiles[li];
+        if (t1 <= t0) return LBM_OK;
         void *args[] = {&d, &m0, &m1, &t0, &t1, &side, &li};
-        he = hipLaunchKernel(reinterpret_cast<const void *>(quad_kernel(e, li)), dim3(blocks), dim3(quad_threads(li)),
+        he = hipLaunchKernel(reinterpret_cast<const void *>(quad_kernel(e, li)), dim3(t1 - t0), dim3(quad_threads(li)),
                              args, 0, s);
     }
     if (he != hipSuccess) return fail((int)he, "lbm_ensemble_run: %s", hipGetErrorString(he));

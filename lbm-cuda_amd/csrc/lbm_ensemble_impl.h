@@ -63,46 +63,43 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_ensemble(const EnsDesc *__re
     }
 }
 
+// Tile of this workgroup in a grid of one workgroup per tile of an n-tile list: blocks are dealt round-robin to
+// the 8 XCDs in dispatch order, so block b takes tile (b % 8) * (n / 8) + b / 8 and XCD i walks the i-th eighth of
+// the list (consecutive tiles -- neighbouring rows of one member -- share an L2); the last n % 8 tiles keep
+// dispatch order.
+__device__ __forceinline__ int ens_tile(int n)
+{
+    const int b = (int)blockIdx.x, per = n / 8;
+    return b < per * 8 ? (b % 8) * per + b / 8 : b;
+}
+
 // Two steps (even then odd) of members [m0, m1) whose tiles are [t0, t1) of the k = 2 list (tile rows
-// ENS_CH[chi] = CH), from buffer `side` into buffer `side ^ 1`.
+// ENS_CH[chi] = CH), from buffer `side` into buffer `side ^ 1`; one workgroup per tile (a persistent loop over
+// tiles kept the loop's invariants live across the tile update: 88-135 VGPRs against the single lattice's 56-75).
 template <int MODEL, int FMT, bool PERIODIC, int CH>
 __global__ __launch_bounds__(pairk::NT) void k_ensemble_pair(const EnsDesc *__restrict__ d, int m0, int m1, int t0,
                                                              int t1, int side, int chi)
 {
     __shared__ pairk::TileLds<FMT, CH> L;
-    const int G = gridDim.x, b = blockIdx.x;
-    const int gper = G / 8;
-    const int xcd = b % 8, l = b / 8;
-    const int per = (t1 - t0 + 7) / 8;
-    const int start = t0 + xcd * per, end = min(start + per, t1);
-    for (int t = start + l; t < end; t += gper) {
-        const EnsDesc &e = d[ens_member(d, m0, m1, t, [chi](const EnsDesc &x) { return x.ptile0[chi]; })];
-        const int lt = t - e.ptile0[chi];
-        const int ty = lt / e.ptiles_x, tx = lt - ty * e.ptiles_x;
-        pairk::pair_tile_any<MODEL, PERIODIC, FMT, CH>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u, nullptr, nullptr,
+    const int t = t0 + ens_tile(t1 - t0);
+    const EnsDesc &e = d[ens_member(d, m0, m1, t, [chi](const EnsDesc &x) { return x.ptile0[chi]; })];
+    const int lt = t - e.ptile0[chi];
+    const int ty = lt / e.ptiles_x, tx = lt - ty * e.ptiles_x;
+    pairk::pair_tile_any<MODEL, PERIODIC, FMT, CH>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u, nullptr, nullptr,
                                                    nullptr, tx, ty, L);
-        __syncthreads(); // the next tile's phase A overwrites the LDS this tile's phase B read
-    }
 }
 
 // Four steps of members [m0, m1) whose tiles are [t0, t1) of the four-step list qi (tile rows ENS_QCH[qi] = CH),
-// from buffer `side` into buffer `side ^ 1`.
+// from buffer `side` into buffer `side ^ 1`; one workgroup per tile.
 #define LBM_ENS_QUAD_BODY                                                                                          \
     constexpr int CW = quadk::Shape<CH>::CW, NT = quadk::Shape<CH>::NT;                                           \
     __shared__ quadk::QuadLds<FMT_F32, CW, CH, NT> L;                                                             \
-    const int G = gridDim.x, b = blockIdx.x;                                                                      \
-    const int gper = G / 8;                                                                                       \
-    const int xcd = b % 8, l = b / 8;                                                                             \
-    const int per = (t1 - t0 + 7) / 8;                                                                            \
-    const int start = t0 + xcd * per, end = min(start + per, t1);                                                 \
-    for (int t = start + l; t < end; t += gper) {                                                                 \
-        const EnsDesc &e = d[ens_member(d, m0, m1, t, [qi](const EnsDesc &x) { return x.qtile0[qi]; })];         \
-        const int lt = t - e.qtile0[qi];                                                                          \
-        const int ty = lt / e.qtiles_x[qi], tx = lt - ty * e.qtiles_x[qi];                                        \
-        quadk::quad_tile_any<MODEL, PERIODIC, FMT_F32, CW, CH, NT>(e.P, e.mask, e.f[side], e.f[side ^ 1],          \
-                                                                    e.inlet_u, nullptr, nullptr, nullptr, tx, ty, L); \
-        __syncthreads(); /* the next tile's first phase overwrites the LDS this tile's last phase read */         \
-    }
+    const int t = t0 + ens_tile(t1 - t0);                                                                         \
+    const EnsDesc &e = d[ens_member(d, m0, m1, t, [qi](const EnsDesc &x) { return x.qtile0[qi]; })];             \
+    const int lt = t - e.qtile0[qi];                                                                              \
+    const int ty = lt / e.qtiles_x[qi], tx = lt - ty * e.qtiles_x[qi];                                            \
+    quadk::quad_tile_any<MODEL, PERIODIC, FMT_F32, CW, CH, NT>(e.P, e.mask, e.f[side], e.f[side ^ 1], e.inlet_u,   \
+                                                                nullptr, nullptr, nullptr, tx, ty, L);
 
 template <int MODEL, bool PERIODIC, int CH>
 __global__ __launch_bounds__(quadk::Shape<CH>::NT) void k_ensemble_quad(const EnsDesc *__restrict__ d, int m0, int m1,

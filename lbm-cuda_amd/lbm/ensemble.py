@@ -195,8 +195,7 @@ class Ensemble:
         return cls(sims, blocks=blocks, two_step=two_step)
 
     def pair_blocks(self, chunk, k=2):
-        """Workgroups of the k-step kernel (k = 2 or 4) with `chunk` tile rows resident at once (its default
-        grid)."""
+        """Workgroups of one k-step launch (k = 2 or 4) with `chunk` tile rows over all members: one per tile."""
         tiles, blk = ctypes.c_int(), ctypes.c_int()
         _lib.check(self.lib.lbm_ensemble_run_info(self.handle, int(k), int(chunk), ctypes.byref(tiles),
                                                   ctypes.byref(blk)), "lbm_ensemble_run_info")
@@ -370,18 +369,14 @@ QUAD_CHUNKS = (8, 12, 20, 24)
 def tune_candidates(ens, steps):
     """(k, chunk, blocks) launch shapes to time: the single-step kernel at 2, 4, 6 or 8 workgroups per CU
     (never more than are resident at once), and, for a two-step ensemble at an even state, the two-step kernel
-    (even `steps`) and the four-step kernel (FP32, `steps` a multiple of 4) at each tile shape on its resident
-    grid (blocks 0) and half of it."""
+    (even `steps`) and the four-step kernel (FP32, `steps` a multiple of 4) at each tile shape (one workgroup per
+    tile: blocks 0)."""
     full = ens.default_blocks
     cands = [(1, 0, b) for b in sorted({max(8, full * w // 8 // 8 * 8) for w in (2, 4, 6, 8)})]
     if ens.two_step and ens.parity == 0 and steps % 2 == 0:
-        for ch in PAIR_CHUNKS:
-            pb = ens.pair_blocks(ch)
-            cands += [(2, ch, b) for b in sorted({max(8, pb // 2 // 8 * 8), pb})]
+        cands += [(2, ch, 0) for ch in PAIR_CHUNKS]
     if ens.two_step and ens.parity == 0 and steps % 4 == 0 and ens.fmt == _lib.FMT_F32:
-        for ch in QUAD_CHUNKS:
-            pb = ens.pair_blocks(ch, k=4)
-            cands += [(4, ch, b) for b in sorted({max(8, pb // 2 // 8 * 8), pb})]
+        cands += [(4, ch, 0) for ch in QUAD_CHUNKS]
     return cands
 
 
@@ -392,7 +387,7 @@ def tune_ensemble(ens, use_cache=True, steps=20, reps=2, verbose=False):
     dev = torch.cuda.get_device_name(ens.members[0].device)
     m0 = ens.members[0]
     two = ens.two_step and ens.parity == 0 and steps % 2 == 0
-    key = f"{dev}|fmt{ens.fmt}|model{m0.model}|bc{m0.bc}|steps{steps}|k{2 if two else 1}|v2|{size_classes(ens)}"
+    key = f"{dev}|fmt{ens.fmt}|model{m0.model}|bc{m0.bc}|steps{steps}|k{2 if two else 1}|v3|{size_classes(ens)}"
     cache = {}
     if use_cache:
         try:
