@@ -518,6 +518,14 @@ class Fp64Reference(threading.Thread):
             self.error = repr(e)
 
 
+def storage_kw(storage):
+    """Simulation keywords of a storage name: f32, f16, bf16, or f16r / bf16r -- the 16-bit formats stored as
+    deviations from the equilibrium of the inlet profile on each row (lbm_row_base) instead of from the rest state."""
+    if storage.endswith("r"):
+        return {"storage": storage[:-1], "row_base": "inlet"}
+    return {"storage": storage}
+
+
 def _observable_cells(case):
     upd = case.mask == 0
     if case.bc == "channel":
@@ -823,14 +831,14 @@ def main():
 
     # ---------------- the other forms: 16-bit deviation storage, and the fused kernels (autotuned)
     def make(storage):
-        return Simulation(W, H, case.tau_inv, **case.sim_kwargs(), device=dev, storage=storage)
+        return Simulation(W, H, case.tau_inv, **case.sim_kwargs(), device=dev, **storage_kw(storage))
 
     def make_dev(storage):
         """A lattice in `storage` from the developed state, run a further spin-up/3 steps in that storage (its
         own rounding developed too); timing only -- the errors start from the seeded state."""
         kw = dict(case.sim_kwargs())
         kw["f0"] = f_dev
-        s = Simulation(W, H, case.tau_inv, **kw, device=dev, storage=storage)
+        s = Simulation(W, H, case.tau_inv, **kw, device=dev, **storage_kw(storage))
         n = (spin // 3) & ~1
         if n:
             s.step(n)

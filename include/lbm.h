@@ -48,7 +48,8 @@ enum lbm_bc {
 
 /* Storage format of the populations.  LBM_FMT_F32 stores f; the 16-bit formats store deviations
  * h = f - w_q from the rest state (w_q the FP32 weights 0.444444444f, 0.1111111111f, 0.0277777778f of
- * lbm_core_smemaa.cu:4-5), rounded to nearest even; the collision always computes in FP32. */
+ * lbm_core_smemaa.cu:4-5), rounded to nearest even; the collision always computes in FP32.  With
+ * lbm_params.row_base set they store deviations from a per-row base state instead (see lbm_row_base). */
 enum lbm_format {
     LBM_FMT_F32 = 0,
     LBM_FMT_F16 = 1, /* IEEE half */
@@ -77,7 +78,29 @@ typedef struct lbm_params {
                        both, for the precision study.  Every stepping entry point takes 1 (single, pull, fused,
                        ensemble; each temporally blocked form bit-identical to its single steps); lbm_neq and the
                        closure step return LBM_ERR_UNSUPPORTED */
+    const float *row_base; /* NULL, or the per-row base of the 16-bit formats: a device table of LBM_ROW_BASE_STRIDE * H
+                              floats, 16-byte aligned (lbm_row_base).  Used by lbm_step_aa_h16(_n), lbm_step_fused_h16(_n)
+                              (k = 2, two-step tile kernel), lbm_macro_h16 and lbm_convert_state; FP32 entry points ignore
+                              it; lbm_convert and lbm_ensemble_create reject it */
 } lbm_params;
+
+/*
+ * Per-row base state of the 16-bit formats (no reference counterpart: the reference stores FP32 only).  A 16-bit
+ * word holds about 3 significant decimal digits of the stored deviation, so the smaller the deviation the smaller its
+ * rounding error.  With lbm_params.row_base = table, a slot on row y that holds a population moving in direction q
+ * stores half(clamp(h) - d) instead of half(clamp(h)), where h = f - w_q is the deviation the collision works on,
+ * clamp the population clamp [-w_q, 10 - w_q] and d = table[LBM_ROW_BASE_STRIDE * y + q]; a load returns
+ * (float)half + d.  Both FP32 operations round once.  A table built from the flow's mean velocity profile (a channel's
+ * inlet profile) stores mostly the departure from that profile: on the 4096 x 1024 cylinder flow it lowers the FP16
+ * velocity error against FP64 about tenfold.  Which direction a slot holds follows the AA parity: canonical slot q
+ * holds direction q, reversed slot q holds opp(q).
+ *
+ * lbm_row_base fills a host table of LBM_ROW_BASE_STRIDE * H floats: entry q < 9 of row y is the deviation from w_q of
+ * the equilibrium at rho = 1, u = (ux[y], uy[y]) (uy may be NULL: 0), w_q (3 c.u + 4.5 (c.u)^2 - 1.5 u.u) evaluated
+ * in FP64 with the exact weights and rounded once; entries 9..15 are 0.  Any other table is valid too.
+ */
+#define LBM_ROW_BASE_STRIDE 16
+int lbm_row_base(int H, const float *ux, const float *uy, float *table);
 
 /* ABI version of this header (bumped on any signature change). */
 int lbm_abi_version(void);
@@ -181,6 +204,14 @@ int lbm_macro_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h, i
  * FP32).  Round to nearest even, no clamping.  src and dst must not overlap.
  */
 int lbm_convert(const lbm_params *p, const void *src, int src_fmt, void *dst, int dst_fmt, lbm_stream_t stream);
+
+/*
+ * lbm_convert for a state of AA parity `parity`, with p->row_base applied to the 16-bit side(s): 16-bit -> FP32 gives
+ * w_q + ((float)half + d), FP32 -> 16-bit half((f - w_q) - d), d the base value of the direction the slot holds at that
+ * parity.  Without p->row_base it is lbm_convert.  Same-format calls copy the stored words.
+ */
+int lbm_convert_state(const lbm_params *p, const void *src, int src_fmt, void *dst, int dst_fmt, int parity,
+                      lbm_stream_t stream);
 
 /*
  * rho, ux, uy of a stored state (macro_kernel of lbm_core_smemaa_endfused.cu:377-379, extended to both

@@ -9,7 +9,10 @@ namespace lbm {
 // One AA step of cell (x, y); cells outside the lattice return at once.  EXT: the stress moments relax at
 // w_ext when w_ext >= 0 (a learned closure's rate), else by the model's own rule.  INTERIOR: the caller knows
 // that 1 <= x <= W - 3 and 1 <= y <= H - 2 (no wall row, inlet, outlet ghost or ghost source, and no neighbour
-// index wraps), which removes the per-cell boundary tests; the arithmetic is the same.
+// index wraps), which removes the per-cell boundary tests; the arithmetic is the same.  Row-based 16-bit formats
+// (row_based<FMT>) add the base value of each slot's row and direction on load and subtract it on store: the even step
+// reads and writes row y, the odd step gathers from rows y - c_qy (bounce-back: this cell's row) and scatters to rows
+// y + c_qy; the three rows' values sit in scalar registers (a wave covers part of one row).
 template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS, bool EXT = false, bool INTERIOR = false>
 __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict__ mask, void *__restrict__ f,
                                         const float *__restrict__ inlet_u, float *__restrict__ oux,
@@ -18,6 +21,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
 {
     using S = Store<FMT>;
     constexpr unsigned B = S::BYTES;
+    constexpr bool RB = row_based<FMT>();
     constexpr bool OPEN = !PERIODIC && !INTERIOR; // walls, inlet and outlet may be in reach
     constexpr bool WRAP = PERIODIC && !INTERIOR;  // neighbour indices may wrap around the lattice
     if (!INTERIOR && (x >= P.W || y >= P.H)) return;
@@ -39,9 +43,27 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
     float g[9], mac[3];
     unsigned nb[9], nbB[9], nb2[9];
     bool solid[9];
+    // row-based formats: base values of rows y - 1, y, y + 1 (wrapped when periodic; channel cells updated here
+    // have 1 <= y <= H - 2)
+    float dm[9] = {}, d0[9] = {}, dp[9] = {};
+    float one = 1.0f;
+    if constexpr (RB) {
+        one = opaque_one();
+        base_row_s(P.base, y, d0);
+        if constexpr (PARITY == 1) {
+            base_row_s(P.base, WRAP && y == 0 ? P.H - 1 : y - 1, dm);
+            base_row_s(P.base, WRAP && y == P.H - 1 ? 0 : y + 1, dp);
+        }
+    }
+    // base value of direction q on the row y + dy (dy = -1, 0, 1)
+    auto drow = [&](int dy, int q) { return dy < 0 ? dm[q] : (dy > 0 ? dp[q] : d0[q]); };
     if constexpr (PARITY == 0) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) g[q] = S::ld(rf, idx * B, (unsigned)q * plane * B, q);
+        for (int q = 0; q < 9; ++q)
+            if constexpr (RB)
+                g[q] = S::ldb(rf, idx * B, (unsigned)q * plane * B, d0[q], one);
+            else
+                g[q] = S::ld(rf, idx * B, (unsigned)q * plane * B, q);
     } else {
         nb[0] = idx;
         solid[0] = false;
@@ -79,23 +101,36 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
                 nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
             nb2[q] = opp(q) < q ? nbB[q] + 2u * pB : 0u; // used for q = 3, 4, 7, 8 only
         }
-        g[0] = S::ld(rf, idxB, 0, 0);
+        if constexpr (RB)
+            g[0] = S::ldb(rf, idxB, 0, d0[0], one);
+        else
+            g[0] = S::ld(rf, idxB, 0, 0);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
             // o < q: slot o at the neighbour = nbB[o] + o pB, slot q here = idx2 + o pB
             // o > q: slot o at the neighbour = nb2[o] + q pB, slot q here = idxB + q pB
             const unsigned off = o < q ? (solid[o] ? idx2 : nbB[o]) : (solid[o] ? idxB : nb2[o]);
-            g[q] = S::ld(rf, off, (unsigned)(o < q ? o : q) * pB, q);
+            if constexpr (RB) // bounce-back: this row's slot holding direction o; else the source row's, direction q
+                g[q] = S::ldb(rf, off, (unsigned)(o < q ? o : q) * pB, solid[o] ? d0[o] : drow(-cy(q), q), one);
+            else
+                g[q] = S::ld(rf, off, (unsigned)(o < q ? o : q) * pB, q);
         }
     }
     collide_dev<MODEL, EXT>(P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac, w_ext);
+    // store of direction q to a slot whose row has the base values dr (row-based formats)
+    auto put = [&](unsigned voff, unsigned soff, int q, float h, float dr) {
+        if constexpr (RB)
+            S::stb(rf, voff, soff, q, h, dr);
+        else
+            S::st(rf, voff, soff, q, h);
+    };
     if constexpr (PARITY == 0) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) S::st(rf, idx * B, (unsigned)opp(q) * plane * B, q, g[q]);
+        for (int q = 0; q < 9; ++q) put(idx * B, (unsigned)opp(q) * plane * B, q, g[q], d0[q]);
         if (OPEN && x == W - 2) { // reversed copy into the outlet ghost (lbm_core4.cu:114-116)
 #pragma unroll
-            for (int q = 0; q < 9; ++q) S::st(rf, (idx + 1) * B, (unsigned)opp(q) * plane * B, q, g[q]);
+            for (int q = 0; q < 9; ++q) put((idx + 1) * B, (unsigned)opp(q) * plane * B, q, g[q], d0[q]);
         }
     } else {
         // Bounce-back writes this cell's slot opp(q), otherwise the target's slot q (again a select).  The
@@ -104,7 +139,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         // the next step's Zou-He rebuilds without reading.
         const unsigned idxB = idx * B, pB = plane * B;
         const unsigned idx2 = idxB + 2u * pB;
-        S::st(rf, idxB, 0, 0, g[0]);
+        put(idxB, 0, 0, g[0], d0[0]);
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
@@ -112,13 +147,14 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
             // q < o: slot q at the target = nbB[q] + q pB, slot o here = idx2 + q pB
             // q > o: slot q at the target = nb2[q] + o pB, slot o here = idxB + o pB
             const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nb2[q]);
-            S::st(rf, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
+            // bounce-back: this row's slot that holds direction o next; else the target row's, direction q
+            put(off, (unsigned)(q < o ? q : o) * pB, q, g[q], RB ? (back ? d0[o] : drow(cy(q), q)) : 0.0f);
         }
         if (OPEN && x == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-            S::st(rf, idx * B, 3u * plane * B, 3, g[3]);
-            if (!solid[2]) S::st(rf, nb[2] * B, 6u * plane * B, 6, g[6]);
-            if (!solid[4]) S::st(rf, nb[4] * B, 7u * plane * B, 7, g[7]);
+            put(idx * B, 3u * plane * B, 3, g[3], d0[3]);
+            if (!solid[2]) put(nb[2] * B, 6u * plane * B, 6, g[6], dp[6]);
+            if (!solid[4]) put(nb[4] * B, 7u * plane * B, 7, g[7], dm[7]);
         }
     }
     if constexpr (MACROS) {

@@ -36,6 +36,7 @@ struct KPar {
     int clamp; // MRT: velocity and density clamps
     float tau_inv, tau0, tau0sq, c18cs2, s_e, s_eps, s_q;
     int sums; // lbm_params.sums: compensated sums (MODEL_MRT_SUMS instantiations)
+    const float *base; // lbm_params.row_base: per-row base of the row-based 16-bit formats (ROWB floats per row)
     int rowmap; // single step: the state (9 W H population bytes) exceeds the 256 MB MALL -> xcd_row_tile order
 };
 
@@ -352,7 +353,13 @@ __device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inle
 // 32-bit vector byte offset per access (no 64-bit vector address arithmetic); accesses past the end of
 // the resource read 0 and write nothing.
 // ------------------------------------------------------------------------------------------------------
-enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2 };
+enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2, FMT_F16R = 3, FMT_BF16R = 4 };
+// FMT_F16R / FMT_BF16R: the 16-bit formats relative to a per-row base (lbm_params.row_base, KPar::base).  A slot on
+// row y that holds a population moving in direction q stores half(clamp(h) - d_q(y)) instead of half(clamp(h)), with
+// d_q(y) = base[ROWB * y + q] (the base population's deviation from w_q), and loads h = (float)half + d_q(y); the
+// subtraction and the addition are FP32 operations, each rounded once.  The memory layout is the plain format's.
+template <int FMT> __host__ __device__ constexpr bool row_based() { return FMT == FMT_F16R || FMT == FMT_BF16R; }
+constexpr int ROWB = 16; // floats per row of a base table (9 used)
 
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 
@@ -421,6 +428,48 @@ template <> struct Conv<FMT_BF16> {
     __device__ static T from_pop(float f, int q) { return __builtin_bit_cast(uint16_t, (__bf16)(f - wq(q))); }
 };
 
+// The row-based formats: the plain format's conversions plus their base-relative forms.  ldb adds the base value d to
+// the stored deviation (FP32, one rounding); with `one` = 1.0 held in a VGPR the FP16 form is one v_fma_mix_f32 (the
+// half converted exactly, then one rounding: the bits of v_cvt_f32_f16 + v_add_f32), which the compiler would
+// otherwise rewrite as a conversion and an add.  stb clamps h like st, subtracts d (FP32, one rounding) and rounds
+// the difference to 16 bits.  (Rounding the exact difference once instead -- one v_fma_mixlo_f16, which is what the
+// compiler makes of a 16-bit rounding of an FP32 fma -- gives other bits in rare ties and measured slower.)
+template <int PLAIN> struct ConvRow : Conv<PLAIN> {
+    using P = Conv<PLAIN>;
+    using T = typename P::T;
+    __device__ static float ldb(T v, float d) { return P::ld(v, 0) + d; }
+    __device__ static float ldb(T v, float d, float one)
+    {
+        if constexpr (PLAIN == FMT_F16)
+            return fma_(P::ld(v, 0), one, d);
+        else
+            return P::ld(v, 0) + d;
+    }
+    __device__ static float clamp_h(float h, int q, const CollideK &K) { return clampf(h, -wq(q), K.hi[wclass(q)]); }
+    __device__ static T rnd(float v)
+    {
+        if constexpr (PLAIN == FMT_F16)
+            return __builtin_bit_cast(uint16_t, (_Float16)v);
+        else
+            return __builtin_bit_cast(uint16_t, (__bf16)v);
+    }
+    __device__ static T stb(float h, int q, float d, const CollideK &K) { return rnd(clamp_h(h, q, K) - d); }
+    __device__ static T stb(float h, int q, float d) { return stb(h, q, d, CollideK{}); }
+    __device__ static unsigned st2b(float h0, int q0, float d0, float h1, int q1, float d1, const CollideK &K)
+    {
+        const f32x2_t v = {clamp_h(h0, q0, K) - d0, clamp_h(h1, q1, K) - d1};
+        if constexpr (PLAIN == FMT_F16)
+            return __builtin_bit_cast(unsigned, __builtin_convertvector(v, f16x2_t));
+        else
+            return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_t));
+    }
+    // populations f = w + (stored + d) and back (no clamp), for conversions and macro fields
+    __device__ static float popb(T v, int q, float d) { return wq(q) + ldb(v, d); }
+    __device__ static T from_popb(float f, int q, float d) { return rnd((f - wq(q)) - d); }
+};
+template <> struct Conv<FMT_F16R> : ConvRow<FMT_F16> {};
+template <> struct Conv<FMT_BF16R> : ConvRow<FMT_BF16> {};
+
 // Buffer-resource loads and stores of one population in storage format FMT.
 template <int FMT> struct Store {
     using C = Conv<FMT>;
@@ -439,7 +488,36 @@ template <int FMT> struct Store {
         else
             __builtin_amdgcn_raw_buffer_store_b16(C::st(h, q), r, voff, soff, 0);
     }
+    // row-based formats: load adding, store subtracting the base value d of the slot
+    __device__ static float ldb(rsrc_t r, unsigned voff, unsigned soff, float d, float one)
+    {
+        return C::ldb((uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0), d, one);
+    }
+    __device__ static void stb(rsrc_t r, unsigned voff, unsigned soff, int q, float h, float d)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16(C::stb(h, q, d), r, voff, soff, 0);
+    }
 };
+
+// Base values of one row (the 9 used entries of a base table row) in scalar registers: y must be the same in every
+// active lane of the wave (it is read from the first one); the table is read through the constant address space,
+// so the loads are scalar (two per row).
+typedef const __attribute__((address_space(4))) float *const_fptr;
+__device__ __forceinline__ void base_row_s(const float *base, int y, float d[9])
+{
+    const int yu = __builtin_amdgcn_readfirstlane(y);
+    const const_fptr t = (const_fptr)base + ROWB * yu;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) d[q] = t[q];
+}
+
+// 1.0 in a VGPR the compiler cannot see through (the multiplier of the FP16 ldb form)
+__device__ __forceinline__ float opaque_one()
+{
+    float one = 1.0f;
+    asm volatile("" : "+v"(one));
+    return one;
+}
 
 // FP32 populations in registers (pull kernel and the fused kernel's LDS): f -> h -> collide -> f'
 template <int MODEL>

@@ -10,7 +10,7 @@ int launch_fused_fmt(const KPar &P, int model, int bc, int k, int strip, int par
                      const uint8_t *mask, const void *fin, void *fout, const float *inlet_u, int CH, float *ux,
                      float *uy, float *rho);
 
-// defined in lbm_pair.hip
+// defined in lbm_pair_impl.h (instantiated in lbm_pair.hip and lbm_pair_rb.hip)
 template <int FMT>
 int launch_pair_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, const uint8_t *mask, const void *fin,
                     void *fout, const float *inlet_u, float *ux, float *uy, float *rho);
@@ -70,8 +70,13 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
     if (k < 1 || k > FUSED_MAXK) return fail(LBM_ERR_ARG, "lbm_step_fused: k=%d outside [1, %d]", k, FUSED_MAXK);
     if (cfg)
         c = *cfg;
+    else if (P.base && bytes == 2 && k == 2)
+        c = lbm_fused_cfg{0, 12}; // row-based 16-bit storage: the two-step tile kernel is its blocked form
     else
         default_cfg(P.W, P.H, k, &c);
+    if (P.base && bytes == 2 && !(c.strip == 0 && k == 2))
+        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: row-based storage (p->row_base) runs k = 2 with the "
+                                         "two-step tile kernel (strip 0) only");
     if (c.strip == 0) {
         if ((k != 2 && k != 4) || parity != 0)
             return fail(LBM_ERR_UNSUPPORTED,
@@ -139,7 +144,12 @@ int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t 
     if (rc) return rc;
     if (c.strip != 0 && c.strip != 128) // the 16-bit strip-sweep kernels are built for 128-column strips only
         return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: 16-bit storage takes strip 0 or 128, not %d", c.strip);
-    if (fmt == LBM_FMT_F16)
+    if (P.base) // row-based: the two-step tile kernel (checked in fused_common)
+        rc = fmt == LBM_FMT_F16 ? launch_pair_fmt<FMT_F16R>(P, p->model, p->bc, c.chunk, to_hip(stream), mask, h_now,
+                                                            h_out, inlet_u, ux, uy, rho)
+                                : launch_pair_fmt<FMT_BF16R>(P, p->model, p->bc, c.chunk, to_hip(stream), mask, h_now,
+                                                             h_out, inlet_u, ux, uy, rho);
+    else if (fmt == LBM_FMT_F16)
         rc = launch_any<FMT_F16>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);
     else
         rc = launch_any<FMT_BF16>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);

@@ -1,90 +1,8 @@
-// Two-step tile kernel: one even (A) and one odd (B) AA step in one launch, bit-identical to two launches of
-// the single-step kernel, for lbm_step_fused / lbm_step_fused_h16 with k = 2 from an even state (selected by a
-// launch configuration with strip = 0; chunk = tile rows, 8, 12, 16 or 24).
-//
-// The reference fuses T steps in square shared-memory tiles (lbm_core_smemaa.cu:29-71,107-223,293-445).  Here
-// a workgroup owns a core of 128 x CH cells:
-//   phase A  loads the core plus a one-cell halo ring (130 x (CH + 2) cells) straight from HBM, performs the
-//            local even step of every one of them and keeps the reversed result in LDS (storage format, so
-//            the rounding is the single-step kernel's); the outlet ghost receives its copy there too;
-//   phase B  after one barrier, performs the odd step of the core cells: gather from LDS, collide, and scatter
-//            directly to the output buffer.  Every output slot has exactly one writer (the AA ownership rule
-//            of the single step), so no LDS staging or inter-tile ordering is needed for the output.
-// Per two steps HBM sees (130 (CH + 2)) / (128 CH) of the state read and the state written once; the halo costs
-// the same fraction of extra even-step collisions.
-#include "lbm_pair_tile.h"
+// Two-step tile kernel for FP32 populations and the plain 16-bit formats (kernels and launcher in
+// lbm_pair_impl.h).
+#include "lbm_pair_impl.h"
 
 namespace lbm {
-namespace pairk {
-
-template <int MODEL, bool PERIODIC, int FMT, int CH>
-__global__ __launch_bounds__(NT) void k_pair(KPar P, const uint8_t *__restrict__ mask, const void *__restrict__ fin,
-                                             void *__restrict__ fout, const float *__restrict__ inlet_u,
-                                             float *__restrict__ oux, float *__restrict__ ouy,
-                                             float *__restrict__ orho)
-{
-    __shared__ TileLds<FMT, CH> L;
-    int bx, by;
-    xcd_tile(bx, by);
-    pair_tile_any<MODEL, PERIODIC, FMT, CH>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L);
-}
-
-template <int MODEL, bool PERIODIC, int FMT>
-static void launch_ch(const KPar &P, int CH, dim3 grid, hipStream_t st, const uint8_t *mask, const void *fin,
-                      void *fout, const float *inlet_u, float *ux, float *uy, float *rho)
-{
-    switch (CH) {
-    case 24:
-        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 24>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
-                           uy, rho);
-        break;
-    case 16:
-        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 16>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
-                           uy, rho);
-        break;
-    case 12:
-        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 12>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
-                           uy, rho);
-        break;
-    default:
-        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 8>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
-                           uy, rho);
-    }
-}
-
-} // namespace pairk
-
-// Launches the two-step tile kernel (k = 2 from an even state; CH = 8, 12, 16 or 24 core rows per tile).
-template <int FMT>
-int launch_pair_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, const uint8_t *mask, const void *fin,
-                    void *fout, const float *inlet_u, float *ux, float *uy, float *rho)
-{
-    using namespace pairk;
-    const dim3 grid((P.W + CW - 1) / CW, (P.H + CH - 1) / CH);
-    const bool per = bc == LBM_BC_PERIODIC;
-    if (model == LBM_BGK) {
-        if (per)
-            launch_ch<MODEL_BGK, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
-        else
-            launch_ch<MODEL_BGK, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
-    } else if (P.sums) { // compensated sums read the clamp setting at run time
-        if (per)
-            launch_ch<MODEL_MRT_SUMS, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
-        else
-            launch_ch<MODEL_MRT_SUMS, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
-    } else if (P.clamp) { // the clamp setting as a template argument: no run-time select per cell
-        if (per)
-            launch_ch<MODEL_MRT_CLAMP1, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
-        else
-            launch_ch<MODEL_MRT_CLAMP1, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
-    } else {
-        if (per)
-            launch_ch<MODEL_MRT_CLAMP0, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
-        else
-            launch_ch<MODEL_MRT_CLAMP0, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
-    }
-    return LBM_OK;
-}
 
 template int launch_pair_fmt<FMT_F32>(const KPar &, int, int, int, hipStream_t, const uint8_t *, const void *, void *,
                                       const float *, float *, float *, float *);

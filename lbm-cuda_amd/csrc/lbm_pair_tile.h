@@ -1,4 +1,4 @@
-// The two-step tile update shared by the single-lattice kernel (lbm_pair.hip) and the batched ensemble
+// The two-step tile update shared by the single-lattice kernel (lbm_pair_impl.h) and the batched ensemble
 // (lbm_ensemble.hip): one even (A) and one odd (B) AA step of a 128 x CH tile, bit-identical to two single
 // steps.  Phase A performs the even step of the tile plus a one-cell halo ring from HBM into LDS; phase B the
 // odd step of the core, gathered from LDS and scattered straight to the output (every output slot has one
@@ -24,7 +24,26 @@ __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >=
 // v mod n for -n <= v < 2n (at most one period outside the lattice): no division
 __device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-template <int FMT, int CH> struct TileLds {
+// The region rows' base values of a row-based 16-bit format (nothing for the other formats).  A staged row holds
+// three groups of three directions by c_qy, each padded to 16 bytes: {0, 1, 3} (c_qy = 0), {2, 5, 6} (+1), {4, 7, 8}
+// (-1).  The odd step of a cell on row r gathers group 0 of row r, group 1 of row r - 1 and group 2 of row r + 1,
+// and scatters to group 0 of row r, group 1 of row r + 1 and group 2 of row r - 1: five 16-byte LDS reads.
+template <bool RB, int RH> struct TileBaseRows {};
+template <int RH> struct TileBaseRows<true, RH> {
+    alignas(16) float base[RH][ROWB];
+};
+__host__ __device__ constexpr int bgrp(int q) { return cy(q) == 0 ? 0 : (cy(q) > 0 ? 1 : 2); }
+__host__ __device__ constexpr int bidx(int q) { return q == 0 || q == 2 || q == 4 ? 0 : (q == 1 || q == 5 || q == 7 ? 1 : 2); }
+// direction at position i < 3 of group g < 3 (the inverse of bgrp, bidx); -1 for the padding
+__host__ __device__ constexpr int bdir(int g, int i)
+{
+    return i > 2 || g > 2 ? -1 : (g == 0 ? (i == 0 ? 0 : (i == 1 ? 1 : 3)) : (g == 1 ? (i == 0 ? 2 : (i == 1 ? 5 : 6))
+                                                                                   : (i == 0 ? 4 : (i == 1 ? 7 : 8))));
+}
+static_assert(bdir(bgrp(7), bidx(7)) == 7 && bdir(bgrp(3), bidx(3)) == 3 && bdir(bgrp(6), bidx(6)) == 6, "groups");
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int FMT, int CH> struct TileLds : TileBaseRows<row_based<FMT>(), CH + 2> {
     typename Conv<FMT>::T sf[9][CH + 2][LS];
     uint8_t sm[CH + 2][LS];           // 1 = solid or outside the lattice
     alignas(8) uint8_t wsolid[NT / 64]; // per wave: 1 if any of its region cells is solid (no open boundary)
@@ -34,6 +53,10 @@ template <int FMT, int CH> struct TileLds {
 // and the outlet cells x >= W - 2 (pair_interior), so every region cell is in the lattice, is updated by the
 // even step, is no inlet cell and feeds no outlet ghost, and no index wraps.  That tile-uniform fact removes
 // the per-cell boundary tests; the arithmetic is the same, so both forms give the same bits.
+// Row-based 16-bit formats (row_based<FMT>): the region rows' base values are staged in LDS for phase A (whose
+// cells' rows vary within a wave) and read into scalar registers for phase B (a wave covers half a core row); the
+// LDS copy of the even step's result is stored relative to the base as in HBM, so both phases round as the single
+// step does.
 template <int MODEL, bool PERIODIC, int FMT, int CH, bool INTERIOR>
 __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restrict__ mask, const void *__restrict__ fin,
                                           void *__restrict__ fout, const float *__restrict__ inlet_u,
@@ -44,6 +67,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     using C = Conv<FMT>;
     using T = typename C::T;
     constexpr unsigned B = S::BYTES;
+    constexpr bool RB = row_based<FMT>();
     constexpr int RH = CH + 2;
     constexpr bool OPEN = !PERIODIC && !INTERIOR; // inlet, outlet and walls may lie in the region
     constexpr bool WRAP = PERIODIC && !INTERIOR;  // region indices may wrap around the lattice
@@ -58,12 +82,38 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     const int t = threadIdx.x;
     const CollideK K = opaque_consts(P); // kept in VGPRs across the unrolled cells
     bool any_solid = false;              // no open boundary: a region cell of this thread is solid
+    const float one = RB ? opaque_one() : 1.0f;
+    // row-based formats: thread t < 16 RH fetches position t % 16 of staged region row t / 16 (the grouped order of
+    // TileBaseRows; padding and rows outside a channel lattice: 0, never used) before the population loads are
+    // issued, and stores it to LDS after them
+    float bstage = 0.0f;
+    if constexpr (RB) {
+        if (t < ROWB * RH) {
+            const int r = t / ROWB, j = t % ROWB;
+            const int q = bdir(j / 4, j % 4);
+            int gy = y0 - 1 + r;
+            bool ok = q >= 0;
+            if constexpr (WRAP)
+                gy = H >= RH ? wrap1(gy, H) : wrap(gy, H);
+            else if constexpr (OPEN)
+                ok = ok && (unsigned)gy < (unsigned)H;
+            bstage = ok ? P.base[ROWB * gy + q] : 0.0f;
+        }
+    }
+    // the staged group g (0, 1, 2) of region row rr, and the base value of direction q on region row rr
+    auto bgroup = [&](int rr, int g) -> f32x4_t {
+        if constexpr (RB)
+            return reinterpret_cast<const f32x4_t *>(&L.base[rr][0])[g];
+        else
+            return f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    };
 
     // ---- phase A: even step of every region cell, reversed result into LDS.  A thread's HBM loads (mask byte
     //      + 9 populations per cell) are issued PD cells ahead of their use, and unconditionally, so their
     //      latencies overlap and no load waits behind a mask test; PD = 4 bounds the registers they hold.
     constexpr int NA = (RH * RW + NT - 1) / NT;
-    constexpr int PD = NA < 4 ? NA : 4;
+    constexpr int PDMAX = RB ? 3 : 4; // row-based: one cell less in flight, for the base values' registers
+    constexpr int PD = NA < PDMAX ? NA : PDMAX;
     float ga[NA][9];
     unsigned ma[NA], ia[NA];
     bool ina[NA], la[NA], ua[NA];
@@ -90,7 +140,13 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         const unsigned idx = in ? (unsigned)(gy * W + gx) : 0u;
         ma[a] = __builtin_amdgcn_raw_buffer_load_b8(rm, idx, 0, 0);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) ga[a][q] = S::ld(rin, idx * B, (unsigned)q * plane * B, q);
+        for (int q = 0; q < 9; ++q) {
+            if constexpr (RB) // the raw bits; the base is added once the cell's row values are read from LDS
+                ga[a][q] = __builtin_bit_cast(
+                    float, (unsigned)__builtin_amdgcn_raw_buffer_load_b16(rin, idx * B, (unsigned)q * plane * B, 0));
+            else
+                ga[a][q] = S::ld(rin, idx * B, (unsigned)q * plane * B, q);
+        }
         ina[a] = in;
         la[a] = OPEN && gx == 0;
         // rows and columns the even step updates (all of them when periodic): 1 <= gy <= H - 2, gx <= W - 2
@@ -106,9 +162,33 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
         const bool inlet = la[a];
         if (!(ina[a] && ua[a] && (inlet || !solid))) return;
         float mac[3];
+        // row-based formats: the base values of the cell's row, read from LDS before the collision (for the loads)
+        // and again after it (for the stores) rather than held in nine registers across it
+        float db[9] = {};
+        auto read_base = [&]() {
+            const f32x4_t b[3] = {bgroup(r, 0), bgroup(r, 1), bgroup(r, 2)};
+#pragma unroll
+            for (int q = 0; q < 9; ++q) db[q] = b[bgrp(q)][bidx(q)];
+        };
+        if constexpr (RB) {
+            read_base();
+#pragma unroll
+            for (int q = 0; q < 9; ++q) ga[a][q] = C::ldb((T)__builtin_bit_cast(unsigned, ga[a][q]), db[q], one);
+        }
         collide_dev<MODEL>(P, ga[a], inlet, inlet ? inlet_u[ia[a]] : 0.0f, mac, -1.0f, K);
         T o[9];
-        if constexpr (B == 2) {
+        if constexpr (RB) {
+            asm volatile("" ::: "memory"); // the second read is not merged with the first
+            read_base();
+            // the reversed slot opp(q) of row r holds direction q: base value db[q]
+#pragma unroll
+            for (int q = 0; q < 8; q += 2) {
+                const unsigned pk = C::st2b(ga[a][q], q, db[q], ga[a][q + 1], q + 1, db[q + 1], K);
+                o[q] = (T)pk;
+                o[q + 1] = (T)(pk >> 16);
+            }
+            o[8] = C::stb(ga[a][8], 8, db[8], K);
+        } else if constexpr (B == 2) {
             // pairs (0,1) (2,3) (4,5) (6,7) in one conversion each; the high halves go to LDS as they are
 #pragma unroll
             for (int q = 0; q < 8; q += 2) {
@@ -130,6 +210,10 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     };
 #pragma unroll
     for (int a = 0; a < PD; ++a) load_cell(a);
+    if constexpr (RB) {
+        if (t < ROWB * RH) L.base[t / ROWB][t % ROWB] = bstage;
+        __syncthreads();
+    }
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         collide_cell(a);
@@ -174,15 +258,24 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
 #pragma unroll
             for (int q = 1; q < 9; ++q)
                 solid[q] = !FLUID && smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(OPEN && inlet && cx(q) < 0);
+            // row-based formats: base value of direction q on region row r + dy (the staged rows; read where used,
+            // with one address for the whole wave)
+            auto drow = [&](int dy, int q) -> float { return bgroup(r + dy, bgrp(q))[bidx(q)]; };
             float g[9], mac[3];
-            g[0] = C::ld(at(0, 0, 0), 0);
+            if constexpr (RB)
+                g[0] = C::ldb(at(0, 0, 0), drow(0, 0), one);
+            else
+                g[0] = C::ld(at(0, 0, 0), 0);
 #pragma unroll
             for (int q = 1; q < 9; ++q) {
                 const int o = opp(q);
                 // bounce-back: this cell's slot q; otherwise the source's slot opp(q) (both read, then selected).
                 // At the inlet the q with c_qx > 0 come from x = -1 and are rebuilt by Zou-He, which never reads them.
                 const T own = at(q, 0, 0), src = at(o, -cy(q), -cx(q));
-                g[q] = C::ld(solid[o] ? own : src, q);
+                if constexpr (RB) // own: direction o on this row; source: direction q on row gy - c_qy
+                    g[q] = C::ldb(solid[o] ? own : src, solid[o] ? drow(0, o) : drow(-cy(q), q), one);
+                else
+                    g[q] = C::ld(solid[o] ? own : src, q);
             }
             collide_dev<MODEL>(P, g, inlet, inlet ? inlet_u[gy] : 0.0f, mac, -1.0f, K);
             const unsigned idx = (unsigned)(gy * W + gx);
@@ -203,11 +296,18 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                     };
 #pragma unroll
                     for (int q = 0; q < 8; q += 2) {
-                        const unsigned pk = C::st2(g[q], q, g[q + 1], q + 1, K);
+                        unsigned pk;
+                        if constexpr (RB) // direction q lands on row gy + c_qy
+                            pk = C::st2b(g[q], q, drow(cy(q), q), g[q + 1], q + 1, drow(cy(q + 1), q + 1), K);
+                        else
+                            pk = C::st2(g[q], q, g[q + 1], q + 1, K);
                         put(q, (uint16_t)pk);
                         put(q + 1, (uint16_t)(pk >> 16));
                     }
-                    put(8, C::st(g[8], 8, K));
+                    if constexpr (RB)
+                        put(8, C::stb(g[8], 8, drow(cy(8), 8), K));
+                    else
+                        put(8, C::st(g[8], 8, K));
                 } else {
                     S::st(rout, vb, lowB, 0, g[0]);
 #pragma unroll
@@ -226,7 +326,14 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                     else
                         nbB[q] = idxB + (unsigned)((cx(q) + cy(q) * W) * (int)B);
                 }
-                S::st(rout, idxB, 0, 0, g[0]);
+                // store of direction q to a slot whose row has the base values dr (row-based formats)
+                auto put = [&](unsigned voff, unsigned soff, int q, float h, float dr) {
+                    if constexpr (RB)
+                        S::stb(rout, voff, soff, q, h, dr);
+                    else
+                        S::st(rout, voff, soff, q, h);
+                };
+                put(idxB, 0, 0, g[0], drow(0, 0));
 #pragma unroll
                 for (int q = 1; q < 9; ++q) {
                     // bounce-back writes this cell's slot opp(q); populations leaving through the open inlet are parked
@@ -236,13 +343,13 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                     const int o = opp(q);
                     const bool back = solid[q] || (OPEN && inlet && cx(q) < 0);
                     const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nbB[q] + 2u * pB);
-                    S::st(rout, off, (unsigned)(q < o ? q : o) * pB, q, g[q]);
+                    put(off, (unsigned)(q < o ? q : o) * pB, q, g[q], back ? drow(0, o) : drow(cy(q), q));
                 }
                 if (OPEN && gx == W - 2) {
                     // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-                    S::st(rout, idxB, 3u * pB, 3, g[3]);
-                    if (!solid[2]) S::st(rout, nbB[2], 6u * pB, 6, g[6]);
-                    if (!solid[4]) S::st(rout, nbB[4], 7u * pB, 7, g[7]);
+                    put(idxB, 3u * pB, 3, g[3], drow(0, 3));
+                    if (!solid[2]) put(nbB[2], 6u * pB, 6, g[6], drow(1, 6));
+                    if (!solid[4]) put(nbB[4], 7u * pB, 7, g[7], drow(-1, 7));
                 }
             }
             if (orho) {

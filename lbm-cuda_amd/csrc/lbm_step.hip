@@ -74,6 +74,9 @@ int make_kpar(const lbm_params *p, KPar *k, unsigned pop_bytes)
     k->s_q = p->s_q;
     if (p->sums != 0 && p->sums != 1) return fail(LBM_ERR_ARG, "sums %d must be 0 or 1", p->sums);
     k->sums = p->sums;
+    if (p->row_base && (reinterpret_cast<uintptr_t>(p->row_base) & 15u))
+        return fail(LBM_ERR_ARG, "row_base must be 16-byte aligned");
+    k->base = p->row_base;
     // measured on the cylinder case: the banded order is 1-2 % faster while the state fits the MALL (226 MB),
     // the row order 5-10 % faster beyond it (302 MB and up)
     k->rowmap = (long long)p->W * p->H * 9 * pop_bytes > (256LL << 20) ? 1 : 0;
@@ -227,8 +230,16 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_macro(KPar P, const uint8_t 
         if (mask[idx]) return;
     }
     float g[9];
+    if constexpr (row_based<FMT>()) {
+        // the slot holding direction q lies on this row: base value d[q] (a wave covers part of one row)
+        float d[9];
+        base_row_s(P.base, y, d);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) g[q] = Conv<FMT>::pop(f[(parity ? opp(q) : q) * P.plane + idx], q);
+        for (int q = 0; q < 9; ++q) g[q] = Conv<FMT>::popb(f[(parity ? opp(q) : q) * P.plane + idx], q, d[q]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = Conv<FMT>::pop(f[(parity ? opp(q) : q) * P.plane + idx], q);
+    }
     float rho, ux, uy;
     if (inlet && parity == 0) {
         const float u = inlet_u[y];
@@ -289,6 +300,32 @@ __global__ __launch_bounds__(256) void k_convert(unsigned plane, const typename 
 #pragma unroll
     for (int q = 0; q < 9; ++q)
         dst[(unsigned)q * plane + i] = Conv<DST>::from_pop(Conv<SRC>::pop(src[(unsigned)q * plane + i], q), q);
+}
+
+// Conversion with a per-row base on either side (row-based 16-bit formats): slot p of a cell on row y holds direction
+// p (parity 0) or opp(p) (parity 1), whose base value on that row is applied.
+template <int SRC, int DST>
+__global__ __launch_bounds__(256) void k_convert_rb(unsigned plane, unsigned W, const float *__restrict__ base,
+                                                     int parity, const typename Conv<SRC>::T *__restrict__ src,
+                                                     typename Conv<DST>::T *__restrict__ dst)
+{
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= plane) return;
+    const float *bt = base + ROWB * (i / W);
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        const int q = parity ? opp(p) : p;
+        const auto v = src[(unsigned)p * plane + i];
+        float f;
+        if constexpr (row_based<SRC>())
+            f = Conv<SRC>::popb(v, q, bt[q]);
+        else
+            f = Conv<SRC>::pop(v, q);
+        if constexpr (row_based<DST>())
+            dst[(unsigned)p * plane + i] = Conv<DST>::from_popb(f, q, bt[q]);
+        else
+            dst[(unsigned)p * plane + i] = Conv<DST>::from_pop(f, q);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -430,7 +467,7 @@ using namespace lbm;
 
 extern "C" {
 
-int lbm_abi_version(void) { return 2; }
+int lbm_abi_version(void) { return 3; }
 
 const char *lbm_last_error(void) { return g_err; }
 
@@ -543,12 +580,18 @@ int lbm_step_aa_h16(const lbm_params *p, const uint8_t *mask, uint16_t *h, int f
     if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_step_aa_h16: parity must be 0 or 1");
     if ((ux == nullptr) != (rho == nullptr) || (uy == nullptr) != (rho == nullptr))
         return fail(LBM_ERR_ARG, "lbm_step_aa_h16: ux, uy, rho must be all NULL or all non-NULL");
-    if (fmt == LBM_FMT_F16)
-        dispatch_aa_fmt<FMT_F16>(P, p->model, p->bc, parity, step_grid(P), to_hip(stream), mask, h, inlet_u, ux, uy,
-                                 rho);
-    else
-        dispatch_aa_fmt<FMT_BF16>(P, p->model, p->bc, parity, step_grid(P), to_hip(stream), mask, h, inlet_u, ux, uy,
-                                  rho);
+    const dim3 grid = step_grid(P);
+    hipStream_t s = to_hip(stream);
+    if (P.base) { // deviations from the per-row base
+        if (fmt == LBM_FMT_F16)
+            dispatch_aa_fmt<FMT_F16R>(P, p->model, p->bc, parity, grid, s, mask, h, inlet_u, ux, uy, rho);
+        else
+            dispatch_aa_fmt<FMT_BF16R>(P, p->model, p->bc, parity, grid, s, mask, h, inlet_u, ux, uy, rho);
+    } else if (fmt == LBM_FMT_F16) {
+        dispatch_aa_fmt<FMT_F16>(P, p->model, p->bc, parity, grid, s, mask, h, inlet_u, ux, uy, rho);
+    } else {
+        dispatch_aa_fmt<FMT_BF16>(P, p->model, p->bc, parity, grid, s, mask, h, inlet_u, ux, uy, rho);
+    }
     return check_launch("lbm_step_aa_h16");
 }
 
@@ -576,7 +619,22 @@ int lbm_macro_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h, i
     const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);
     hipStream_t s = to_hip(stream);
     const bool per = p->bc == LBM_BC_PERIODIC;
-    if (fmt == LBM_FMT_F16) {
+    if (P.base) {
+        if (fmt == LBM_FMT_F16) {
+            if (per)
+                hipLaunchKernelGGL((k_macro<true, FMT_F16R>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy, rho);
+            else
+                hipLaunchKernelGGL((k_macro<false, FMT_F16R>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy,
+                                   rho);
+        } else {
+            if (per)
+                hipLaunchKernelGGL((k_macro<true, FMT_BF16R>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy,
+                                   rho);
+            else
+                hipLaunchKernelGGL((k_macro<false, FMT_BF16R>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy,
+                                   rho);
+        }
+    } else if (fmt == LBM_FMT_F16) {
         if (per)
             hipLaunchKernelGGL((k_macro<true, FMT_F16>), grid, block, 0, s, P, mask, h, inlet_u, parity, ux, uy, rho);
         else
@@ -596,6 +654,8 @@ int lbm_convert(const lbm_params *p, const void *src, int src_fmt, void *dst, in
     int rc = make_kpar(p, &P, 0);
     if (rc) return rc;
     if (!src || !dst) return fail(LBM_ERR_ARG, "lbm_convert: NULL pointer");
+    if (P.base)
+        return fail(LBM_ERR_ARG, "lbm_convert: p->row_base is set; lbm_convert_state applies it (it needs the parity)");
     auto bytes = [](int f) { return f == LBM_FMT_F32 ? 4u : 2u; };
     if (src_fmt < 0 || src_fmt > 2 || dst_fmt < 0 || dst_fmt > 2) return fail(LBM_ERR_ARG, "lbm_convert: bad format");
     {
@@ -621,6 +681,67 @@ int lbm_convert(const lbm_params *p, const void *src, int src_fmt, void *dst, in
     LBM_CONV(FMT_F32, FMT_F32)
 #undef LBM_CONV
     return fail(LBM_ERR_UNSUPPORTED, "lbm_convert: %d -> %d not supported", src_fmt, dst_fmt);
+}
+
+int lbm_convert_state(const lbm_params *p, const void *src, int src_fmt, void *dst, int dst_fmt, int parity,
+                      lbm_stream_t stream)
+{
+    if (!p) return fail(LBM_ERR_ARG, "params is NULL");
+    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_convert_state: parity must be 0 or 1");
+    if (!p->row_base) return lbm_convert(p, src, src_fmt, dst, dst_fmt, stream);
+    KPar P;
+    int rc = make_kpar(p, &P, 0);
+    if (rc) return rc;
+    if (!src || !dst) return fail(LBM_ERR_ARG, "lbm_convert_state: NULL pointer");
+    auto bytes = [](int f) { return f == LBM_FMT_F32 ? 4u : 2u; };
+    if (src_fmt < 0 || src_fmt > 2 || dst_fmt < 0 || dst_fmt > 2)
+        return fail(LBM_ERR_ARG, "lbm_convert_state: bad format");
+    {
+        const char *a = static_cast<const char *>(src), *b = static_cast<const char *>(dst);
+        if (a < b + 9ull * P.plane * bytes(dst_fmt) && b < a + 9ull * P.plane * bytes(src_fmt))
+            return fail(LBM_ERR_ARG, "lbm_convert_state: src and dst overlap");
+    }
+    const dim3 grid((P.plane + 255) / 256), block(256);
+    hipStream_t s = to_hip(stream);
+    const unsigned plane = (unsigned)P.plane, W = (unsigned)P.W;
+    // the 16-bit sides are row-based
+#define LBM_CONV(S, D, KS, KD)                                                                                          \
+    if (src_fmt == S && dst_fmt == D) {                                                                                 \
+        hipLaunchKernelGGL((k_convert_rb<KS, KD>), grid, block, 0, s, plane, W, P.base, parity,                        \
+                           static_cast<const typename Conv<KS>::T *>(src), static_cast<typename Conv<KD>::T *>(dst));   \
+        return check_launch("lbm_convert_state");                                                                       \
+    }
+    LBM_CONV(FMT_F32, FMT_F16, FMT_F32, FMT_F16R)
+    LBM_CONV(FMT_F32, FMT_BF16, FMT_F32, FMT_BF16R)
+    LBM_CONV(FMT_F16, FMT_F32, FMT_F16R, FMT_F32)
+    LBM_CONV(FMT_BF16, FMT_F32, FMT_BF16R, FMT_F32)
+    LBM_CONV(FMT_F16, FMT_BF16, FMT_F16R, FMT_BF16R)
+    LBM_CONV(FMT_BF16, FMT_F16, FMT_BF16R, FMT_F16R)
+#undef LBM_CONV
+    // src_fmt == dst_fmt: the stored words themselves
+    const hipError_t e = hipMemcpyAsync(dst, src, 9ull * P.plane * bytes(src_fmt), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return fail((int)e, "lbm_convert_state: copy failed: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_row_base(int H, const float *ux, const float *uy, float *table)
+{
+    if (H < 1 || !ux || !table) return fail(LBM_ERR_ARG, "lbm_row_base: H >= 1, ux and table are required");
+    static const int CXh[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1}, CYh[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    static const double Wh[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+    for (int y = 0; y < H; ++y) {
+        const double u = ux[y], v = uy ? uy[y] : 0.0;
+        const double usq = u * u + v * v;
+        for (int q = 0; q < ROWB; ++q) {
+            double d = 0.0;
+            if (q < 9) {
+                const double cu = CXh[q] * u + CYh[q] * v;
+                d = Wh[q] * (3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+            }
+            table[(size_t)ROWB * y + q] = (float)d;
+        }
+    }
+    return LBM_OK;
 }
 
 size_t lbm_diagnostics_work_size(const lbm_params *) { return sizeof(double) * DIAG_BLOCKS * DIAG_N; }

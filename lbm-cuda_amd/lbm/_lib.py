@@ -15,7 +15,8 @@ MRT_LES, BGK = 0, 1
 CHANNEL, PERIODIC = 0, 1
 FMT_F32, FMT_F16, FMT_BF16 = 0, 1, 2
 
-ABI_VERSION = 2
+ABI_VERSION = 3
+ROW_BASE_STRIDE = 16  # LBM_ROW_BASE_STRIDE
 
 
 class LbmParams(ctypes.Structure):
@@ -33,6 +34,7 @@ class LbmParams(ctypes.Structure):
         ("s_eps", ctypes.c_float),
         ("s_q", ctypes.c_float),
         ("sums", ctypes.c_int32),
+        ("row_base", ctypes.c_void_p),
     ]
 
 
@@ -82,6 +84,8 @@ SIGNATURES = {
     "lbm_step_fused_h16": (_i, [_P, _vp, _vp, _vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "lbm_macro_h16": (_i, [_P, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp]),
     "lbm_convert": (_i, [_P, _vp, _i, _vp, _i, _vp]),
+    "lbm_convert_state": (_i, [_P, _vp, _i, _vp, _i, _i, _vp]),
+    "lbm_row_base": (_i, [_i, _vp, _vp, _vp]),
     "lbm_visualize": (_i, [_P, _vp, _vp, _vp, _vp, _f, _vp]),
     "lbm_diagnostics_work_size": (ctypes.c_size_t, [_P]),
     "lbm_diagnostics": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
@@ -146,6 +150,20 @@ def check(rc, what):
         raise LbmError(f"{what} failed ({rc}): {msg}")
 
 
-def params(W, H, tau_inv, cs=0.0, model=MRT_LES, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1, sums=0):
+def params(W, H, tau_inv, cs=0.0, model=MRT_LES, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1, sums=0,
+           row_base=None):
+    """row_base: None, or the device address of a per-row base table (lbm_row_base) for 16-bit storage"""
     return LbmParams(int(W), int(H), int(model), int(bc), int(clamp), float(tau_inv), float(cs), float(s_e),
-                     float(s_eps), float(s_q), int(sums))
+                     float(s_eps), float(s_q), int(sums), row_base)
+
+
+def row_base_table(ux, uy=None):
+    """Host table of lbm_row_base (numpy float32 [H, ROW_BASE_STRIDE]) for a velocity per row, computed by the
+    library's own host routine."""
+    import numpy as np
+    ux = np.ascontiguousarray(ux, np.float32)
+    uy = None if uy is None else np.ascontiguousarray(uy, np.float32)
+    out = np.zeros((ux.shape[0], ROW_BASE_STRIDE), np.float32)
+    check(load().lbm_row_base(ux.shape[0], ux.ctypes.data_as(_vp), None if uy is None else uy.ctypes.data_as(_vp),
+                              out.ctypes.data_as(_vp)), "lbm_row_base")
+    return out

@@ -31,7 +31,8 @@ def _cache_path():
 def _key(sim, k):
     p = sim.params
     dev = torch.cuda.get_device_name(sim.device)
-    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|fmt{sim.fmt}|k{k}"
+    rb = int(bool(p.row_base)) and sim.fmt != _lib.FMT_F32
+    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|fmt{sim.fmt}|rb{rb}|k{k}"
 
 
 def load_cache():
@@ -98,8 +99,11 @@ def candidates(sim, k):
     """(strip, chunk) pairs to time.  From an even state: the tile kernels only (strip 0, chunk = tile rows) --
     the two-step kernel for k = 2, the four-step kernel for k = 4 (FP32 populations) -- and nothing for other k:
     on 4096 x 1024 the strip sweep ran 61-85 us per step against 20-36 us for the tile kernels and 34-45 us for
-    single steps, so it cannot win there.  From an odd state (no tile kernel applies): strip-sweep shapes."""
+    single steps, so it cannot win there.  From an odd state (no tile kernel applies): strip-sweep shapes.
+    Row-based 16-bit storage runs the two-step kernel only."""
     H = sim.params.H
+    if sim.fmt != _lib.FMT_F32 and sim.params.row_base:  # row-based 16-bit storage: the two-step kernel only
+        return [(0, ch) for ch in (8, 12, 16, 24)] if sim.parity == 0 and k == 2 else []
     if sim.parity == 0:
         if k == 2:
             return [(0, ch) for ch in (8, 12, 16, 24)]

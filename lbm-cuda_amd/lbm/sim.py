@@ -5,8 +5,9 @@ mask on the host, move them to the GPU, then advance with single steps or k-step
 state lives in torch tensors only for device memory and streams; all compute is liblbm's HIP kernels.
 
 Storage: ``f`` is float32 [9, H, W] (SoA planes, x fastest) for ``storage="f32"``; for "f16"/"bf16" it
-holds the 16-bit deviations h = f - w (int16 tensor of raw bits).  ``parity`` is the AA parity of the
-state (0 = canonical, 1 = reversed).  The fused kernel needs a second buffer, allocated on first use.
+holds the 16-bit deviations h = f - w (int16 tensor of raw bits), or, with ``row_base``, the deviations from a
+per-row base state (lbm_row_base in include/lbm.h).  ``parity`` is the AA parity of the state (0 = canonical,
+1 = reversed).  The fused kernel needs a second buffer, allocated on first use.
 """
 from __future__ import annotations
 
@@ -45,10 +46,16 @@ class Simulation:
     Parameters mirror the reference driver surface: lattice size (W, H), ``tau_inv``, Smagorinsky ``cs``,
     the solid ``mask`` [H, W], the inlet speed (``inlet_u`` per row), steps-per-launch ``k`` (``step``), and
     the AA parity of the stored state.
+
+    ``row_base`` (16-bit storage only; ignored for FP32): store deviations from the equilibrium of a velocity per
+    row instead of from the rest state -- "inlet" (the inlet profile), an array ux[H], a pair (ux[H], uy[H]), or
+    a ready table [H, 16] (lbm_row_base's layout).  The blocked form is then the two-step kernel (k = 2, even
+    parity); single steps take either parity.
     """
 
     def __init__(self, W, H, tau_inv, *, cs=0.0, model="mrt", bc="channel", mask=None, inlet_u=None, f0=None,
-                 clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda", storage="f32", sums="plain"):
+                 clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda", storage="f32", sums="plain",
+                 row_base=None):
         self.lib = _lib.load()
         self.W, self.H = int(W), int(H)
         self.model = _MODELS[model] if isinstance(model, str) else int(model)
@@ -79,23 +86,45 @@ class Simulation:
         if f0.shape != (9, self.H, self.W):
             raise ValueError(f"f0 shape {f0.shape} != (9, {self.H}, {self.W})")
         self.fmt = _FMTS[storage] if isinstance(storage, str) else int(storage)
+        self.parity = int(parity)
+        self.row_base_host = None
+        self._row_base = None
+        if row_base is not None and self.fmt != _lib.FMT_F32:
+            self.row_base_host = self._base_table(row_base)
+            self._row_base = torch.from_numpy(self.row_base_host).to(self.device)  # 256-byte aligned
+            self.params.row_base = self._row_base.data_ptr()
         f32 = torch.from_numpy(np.ascontiguousarray(f0, dtype=np.float32)).to(self.device)
         if self.fmt == _lib.FMT_F32:
             self.f = f32
         else:
-            # deviations h = f - w rounded to nearest even on the device
+            # deviations h = f - w (minus the row base) rounded to nearest even on the device
             self.f = torch.empty(f32.shape, dtype=torch.int16, device=self.device)
-            _lib.check(self.lib.lbm_convert(ctypes.byref(self.params), _ptr(f32), _lib.FMT_F32, _ptr(self.f), self.fmt,
-                                            _stream()), "lbm_convert")
+            _lib.check(self.lib.lbm_convert_state(ctypes.byref(self.params), _ptr(f32), _lib.FMT_F32, _ptr(self.f),
+                                                  self.fmt, self.parity, _stream()), "lbm_convert_state")
             del f32
         self.f_alt = None
-        self.parity = int(parity)
         self.steps = 0
         self.rho = torch.zeros((self.H, self.W), dtype=torch.float32, device=self.device)
         self.ux = torch.zeros_like(self.rho)
         self.uy = torch.zeros_like(self.rho)
         self._diag_work = None
         self._diag_out = None
+
+    def _base_table(self, row_base):
+        if isinstance(row_base, str):
+            if row_base != "inlet":
+                raise ValueError(f"row_base {row_base!r}: expected 'inlet', ux[H], (ux, uy) or a table [H, 16]")
+            return _lib.row_base_table(self.inlet_u_host)
+        if isinstance(row_base, tuple):
+            ux, uy = row_base
+            return _lib.row_base_table(np.asarray(ux, np.float32).reshape(self.H),
+                                       np.asarray(uy, np.float32).reshape(self.H))
+        a = np.asarray(row_base, np.float32)
+        if a.shape == (self.H, _lib.ROW_BASE_STRIDE):
+            return np.ascontiguousarray(a)
+        if a.shape == (self.H,):
+            return _lib.row_base_table(a)
+        raise ValueError(f"row_base shape {a.shape}: expected ({self.H},) or ({self.H}, {_lib.ROW_BASE_STRIDE})")
 
     # ------------------------------------------------------------------ stepping
     def step(self, n=1, k=1, macros=False, cfg=None):
@@ -201,8 +230,8 @@ class Simulation:
         if self.fmt == _lib.FMT_F32:
             return self.f
         out = torch.empty(self.f.shape, dtype=torch.float32, device=self.device)
-        _lib.check(self.lib.lbm_convert(ctypes.byref(self.params), _ptr(self.f), self.fmt, _ptr(out), _lib.FMT_F32,
-                                        _stream()), "lbm_convert")
+        _lib.check(self.lib.lbm_convert_state(ctypes.byref(self.params), _ptr(self.f), self.fmt, _ptr(out),
+                                              _lib.FMT_F32, self.parity, _stream()), "lbm_convert_state")
         return out
 
     def diagnostics(self):
@@ -267,10 +296,11 @@ class Simulation:
     def save(self, path):
         """Write f, parity, step count and parameters to an .npz file."""
         p = self.params
+        extra = {} if self.row_base_host is None else {"row_base": self.row_base_host}
         np.savez(path, f=self.populations().cpu().numpy(), parity=self.parity, steps=self.steps, mask=self.mask_host,
                  inlet_u=self.inlet_u_host, raw=self.f.cpu().numpy(),
                  params=np.array([p.W, p.H, p.model, p.bc, p.clamp, self.fmt], np.int64),
-                 fparams=np.array([p.tau_inv, p.cs, p.s_e, p.s_eps, p.s_q], np.float32))
+                 fparams=np.array([p.tau_inv, p.cs, p.s_e, p.s_eps, p.s_q], np.float32), **extra)
 
     @classmethod
     def load(cls, path, device="cuda"):
@@ -279,7 +309,7 @@ class Simulation:
         tau_inv, cs, s_e, s_eps, s_q = (float(v) for v in z["fparams"])
         sim = cls(W, H, tau_inv, cs=cs, model=model, bc=bc, mask=z["mask"], inlet_u=z["inlet_u"], f0=z["f"],
                   clamp=bool(clamp), s_e=s_e, s_eps=s_eps, s_q=s_q, parity=int(z["parity"]), device=device,
-                  storage=fmt)
+                  storage=fmt, row_base=z["row_base"] if "row_base" in z.files else None)
         sim.f.copy_(torch.from_numpy(z["raw"]).to(sim.device))  # exact stored bits
         sim.steps = int(z["steps"])
         return sim

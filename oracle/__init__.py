@@ -98,7 +98,7 @@ def lib():
         L.oracle_mrt_forward_f64.argtypes = [vp, vp]
         L.oracle_mrt_inverse_f64.argtypes = [vp, vp]
         L.oracle_diagnostics_f32.argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
-        L.oracle_aa_step_h16.argtypes = [P, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp]
+        L.oracle_aa_step_h16.argtypes = [P, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp]
         L.oracle_convert_f32_h16.argtypes = [vp, vp, ctypes.c_long, ctypes.c_int]
         L.oracle_convert_h16_f32.argtypes = [vp, vp, ctypes.c_long, ctypes.c_int]
         _lib = L
@@ -232,24 +232,46 @@ def from_h16(bits, fmt):
     return y
 
 
-def deviations(f64, fmt):
-    """16-bit deviation state h = f - w (w: the FP32 weight literals) from an FP64 population array."""
-    h = (np.asarray(f64, np.float64) - W_F32.astype(np.float64)[:, None, None]).astype(np.float32)
+def _slot_base(base, parity, shape):
+    """FP32 base value of every slot [9, H, W] of a state at AA parity `parity` (slot p holds direction p, or opp(p)
+    when reversed), from a base table [H, 16]; zeros without a table."""
+    if base is None:
+        return np.zeros(shape, np.float32)
+    b = np.asarray(base, np.float32)
+    dirs = OPP if parity else np.arange(9)
+    return np.ascontiguousarray(np.broadcast_to(b[:, dirs].T[:, :, None], shape))
+
+
+def deviations(f64, fmt, base=None, parity=0):
+    """16-bit state from an FP64 population array: h = f - w rounded to FP32 (w: the FP32 weight literals), minus
+    the slot's base value (FP32) when a per-row base table [H, 16] is given, rounded to 16 bits."""
+    f64 = np.asarray(f64, np.float64)
+    h = (f64 - W_F32.astype(np.float64)[:, None, None]).astype(np.float32)
+    if base is not None:
+        h = (h - _slot_base(base, parity, h.shape)).astype(np.float32)
     return to_h16(h, fmt)
 
 
-def populations(h16, fmt):
-    """FP64 populations w + h of a 16-bit deviation state."""
-    return from_h16(h16, fmt).astype(np.float64) + W_F32.astype(np.float64)[:, None, None]
+def populations(h16, fmt, base=None, parity=0):
+    """FP64 populations w + h of a 16-bit state (h = stored + the slot's base value, in FP32, with a base table)."""
+    h = from_h16(h16, fmt)
+    if base is not None:
+        h = (h + _slot_base(base, parity, h.shape)).astype(np.float32)
+    return h.astype(np.float64) + W_F32.astype(np.float64)[:, None, None]
 
 
-def aa_step_h16(p, mask, h16, fmt, inlet_u32, parity, macros=False):
-    """One AA step in place on a 16-bit deviation state (FP32 collision, RNE storage)."""
+def aa_step_h16(p, mask, h16, fmt, inlet_u32, parity, macros=False, base=None):
+    """One AA step in place on a 16-bit deviation state (FP32 collision, RNE storage); base: None or a per-row
+    base table [H, 16] (float32) applied to every load and store."""
     assert h16.dtype == np.uint16 and h16.shape == (9, p.H, p.W) and h16.flags.c_contiguous
     prof = np.ascontiguousarray(inlet_u32, dtype=np.float32)
     mac = [np.zeros((p.H, p.W), np.float32) for _ in range(3)] if macros else [None] * 3
+    b = None
+    if base is not None:
+        b = np.ascontiguousarray(base, np.float32)
+        assert b.shape == (p.H, 16)
     lib().oracle_aa_step_h16(ctypes.byref(p), _ptr(mask), _ptr(h16), int(fmt), _ptr(prof), int(parity),
-                             _ptr(mac[1]), _ptr(mac[2]), _ptr(mac[0]))
+                             _ptr(mac[1]), _ptr(mac[2]), _ptr(mac[0]), None if b is None else _ptr(b))
     return tuple(mac) if macros else None
 
 

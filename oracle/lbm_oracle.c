@@ -241,6 +241,23 @@ static inline uint16_t h16_store(int fmt, int q, float h)
     return fmt == 2 ? float_to_bf16(c) : float_to_half(c);
 }
 
+/* Per-row base of the 16-bit formats (the kernels' lbm_params.row_base): a slot on row y holding direction q stores
+   the clamped deviation minus d = base[16 y + q] (one FP32 subtraction, then the 16-bit rounding) and loads the
+   stored value plus d (one FP32 addition).  base == NULL: d = 0, the plain formats. */
+#define ROWB 16
+static inline float h16_load_b(int fmt, uint16_t b, const float *base, int q, int y)
+{
+    return base ? h16_load(fmt, b) + base[ROWB * y + q] : h16_load(fmt, b);
+}
+
+static inline uint16_t h16_store_b(int fmt, int q, float h, const float *base, int y)
+{
+    if (!base) return h16_store(fmt, q, h);
+    float w = wq_f32(q);
+    float c = clampr_f32(h, -w, 10.0f - w) - base[ROWB * y + q];
+    return fmt == 2 ? float_to_bf16(c) : float_to_half(c);
+}
+
 void oracle_convert_f32_h16(const float *x, uint16_t *y, long n, int fmt)
 {
     for (long i = 0; i < n; ++i) y[i] = fmt == 2 ? float_to_bf16(x[i]) : float_to_half(x[i]);
@@ -251,9 +268,10 @@ void oracle_convert_h16_f32(const uint16_t *x, float *y, long n, int fmt)
     for (long i = 0; i < n; ++i) y[i] = h16_load(fmt, x[i]);
 }
 
-/* One AA step in place on 16-bit deviations; same streaming as oracle_aa_step_f32. */
+/* One AA step in place on 16-bit deviations; same streaming as oracle_aa_step_f32.  base: NULL, or the per-row
+   base table (16 floats per row) that the row-based formats apply to every load and store. */
 void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *f, int fmt, const float *inlet_u,
-                        int parity, float *oux, float *ouy, float *orho)
+                        int parity, float *oux, float *ouy, float *orho, const float *base)
 {
     kpar_t_f32 P;
     make_kpar_f32(pp, &P);
@@ -268,7 +286,7 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
             const long idx = (long)y * P.W + x;
             float g[9], mac[3];
             if (parity == 0) {
-                for (int q = 0; q < 9; ++q) g[q] = h16_load(fmt, f[q * plane + idx]);
+                for (int q = 0; q < 9; ++q) g[q] = h16_load_b(fmt, f[q * plane + idx], base, q, y);
             } else {
                 for (int q = 0; q < 9; ++q) {
                     if (inlet && CX[q] == 1) {
@@ -276,7 +294,10 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
                         continue;
                     }
                     long s = nbr_f32(&P, x, y, -CX[q], -CY[q]);
-                    g[q] = h16_load(fmt, mask[s] ? f[q * plane + idx] : f[OPP[q] * plane + s]);
+                    /* bounce-back: this cell's slot q holds direction opp(q); else the source's slot opp(q), on the
+                       source's row, holds direction q */
+                    g[q] = mask[s] ? h16_load_b(fmt, f[q * plane + idx], base, OPP[q], y)
+                                   : h16_load_b(fmt, f[OPP[q] * plane + s], base, q, (int)(s / P.W));
                 }
             }
             if (P.model == ORACLE_BGK)
@@ -284,23 +305,25 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
             else
                 collide_mrt_f32(&P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac, -1.0f);
             if (parity == 0) {
-                for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx] = h16_store(fmt, q, g[q]);
+                for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx] = h16_store_b(fmt, q, g[q], base, y);
                 if (channel && x == P.W - 2)
-                    for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx + 1] = h16_store(fmt, q, g[q]);
+                    for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx + 1] = h16_store_b(fmt, q, g[q], base, y);
             } else {
                 for (int q = 0; q < 9; ++q) {
                     if (inlet && CX[q] == -1) continue;
                     long d = nbr_f32(&P, x, y, CX[q], CY[q]);
+                    /* bounce-back: this cell's slot opp(q) then holds direction opp(q); else the target's slot q, on
+                       the target's row (the clamp interval is the same for q and opp(q)) */
                     if (mask[d])
-                        f[OPP[q] * plane + idx] = h16_store(fmt, q, g[q]);
+                        f[OPP[q] * plane + idx] = h16_store_b(fmt, OPP[q], g[q], base, y);
                     else
-                        f[q * plane + d] = h16_store(fmt, q, g[q]);
+                        f[q * plane + d] = h16_store_b(fmt, q, g[q], base, (int)(d / P.W));
                 }
                 if (channel && x == P.W - 2) {
                     for (int q = 0; q < 9; ++q) {
                         if (CX[q] != -1) continue;
                         long d = (long)(y + CY[q]) * P.W + x;
-                        if (!mask[d]) f[q * plane + d] = h16_store(fmt, q, g[q]);
+                        if (!mask[d]) f[q * plane + d] = h16_store_b(fmt, q, g[q], base, y + CY[q]);
                     }
                 }
             }
