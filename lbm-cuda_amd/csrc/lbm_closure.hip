@@ -141,8 +141,8 @@ int lbm_neq(const lbm_params *p, const uint8_t *mask, const float *f, int parity
     KPar P;
     int rc = make_kpar(p, &P, 4);
     if (rc) return rc;
-    if (p->bc != LBM_BC_PERIODIC || p->model != LBM_MRT_LES || P.sums)
-        return fail(LBM_ERR_UNSUPPORTED, "lbm_neq: periodic MRT lattices with plain sums only");
+    if (p->bc != LBM_BC_PERIODIC || p->model != LBM_MRT_LES || P.sums || P.fast)
+        return fail(LBM_ERR_UNSUPPORTED, "lbm_neq: periodic MRT lattices with plain sums and exact arithmetic only");
     if (!mask || !f || !neq) return fail(LBM_ERR_ARG, "lbm_neq: NULL buffer");
     if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_neq: parity must be 0 or 1");
     dim3 grid((P.W + TILE_X - 1) / TILE_X, (P.H + TILE_Y - 1) / TILE_Y);
@@ -165,6 +165,7 @@ int lbm_step_closure(const lbm_params *p, const uint8_t *mask, float *f, int par
     KPar P;
     rc = make_kpar(p, &P, 4);
     if (rc) return rc;
+    if (P.fast) return fail(LBM_ERR_UNSUPPORTED, "lbm_step_closure: fast arithmetic is not built for the closure step");
     dim3 grid((P.W + TILE_X - 1) / TILE_X, (P.H + TILE_Y - 1) / TILE_Y);
     if (parity)
         hipLaunchKernelGGL(k_closure<1>, grid, dim3(TILE_X, TILE_Y), 0, to_hip(stream), P, mask, f, neq, packed,

@@ -26,7 +26,12 @@ static_assert(opp(5) == 7 && opp(6) == 8 && opp(7) == 5 && opp(8) == 6, "opp");
 // MODEL_MRT_SUMS: MRT + LES with compensated (Neumaier) sums of rho - 1 and j (lbm_params.sums = 1).
 // MODEL_MRT_CLAMP0 / _CLAMP1: MODEL_MRT with the velocity/density clamps known off / on at compile time (the
 // same arithmetic as MODEL_MRT with that lbm_params.clamp, without the run-time select).
-enum { MODEL_MRT = 0, MODEL_BGK = 1, MODEL_MRT_SUMS = 2, MODEL_MRT_CLAMP0 = 3, MODEL_MRT_CLAMP1 = 4 };
+// MODEL_FAST (a flag or-ed into any of them but MODEL_MRT_SUMS): the collision's reciprocals and square roots are the
+// hardware's v_rcp_f32 / v_sqrt_f32 (within 1 ulp; lbm_params.fast), as the reference's -use_fast_math build computes
+// them, instead of the correctly rounded forms the CPU mirror reproduces bit for bit.
+enum { MODEL_MRT = 0, MODEL_BGK = 1, MODEL_MRT_SUMS = 2, MODEL_MRT_CLAMP0 = 3, MODEL_MRT_CLAMP1 = 4, MODEL_FAST = 8 };
+__host__ __device__ constexpr int model_base(int m) { return m & ~MODEL_FAST; }
+__host__ __device__ constexpr bool model_fast(int m) { return (m & MODEL_FAST) != 0; }
 
 // Kernel-side parameters, derived on the host in FP32 (tau0 = 1/tau_inv etc., as lbm_core4.cu:175-176).
 struct KPar {
@@ -38,6 +43,7 @@ struct KPar {
     int sums; // lbm_params.sums: compensated sums (MODEL_MRT_SUMS instantiations)
     const float *base; // lbm_params.row_base: per-row base of the row-based 16-bit formats (ROWB floats per row)
     int rowmap; // single step: the state (9 W H population bytes) exceeds the 256 MB MALL -> xcd_row_tile order
+    int fast;   // lbm_params.fast: MODEL_FAST instantiations
 };
 
 // The reference's FP32 literals (lbm_core4.cu:15-16,55-57; lbm_core_smemaa.cu:4-7)
@@ -78,6 +84,12 @@ __device__ __forceinline__ float recip_rn(float x)
         return fma_(fma_(-x, y, 1.0f), y, y);
     }
     return __fdiv_rn(1.0f, x);
+}
+// FAST: the hardware estimates themselves (within 1 ulp); otherwise the correctly rounded forms above
+template <bool FAST> __device__ __forceinline__ float recip_m(float x) { return FAST ? __builtin_amdgcn_rcpf(x) : recip_rn(x); }
+template <bool FAST> __device__ __forceinline__ float sqrt_m(float x)
+{
+    return FAST ? __builtin_amdgcn_sqrtf(x) : sqrt_rn_normal(x);
 }
 
 // clamp to [lo, hi] (lo <= hi) in one v_med3_f32; equal to min(max(v, lo), hi) for every non-NaN v
@@ -169,7 +181,7 @@ __device__ __forceinline__ void moments_neu(const float h[9], float &drho, float
 // mac = (rho, ux, uy) as written by the reference (lbm_core4.cu:59-61,145-147).
 // EXT: the stress moments relax at w_ext when w_ext >= 0 (a learned closure's rate), else as usual.
 // SUMS: rho - 1 and j as compensated sums (moments_neu) instead of the pairwise FP32 sums.
-template <bool EXT = false, bool SUMS = false, int CLAMP = -1>
+template <bool EXT = false, bool SUMS = false, int CLAMP = -1, bool FAST = false>
 __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
                                             float w_ext, const CollideK &K)
 {
@@ -207,7 +219,7 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
         float jx = a13 + sx, jy = a24 + sy;
         if constexpr (SUMS) moments_neu(h, drho, jx, jy);
         rho = 1.0f + drho;
-        const float inv_rho = recip_rn(rho);
+        const float inv_rho = recip_m<FAST>(rho);
         ux = jx * inv_rho;
         uy = jy * inv_rho;
         drc = drho;
@@ -256,10 +268,10 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
     }
     if (smag) {
         // Smagorinsky (lbm_core4.cu:172-177)
-        const float Q = sqrt_rn_normal(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
+        const float Q = sqrt_m<FAST>(fma_(n7, n7, fma_(n8, n8, 1e-8f)));
         const float d = fma_(P.c18cs2, Q * inv_rc, K.tau0sq);
-        const float tau_eff = 0.5f * (P.tau0 + sqrt_rn_normal(d));
-        w = recip_rn(tau_eff);
+        const float tau_eff = 0.5f * (P.tau0 + sqrt_m<FAST>(d));
+        w = recip_m<FAST>(tau_eff);
     }
     // relaxation (lbm_core4.cu:180-185)
     m1d = fma_(-P.s_e, n1, m1d);
@@ -294,6 +306,7 @@ __device__ __forceinline__ void collide_mrt(const KPar &P, float h[9], bool inle
 
 // BGK collision (lbm_core_smemaa.cu:159-212) on deviations: h' = h + tau_inv (heq - h) with
 // heq_q = feq_q - w_q = w_q (drho + rho (3cu + 4.5cu^2 - 1.5u^2)).  No velocity/density clamps.
+template <bool FAST = false>
 __device__ __forceinline__ void collide_bgk(const KPar &P, float h[9], bool inlet, float uin, float mac[3])
 {
     float rho, drho, ux, uy;
@@ -306,7 +319,7 @@ __device__ __forceinline__ void collide_bgk(const KPar &P, float h[9], bool inle
         const float a13 = h[1] - h[3], a24 = h[2] - h[4], a57 = h[5] - h[7], a86 = h[8] - h[6];
         drho = (h[0] + (s13 + s24)) + (s57 + s68);
         rho = 1.0f + drho;
-        const float inv_rho = recip_rn(rho);
+        const float inv_rho = recip_m<FAST>(rho);
         ux = (a13 + (a57 + a86)) * inv_rho;
         uy = (a24 + (a57 - a86)) * inv_rho;
     }
@@ -327,12 +340,14 @@ template <int MODEL, bool EXT = false>
 __device__ __forceinline__ void collide_dev(const KPar &P, float h[9], bool inlet, float uin, float mac[3],
                                             float w_ext, const CollideK &K)
 {
-    if constexpr (MODEL == MODEL_BGK)
-        collide_bgk(P, h, inlet, uin, mac);
+    constexpr int M = model_base(MODEL);
+    constexpr bool FAST = model_fast(MODEL);
+    static_assert(!(FAST && M == MODEL_MRT_SUMS), "compensated sums come with correctly rounded arithmetic");
+    if constexpr (M == MODEL_BGK)
+        collide_bgk<FAST>(P, h, inlet, uin, mac);
     else
-        collide_mrt<EXT, MODEL == MODEL_MRT_SUMS,
-                    MODEL == MODEL_MRT_CLAMP1 ? 1 : (MODEL == MODEL_MRT_CLAMP0 ? 0 : -1)>(P, h, inlet, uin, mac,
-                                                                                              w_ext, K);
+        collide_mrt<EXT, M == MODEL_MRT_SUMS, M == MODEL_MRT_CLAMP1 ? 1 : (M == MODEL_MRT_CLAMP0 ? 0 : -1), FAST>(
+            P, h, inlet, uin, mac, w_ext, K);
 }
 
 template <int MODEL, bool EXT = false>

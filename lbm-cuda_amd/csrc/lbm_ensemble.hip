@@ -147,6 +147,10 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
             delete e;
             return rc;
         }
+        if (inst[i].p.fast) {
+            delete e;
+            return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_create: member %d: fast arithmetic is for single lattices", i);
+        }
         if (inst[i].p.row_base) {
             delete e;
             return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_create: member %d: row-based storage (row_base) is for "

@@ -74,6 +74,8 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
         c = lbm_fused_cfg{0, 12}; // row-based 16-bit storage: the two-step tile kernel is its blocked form
     else
         default_cfg(P.W, P.H, k, &c);
+    if (P.fast && c.strip != 0)
+        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: fast arithmetic runs the tile kernels (strip 0) only");
     if (P.base && bytes == 2 && !(c.strip == 0 && k == 2))
         return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: row-based storage (p->row_base) runs k = 2 with the "
                                          "two-step tile kernel (strip 0) only");

@@ -77,6 +77,18 @@ int launch_pair_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, co
             launch_ch<MODEL_MRT_SUMS, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
         else
             launch_ch<MODEL_MRT_SUMS, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+    } else if (P.fast) { // hardware reciprocal and square root (lbm_params.fast)
+        if (P.clamp) {
+            if (per)
+                launch_ch<MODEL_MRT_CLAMP1 | MODEL_FAST, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            else
+                launch_ch<MODEL_MRT_CLAMP1 | MODEL_FAST, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+        } else {
+            if (per)
+                launch_ch<MODEL_MRT_CLAMP0 | MODEL_FAST, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            else
+                launch_ch<MODEL_MRT_CLAMP0 | MODEL_FAST, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+        }
     } else if (P.clamp) { // the clamp setting as a template argument: no run-time select per cell
         if (per)
             launch_ch<MODEL_MRT_CLAMP1, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);

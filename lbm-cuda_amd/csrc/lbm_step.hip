@@ -74,6 +74,10 @@ int make_kpar(const lbm_params *p, KPar *k, unsigned pop_bytes)
     k->s_q = p->s_q;
     if (p->sums != 0 && p->sums != 1) return fail(LBM_ERR_ARG, "sums %d must be 0 or 1", p->sums);
     k->sums = p->sums;
+    if (p->fast != 0 && p->fast != 1) return fail(LBM_ERR_ARG, "fast %d must be 0 or 1", p->fast);
+    if (p->fast && (p->sums || p->model != LBM_MRT_LES))
+        return fail(LBM_ERR_UNSUPPORTED, "fast arithmetic is for the MRT model without compensated sums");
+    k->fast = p->fast;
     if (p->row_base && (reinterpret_cast<uintptr_t>(p->row_base) & 15u))
         return fail(LBM_ERR_ARG, "row_base must be 16-byte aligned");
     k->base = p->row_base;
@@ -153,6 +157,8 @@ static void dispatch_aa_fmt(const KPar &P, int model, int bc, int parity, dim3 g
 {
     if (model == LBM_BGK)
         dispatch_aa<MODEL_BGK, FMT>(P, bc, parity, grid, s, mask, f, inlet_u, ux, uy, rho);
+    else if (P.fast)
+        dispatch_aa<MODEL_MRT | MODEL_FAST, FMT>(P, bc, parity, grid, s, mask, f, inlet_u, ux, uy, rho);
     else if (P.sums)
         dispatch_aa<MODEL_MRT_SUMS, FMT>(P, bc, parity, grid, s, mask, f, inlet_u, ux, uy, rho);
     else
@@ -467,7 +473,7 @@ using namespace lbm;
 
 extern "C" {
 
-int lbm_abi_version(void) { return 3; }
+int lbm_abi_version(void) { return 4; }
 
 const char *lbm_last_error(void) { return g_err; }
 
@@ -512,6 +518,7 @@ int lbm_step_pull(const lbm_params *p, const uint8_t *mask, const float *f_now, 
     if (rc) return rc;
     if (!mask || !f_now || !f_out || !ux || !uy || !rho)
         return fail(LBM_ERR_ARG, "lbm_step_pull: mask, f_now, f_out, ux, uy, rho must be non-NULL");
+    if (P.fast) return fail(LBM_ERR_UNSUPPORTED, "lbm_step_pull: fast arithmetic is not built for the pull step");
     if (f_now == f_out) return fail(LBM_ERR_ARG, "lbm_step_pull: f_now and f_out must differ");
     if (p->bc == LBM_BC_CHANNEL && !inlet_u) return fail(LBM_ERR_ARG, "lbm_step_pull: channel mode needs inlet_u");
     const dim3 grid = step_grid(P), block(TILE_X, TILE_Y);

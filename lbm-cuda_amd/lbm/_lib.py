@@ -15,7 +15,7 @@ MRT_LES, BGK = 0, 1
 CHANNEL, PERIODIC = 0, 1
 FMT_F32, FMT_F16, FMT_BF16 = 0, 1, 2
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 ROW_BASE_STRIDE = 16  # LBM_ROW_BASE_STRIDE
 
 
@@ -34,6 +34,7 @@ class LbmParams(ctypes.Structure):
         ("s_eps", ctypes.c_float),
         ("s_q", ctypes.c_float),
         ("sums", ctypes.c_int32),
+        ("fast", ctypes.c_int32),
         ("row_base", ctypes.c_void_p),
     ]
 
@@ -151,10 +152,11 @@ def check(rc, what):
 
 
 def params(W, H, tau_inv, cs=0.0, model=MRT_LES, bc=CHANNEL, clamp=1, s_e=1.1, s_eps=1.1, s_q=1.1, sums=0,
-           row_base=None):
-    """row_base: None, or the device address of a per-row base table (lbm_row_base) for 16-bit storage"""
+           row_base=None, fast=0):
+    """row_base: None, or the device address of a per-row base table (lbm_row_base) for 16-bit storage; fast: 1 for
+    the hardware's approximate reciprocal and square root in the collision (lbm_params.fast)"""
     return LbmParams(int(W), int(H), int(model), int(bc), int(clamp), float(tau_inv), float(cs), float(s_e),
-                     float(s_eps), float(s_q), int(sums), row_base)
+                     float(s_eps), float(s_q), int(sums), int(fast), row_base)
 
 
 def row_base_table(ux, uy=None):

@@ -32,7 +32,7 @@ def _key(sim, k):
     p = sim.params
     dev = torch.cuda.get_device_name(sim.device)
     rb = int(bool(p.row_base)) and sim.fmt != _lib.FMT_F32
-    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|fmt{sim.fmt}|rb{rb}|k{k}"
+    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|fmt{sim.fmt}|rb{rb}|fast{p.fast}|k{k}"
 
 
 def load_cache():
@@ -95,6 +95,10 @@ def time_config(sim, k, cfg, reps=10, bufs=None):
     return best
 
 
+def p_fast(sim):
+    return bool(sim.params.fast)
+
+
 def candidates(sim, k):
     """(strip, chunk) pairs to time.  From an even state: the tile kernels only (strip 0, chunk = tile rows) --
     the two-step kernel for k = 2, the four-step kernel for k = 4 (FP32 populations) -- and nothing for other k:
@@ -104,6 +108,8 @@ def candidates(sim, k):
     H = sim.params.H
     if sim.fmt != _lib.FMT_F32 and sim.params.row_base:  # row-based 16-bit storage: the two-step kernel only
         return [(0, ch) for ch in (8, 12, 16, 24)] if sim.parity == 0 and k == 2 else []
+    if p_fast(sim) and sim.parity == 1:  # fast arithmetic: the tile kernels only
+        return []
     if sim.parity == 0:
         if k == 2:
             return [(0, ch) for ch in (8, 12, 16, 24)]

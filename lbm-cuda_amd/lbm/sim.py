@@ -51,18 +51,24 @@ class Simulation:
     row instead of from the rest state -- "inlet" (the inlet profile), an array ux[H], a pair (ux[H], uy[H]), or
     a ready table [H, 16] (lbm_row_base's layout).  The blocked form is then the two-step kernel (k = 2, even
     parity); single steps take either parity.
+
+    ``arith="fast"`` (MRT only): the collision uses the hardware's approximate reciprocal and square root (within 1
+    ulp), as the reference's -use_fast_math build does; single steps and the tile kernels (k = 2, 4 from even
+    parity) take it, each blocked form bit-identical to its single steps.
     """
 
     def __init__(self, W, H, tau_inv, *, cs=0.0, model="mrt", bc="channel", mask=None, inlet_u=None, f0=None,
                  clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda", storage="f32", sums="plain",
-                 row_base=None):
+                 row_base=None, arith="exact"):
         self.lib = _lib.load()
         self.W, self.H = int(W), int(H)
         self.model = _MODELS[model] if isinstance(model, str) else int(model)
         self.bc = _BCS[bc] if isinstance(bc, str) else int(bc)
         # sums="neumaier": compensated sums of rho - 1 and j in the MRT collision (every kernel)
+        # arith="fast": the hardware's approximate reciprocal and square root in the collision (the reference's
+        # -use_fast_math arithmetic); "exact": correctly rounded, reproduced bit for bit by the CPU mirror
         self.params = _lib.params(self.W, self.H, tau_inv, cs, self.model, self.bc, int(bool(clamp)), s_e, s_eps,
-                                  s_q, {"plain": 0, "neumaier": 1}[sums])
+                                  s_q, {"plain": 0, "neumaier": 1}[sums], fast={"exact": 0, "fast": 1}[arith])
         self.device = torch.device(device)
         if mask is None:
             mask = geo.channel_mask(W, H) if self.bc == _lib.CHANNEL else geo.periodic_mask(W, H)
@@ -300,7 +306,7 @@ class Simulation:
         np.savez(path, f=self.populations().cpu().numpy(), parity=self.parity, steps=self.steps, mask=self.mask_host,
                  inlet_u=self.inlet_u_host, raw=self.f.cpu().numpy(),
                  params=np.array([p.W, p.H, p.model, p.bc, p.clamp, self.fmt], np.int64),
-                 fparams=np.array([p.tau_inv, p.cs, p.s_e, p.s_eps, p.s_q], np.float32), **extra)
+                 fparams=np.array([p.tau_inv, p.cs, p.s_e, p.s_eps, p.s_q], np.float32), fast=p.fast, **extra)
 
     @classmethod
     def load(cls, path, device="cuda"):
@@ -309,7 +315,8 @@ class Simulation:
         tau_inv, cs, s_e, s_eps, s_q = (float(v) for v in z["fparams"])
         sim = cls(W, H, tau_inv, cs=cs, model=model, bc=bc, mask=z["mask"], inlet_u=z["inlet_u"], f0=z["f"],
                   clamp=bool(clamp), s_e=s_e, s_eps=s_eps, s_q=s_q, parity=int(z["parity"]), device=device,
-                  storage=fmt, row_base=z["row_base"] if "row_base" in z.files else None)
+                  storage=fmt, row_base=z["row_base"] if "row_base" in z.files else None,
+                  arith="fast" if ("fast" in z.files and int(z["fast"])) else "exact")
         sim.f.copy_(torch.from_numpy(z["raw"]).to(sim.device))  # exact stored bits
         sim.steps = int(z["steps"])
         return sim
