@@ -1,7 +1,7 @@
 """D2Q9 throughput on one MI355X: GLUPS at 4096 x 1024 (plain FP32 vs chosen form), % HBM roofline, L2 error.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--k 8] [--cpu-seconds 10] [--l2-steps 20]
-                    [--ensemble-steps 200]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--k 8] [--cpu-seconds 10] [--l2 full|short|off]
+                    [--fused-storages f32,f32+fast,f16,f16r,f16r+fast,bf16] [--ensemble-steps 200] [--only ...]
 
 Workload: channel flow past a cylinder, 4096 x 1024, R = 64 at (1024, 512), Re = 1000, Cs = 0.15,
 parabolic inlet u_max = 0.05 (synthetic initial state: feq at rho = 1, u = inlet profile, seeded noise).
@@ -63,8 +63,9 @@ def roofline(name, form, cells, achieved_fp32):
                 "frac": round(achieved_fp32 / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES_PER_STEP["f32"],
                 "traffic_source": TRAFFIC_SOURCE, "algorithmic_bytes": BYTES_PER_CELL_FP32 * cells,
                 "kernel": "k_aa FP32 (plain form; even/odd average)", "bytes_per_cell": BYTES_PER_CELL_FP32}
-    storage = name.split("_")[1]
-    elem = 4 if storage in ("f32", "fp32") else 2
+    storage = form.get("storage", "f32")
+    spec = form.get("spec", storage)
+    elem = 4 if storage == "f32" else 2
     k = form.get("k", 1)
     launch_us = form["us_per_step"] * k
     alg = 2 * 9 * elem * cells  # per launch, whatever k
@@ -77,14 +78,14 @@ def roofline(name, form, cells, achieved_fp32):
                    traffic_source=TRAFFIC_SOURCE)
         return out
     tile = form.get("strip") == 0 and k in (2, 4)
-    key = (storage, form.get("chunk"))
+    key = (spec, form.get("chunk"))
     ch = form.get("chunk")
     if tile and k == 2:
-        out["kernel"] = f"k_pair {storage} 128x{ch} tiles (two steps per launch)"
+        out["kernel"] = f"k_pair {spec} 128x{ch} tiles (two steps per launch)"
     elif tile:
-        out["kernel"] = f"k_quad {storage} {128 if ch in (8, 16) else 64}x{ch} tiles (four steps per launch)"
+        out["kernel"] = f"k_quad {spec} {128 if ch in (8, 16) else 64}x{ch} tiles (four steps per launch)"
     else:
-        out["kernel"] = f"k_fused {storage} k={k} strip={form.get('strip')} chunk={ch}"
+        out["kernel"] = f"k_fused {spec} k={k} strip={form.get('strip')} chunk={ch}"
     traffic = (PAIR_TRAFFIC_PER_LAUNCH if k == 2 else QUAD_TRAFFIC_PER_LAUNCH) if tile else {}
     valu = (PAIR_VALU_PER_CELL if k == 2 else QUAD_VALU_PER_CELL) if tile else {}
     out["traffic"] = traffic.get(key)
@@ -518,12 +519,26 @@ class Fp64Reference(threading.Thread):
             self.error = repr(e)
 
 
-def storage_kw(storage):
-    """Simulation keywords of a storage name: f32, f16, bf16, or f16r / bf16r -- the 16-bit formats stored as
-    deviations from the equilibrium of the inlet profile on each row (lbm_row_base) instead of from the rest state."""
-    if storage.endswith("r"):
-        return {"storage": storage[:-1], "row_base": "inlet"}
-    return {"storage": storage}
+def parse_spec(spec):
+    """A form's storage spec: f32, f16 or bf16; an "r" suffix on a 16-bit format stores deviations from the
+    equilibrium of the inlet profile on each row (lbm_row_base) instead of from the rest state; "+fast" computes the
+    collision with the hardware's approximate reciprocal and square root (lbm_params.fast), the reference's
+    -use_fast_math arithmetic.  Returns (storage, row_base, arith)."""
+    arith = "exact"
+    if spec.endswith("+fast"):
+        spec, arith = spec[:-5], "fast"
+    rb = spec.endswith("r") and spec != "f32"
+    return (spec[:-1] if rb else spec), rb, arith
+
+
+def storage_kw(spec):
+    """Simulation keywords of a storage spec (parse_spec)."""
+    storage, rb, arith = parse_spec(spec)
+    return {"storage": storage, "row_base": "inlet" if rb else None, "arith": arith}
+
+
+def form_name(kind, spec):
+    return f"{kind}_{spec.replace('+', '_')}"
 
 
 def _observable_cells(case):
@@ -636,8 +651,10 @@ def main():
     ap.add_argument("--k", type=int, default=8, help="largest steps per launch the tuner tries")
     ap.add_argument("--fused-cfg", default="", metavar="K,STRIP,CHUNK",
                     help="time the fused form at this configuration instead of autotuning it")
-    ap.add_argument("--fused-storages", default="f32,f16,bf16",
-                    help="storages whose fused form is tuned and timed (comma-separated)")
+    ap.add_argument("--fused-storages", default="f32,f32+fast,f16,f16r,f16r+fast,bf16",
+                    help="storage specs whose fused form is tuned and timed (comma-separated): f32, f16, bf16, an r "
+                         "suffix for 16-bit deviations from the inlet profile's per-row equilibrium, +fast for the "
+                         "hardware reciprocal and square root")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--l2", choices=("full", "short", "off"), default="full",
                     help="L2 errors against FP64: 'full' = the cylinder at 1000/2000/5000/10000 steps and the "
@@ -709,7 +726,7 @@ def main():
         op = O.params(W, H, p.tau_inv, cs=p.cs, model=p.model, bc=p.bc, clamp=p.clamp)
 
         def cpu_rate(dtype, seconds, max_steps):
-            fc = np.ascontiguousarray(case.f0, dtype=dtype)
+            fc = np.array(case.f0, dtype=dtype, copy=True)  # a copy: the steps run in place on it
             prof = case.inlet_u.astype(dtype)
             n, par = 0, 0
             t1 = time.perf_counter()
@@ -817,7 +834,7 @@ def main():
     log(f"plain fp32 AA: {kern_us:.1f} us/step (events), {wall * 1e6 / args.steps:.1f} us/step wall, "
         f"{glups:.2f} GLUPS, {achieved:.0f} GB/s")
     forms = {"aa_fp32": {"glups": glups, "us_per_step": kern_us, "achieved_gbs": achieved, "bytes_per_cell": 72,
-                         "storage": "f32", "k": 1}}
+                         "storage": "f32", "spec": "f32", "arith": "exact", "k": 1}}
     fluid = int(np.count_nonzero(updated_cells(sim)))
     ceiling = copy_ceiling(BYTES_PER_CELL_FP32 // 2 * cells, dev) if args.only != "fused" else None
     rmw = rmw_ceiling(W, H, dev) if args.only != "fused" else None
@@ -845,6 +862,20 @@ def main():
         return s
 
     runners = {"aa_fp32": ("f32", 1, None)}
+    # the plain single step with the fast arithmetic as well: the ratio's denominator is the faster of the two
+    s_fast = make_dev("f32+fast")
+
+    def run_fast(n, s=s_fast):
+        s.step(n)
+
+    us_fast = timed_replay(s_fast, run_fast, args.steps, args.warmup, 1)[0]
+    del s_fast
+    forms["aa_fp32_fast"] = {"glups": cells / us_fast / 1e3, "us_per_step": us_fast, "bytes_per_cell": 72,
+                             "achieved_gbs": 72 * cells / us_fast / 1e3, "storage": "f32", "spec": "f32+fast",
+                             "arith": "fast", "k": 1}
+    runners["aa_fp32_fast"] = ("f32+fast", 1, None)
+    log(f"plain fp32 AA, fast arithmetic: {us_fast:.1f} us/step, {cells / us_fast / 1e3:.2f} GLUPS")
+    plain_ref = max(glups, forms["aa_fp32_fast"]["glups"])
     if args.only != "fused":
         for storage in ("f16", "bf16"):
             s16 = make_dev(storage)
@@ -859,15 +890,17 @@ def main():
             forms[f"aa_{storage}"] = {"glups": cells / us / 1e3, "us_per_step": us, "bytes_per_cell": 36,
                                       "achieved_gbs": 36 * cells / us / 1e3,
                                       "frac": 36 * cells / us / 1e3 / HBM_PEAK_GBS,
-                                      "traffic": TRAFFIC_BYTES_PER_STEP[storage], "storage": storage, "k": 1}
+                                      "traffic": TRAFFIC_BYTES_PER_STEP[storage], "storage": storage,
+                                      "spec": storage, "arith": "exact", "k": 1}
             runners[f"aa_{storage}"] = (storage, 1, None)
             log(f"{storage} AA: {us:.1f} us/step, {cells / us / 1e3:.2f} GLUPS")
             del s16
     if L.lbm_fused_max_k() >= 2 and args.k >= 2 and args.only != "aa":
         from lbm import autotune
         ks = tuple(k for k in autotune.KS if k <= args.k)
-        for storage in [v.strip() for v in args.fused_storages.split(",") if v.strip()]:
-            sf = sim if storage == "f32" else make_dev(storage)
+        for spec in [v.strip() for v in args.fused_storages.split(",") if v.strip()]:
+            storage, rb, arith = parse_spec(spec)
+            sf = sim if spec == "f32" else make_dev(spec)
             if sf.parity:  # the tuner offers the two-step tile kernel from even states only
                 sf.step(1)
             if args.fused_cfg:
@@ -880,19 +913,19 @@ def main():
             nlaunch = max(1, fused_steps // k)
             us = timed_replay(sf, lambda n, sf=sf, k=k, cfg=cfg: sf.step(n, k=k, cfg=cfg), nlaunch * k,
                               SETTLE_STEPS + fused_warmup, k)[0]
-            name = f"fused_{storage}"
+            name = form_name("fused", spec)
             kernel = ("k_pair" if k == 2 else "k_quad") if strip == 0 else "k_fused"
             forms[name] = {"glups": cells / us / 1e3, "us_per_step": us, "k": k, "strip": strip, "chunk": chunk,
-                           "kernel": kernel, "storage": storage,
+                           "kernel": kernel, "storage": storage, "spec": spec, "row_base": rb, "arith": arith,
                            "tuned": {f"k{r[0]}": {"strip": r[1], "chunk": r[2], "us_per_step": round(r[3], 2)}
                                      for r in table}}
-            runners[name] = (storage, k, cfg)
-            log(f"fused {storage} {kernel} k={k} strip={strip} chunk={chunk}: {us:.1f} us/step, "
+            runners[name] = (spec, k, cfg)
+            log(f"fused {spec} {kernel} k={k} strip={strip} chunk={chunk}: {us:.1f} us/step, "
                 f"{cells / us / 1e3:.2f} GLUPS")
-            if storage != "f32":
+            if spec != "f32":
                 del sf
     for v in forms.values():
-        v["x_plain"] = v["glups"] / glups  # HIP events on both sides
+        v["x_plain"] = v["glups"] / plain_ref  # HIP events on both sides
 
     # ---------------- drag and lift over the case's 10000 steps (and an extended record for the shedding)
     forces = None
@@ -904,8 +937,8 @@ def main():
     gpu_fields = {}
     if refs:
         marks = refs["c2"].marks
-        for name, (storage, k, cfg) in runners.items():
-            gpu_fields[name] = gpu_macro_marks(lambda storage=storage: make(storage), k, cfg, marks)
+        for name, (spec, k, cfg) in runners.items():
+            gpu_fields[name] = gpu_macro_marks(lambda spec=spec: make(spec), k, cfg, marks)
         if "c3" in refs:
             sc = refs["c3"].case
             for storage in ("f32", "f16", "bf16"):
@@ -962,11 +995,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "arith": ch.get("arith", "exact"),
         "data": "synthetic",
-        "config": {"workload": case.name, "form": chosen, "storage": ch.get("storage", "f32"),
+        "config": {"workload": case.name, "form": chosen, "storage": ch.get("spec", ch.get("storage", "f32")),
                    "kernel": ch.get("kernel", "k_aa"), "k": ch.get("k", 1), "W": W, "H": H, "re": 1000,
                    "cs": 0.15, "spinup_steps": spin, "parallelism": "single_gpu",
-                   "note": "dtype is the arithmetic (FP32 collision); storage is the population format in HBM"},
+                   "note": "dtype is the arithmetic (FP32 collision; arith: exact = correctly rounded reciprocal and "
+                           "square root, fast = the hardware estimates, the reference's -use_fast_math); storage is "
+                           "the population format in HBM (an r suffix: 16-bit deviations from a per-row base)"},
         "plain_fp32": {"glups": round(glups, 3), "us_per_step": round(kern_us, 2), "glups_wall": round(glups_wall, 3),
                        "wall_ms_per_step": round(wall * 1e3 / args.steps, 5),
                        "fluid_glups": round(glups * fluid / cells, 3),
@@ -974,7 +1010,8 @@ def main():
                        "frac_of_rmw_ceiling": (round(achieved / rmw["pair_gbs"], 4) if rmw else None)},
         "rmw_ceiling": rmw,
         "plain_fp32_beyond_mall": beyond,
-        "x_plain": round(ch["glups"] / glups, 4),
+        "x_plain": round(ch["glups"] / plain_ref, 4),
+        "x_plain_denominator": {"glups": round(plain_ref, 3), "forms": "the faster of aa_fp32 and aa_fp32_fast"},
         "copy_ceiling": ceiling,
         "fluid_cells": fluid,
         "roofline": roofline(chosen, ch, cells, achieved),
