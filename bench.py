@@ -41,11 +41,18 @@ TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-
 # The same for one launch (two steps) of the two-step tile kernel, per (storage, tile rows), and its VALU
 # instructions per cell and launch (rocprofv3 --pmc SQ_INSTS_VALU x 64 / cells); the summary with the command
 # lines is profiles/k_pair_cylinder_4096x1024.md.  Configurations not listed there report null.
-PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.93e6, ("bf16", 12): 156.97e6, ("f32", 12): 311.19e6}
-PAIR_VALU_PER_CELL = {("f16", 12): 363.8, ("bf16", 12): 363.4, ("f32", 12): 376.9}
-# The same for the four-step tile kernel (FP32, per tile shape); profiles/k_quad_cylinder_4096x1024.md.
-QUAD_TRAFFIC_PER_LAUNCH = {("f32", 12): 327.2e6, ("f32", 14): 336.1e6, ("f32", 24): 345.5e6}
-QUAD_VALU_PER_CELL = {("f32", 12): 898.9, ("f32", 14): 873.6, ("f32", 24): 799.8, ("f32", 8): 965.7}
+# Keys are (storage spec, tile rows): the spec of parse_spec (an r suffix: row-based 16-bit storage, +fast: the
+# hardware reciprocal and square root); the row-based and fast rows are in profiles/k_pair_rowbase_fast_4096x1024.md.
+PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.93e6, ("bf16", 12): 156.97e6, ("f32", 12): 311.19e6,
+                           ("f16+fast", 12): 156.9e6, ("f16r", 12): 157.2e6, ("f16r+fast", 12): 157.1e6}
+PAIR_VALU_PER_CELL = {("f16", 12): 363.8, ("bf16", 12): 363.4, ("f32", 12): 376.9, ("f16+fast", 12): 311.0,
+                      ("f16r", 12): 397.9, ("f16r+fast", 12): 345.0}
+# The same for the four-step tile kernel (FP32, per tile shape); profiles/k_quad_cylinder_4096x1024.md (fast
+# arithmetic: profiles/k_quad_fast_cylinder_4096x1024.md).
+QUAD_TRAFFIC_PER_LAUNCH = {("f32", 12): 327.2e6, ("f32", 14): 336.1e6, ("f32", 24): 345.5e6,
+                           ("f32+fast", 12): 325.7e6}
+QUAD_VALU_PER_CELL = {("f32", 12): 898.9, ("f32", 14): 873.6, ("f32", 24): 799.8, ("f32", 8): 965.7,
+                      ("f32+fast", 12): 772.4}
 # VALU issue capacity of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles each, 2.4 GHz
 VALU_WAVE_INSTS_PER_US = 256 * 4 / 4 * 2400.0
 
