@@ -212,6 +212,17 @@ def bench_ensemble(nsteps, graph_steps=24, l2_steps=1000):
     ens.close()
     del ens
     torch.cuda.empty_cache()
+    # the same members with the fast arithmetic (hardware reciprocal and square root), tuned and timed the same way
+    ens = Ensemble.from_members(members, two_step=True, arith="fast")
+    bF, groupF, _, kF, rowsF = tune_ensemble(ens, use_cache=False, steps=graph_steps, reps=2)
+    g = ens.capture(graph_steps)
+    usF, _ = timed(lambda: ens.replay(g))
+    g.close()
+    res["batched_graph_fast"] = {"glups": rate(usF), "us_per_step": round(usF, 2), "k": kF,
+                                 "tile_rows": rowsF if kF > 1 else None, "blocks": bF, "group_mb": groupF >> 20}
+    ens.close()
+    del ens
+    torch.cuda.empty_cache()
     # the same members with FP16 deviation storage, tuned and timed the same way
     ens = Ensemble.from_members(members, storage="f16", two_step=True)
     b16, group16, _, k16, rows16 = tune_ensemble(ens, use_cache=False, steps=graph_steps, reps=2)

@@ -81,9 +81,9 @@ typedef struct lbm_params {
     int32_t fast;   /* 0: the collision's reciprocals and square roots are correctly rounded (the CPU FP32 mirror of the
                        oracle reproduces every kernel bit for bit); 1: the hardware's approximate v_rcp_f32 / v_sqrt_f32
                        (within 1 ulp), the arithmetic of the reference's -use_fast_math build (lbm_smemaa_main.py:84):
-                       MRT only, taken by lbm_step_aa(_h16)(_n) and the tile kernels of lbm_step_fused(_h16)(_n) (strip 0),
-                       each blocked form still bit-identical to its single steps; other entry points return
-                       LBM_ERR_UNSUPPORTED, and so does fast = 1 with sums = 1 */
+                       MRT only, taken by lbm_step_aa(_h16)(_n), the tile kernels of lbm_step_fused(_h16)(_n) (strip 0)
+                       and the ensembles (all members alike), each blocked form still bit-identical to its single
+                       steps; other entry points return LBM_ERR_UNSUPPORTED, and so does fast = 1 with sums = 1 */
     const float *row_base; /* NULL, or the per-row base of the 16-bit formats: a device table of LBM_ROW_BASE_STRIDE * H
                               floats, 16-byte aligned (lbm_row_base).  Used by lbm_step_aa_h16(_n), lbm_step_fused_h16(_n)
                               (k = 2, two-step tile kernel), lbm_macro_h16 and lbm_convert_state; FP32 entry points ignore

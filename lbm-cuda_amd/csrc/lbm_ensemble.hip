@@ -21,7 +21,7 @@
 
 struct lbm_ensemble {
     int n = 0;
-    int fmt = 0, model = 0, bc = 0, sums = 0;
+    int fmt = 0, model = 0, bc = 0, sums = 0, fast = 0;
     std::vector<int> side;  // per member: which of its buffers holds the state
     bool has_alt = false;   // every member has a second buffer
     int default_blocks = 0; // single-step workgroups resident at once on the whole device
@@ -44,6 +44,7 @@ static PairKernel quad_kernel(const lbm_ensemble *e, int qi)
 {
     const bool periodic = e->bc == LBM_BC_PERIODIC;
     if (e->sums) return quad_kernel_sums(periodic, qi);
+    if (e->fast) return quad_kernel_fast(periodic, qi);
     if (e->model == LBM_BGK) return periodic ? quad_kernel_c<MODEL_BGK, true>(qi) : quad_kernel_c<MODEL_BGK, false>(qi);
     return periodic ? quad_kernel_c<MODEL_MRT, true>(qi) : quad_kernel_c<MODEL_MRT, false>(qi);
 }
@@ -61,6 +62,7 @@ static EnsKernel ens_kernel(const lbm_ensemble *e, int parity)
 {
     const bool periodic = e->bc == LBM_BC_PERIODIC;
     if (e->sums) return ens_kernel_sums(e->fmt, parity, periodic);
+    if (e->fast) return ens_kernel_fast(e->fmt, parity, periodic);
     if (e->fmt == LBM_FMT_F32) return ens_kernel_f<FMT_F32>(e->model, parity, periodic);
     if (e->fmt == LBM_FMT_F16) return ens_kernel_f<FMT_F16>(e->model, parity, periodic);
     return ens_kernel_f<FMT_BF16>(e->model, parity, periodic);
@@ -78,6 +80,7 @@ static PairKernel pair_kernel(const lbm_ensemble *e, int chi)
 {
     const bool periodic = e->bc == LBM_BC_PERIODIC;
     if (e->sums) return pair_kernel_sums(e->fmt, periodic, chi);
+    if (e->fast) return pair_kernel_fast(e->fmt, periodic, chi);
     if (e->fmt == LBM_FMT_F32) return pair_kernel_f<FMT_F32>(e->model, periodic, chi);
     if (e->fmt == LBM_FMT_F16) return pair_kernel_f<FMT_F16>(e->model, periodic, chi);
     return pair_kernel_f<FMT_BF16>(e->model, periodic, chi);
@@ -147,23 +150,22 @@ int lbm_ensemble_create(const lbm_instance *inst, int n, int fmt, lbm_ensemble *
             delete e;
             return rc;
         }
-        if (inst[i].p.fast) {
-            delete e;
-            return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_create: member %d: fast arithmetic is for single lattices", i);
-        }
         if (inst[i].p.row_base) {
             delete e;
             return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_create: member %d: row-based storage (row_base) is for "
                                              "single lattices", i);
         }
-        if (i == 0) e->sums = d.P.sums;
+        if (i == 0) {
+            e->sums = d.P.sums;
+            e->fast = d.P.fast;
+        }
         if (d.P.sums && e->model != LBM_MRT_LES) {
             delete e;
             return fail(LBM_ERR_UNSUPPORTED, "lbm_ensemble_create: compensated sums need the MRT model");
         }
-        if (inst[i].p.model != e->model || inst[i].p.bc != e->bc || d.P.sums != e->sums) {
+        if (inst[i].p.model != e->model || inst[i].p.bc != e->bc || d.P.sums != e->sums || d.P.fast != e->fast) {
             delete e;
-            return fail(LBM_ERR_ARG, "lbm_ensemble_create: instance %d differs in model, bc or sums", i);
+            return fail(LBM_ERR_ARG, "lbm_ensemble_create: instance %d differs in model, bc, sums or fast", i);
         }
         if (!inst[i].mask || !inst[i].f || (e->bc == LBM_BC_CHANNEL && !inst[i].inlet_u)) {
             delete e;

@@ -155,5 +155,9 @@ inline PairKernel pair_kernel_c(int chi)
 EnsKernel ens_kernel_sums(int fmt, int parity, bool periodic);
 PairKernel pair_kernel_sums(int fmt, bool periodic, int chi);
 PairKernel quad_kernel_sums(bool periodic, int qi);
+// the fast-arithmetic (MODEL_MRT | MODEL_FAST) instantiations, defined in lbm_ensemble_fast.hip
+EnsKernel ens_kernel_fast(int fmt, int parity, bool periodic);
+PairKernel pair_kernel_fast(int fmt, bool periodic, int chi);
+PairKernel quad_kernel_fast(bool periodic, int qi);
 
 } // namespace lbm

@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 LIB_PATH = os.path.join(HERE, "liblbm.so")
 
 SOURCES = ["lbm_step.hip", "lbm_fused.hip", "lbm_fused_f32.hip", "lbm_fused_f16.hip", "lbm_fused_bf16.hip",
-           "lbm_fused_sums.hip", "lbm_ensemble_sums.hip",
+           "lbm_fused_sums.hip", "lbm_ensemble_sums.hip", "lbm_ensemble_fast.hip",
            "lbm_ensemble.hip", "lbm_mlp.hip", "lbm_closure.hip",
            "lbm_pair.hip", "lbm_pair_rb.hip", "lbm_quad.hip", "lbm_selftest.hip", "lbm_probe.hip"]
 

@@ -176,10 +176,11 @@ class Ensemble:
 
     @classmethod
     def from_members(cls, members, storage="f32", device="cuda", blocks=0, workers=8, two_step=False,
-                     by_class=False):
+                     by_class=False, arith="exact"):
         """Build the Simulations of ``configs.ensemble`` members (periodic, MRT + LES) and group them (in size-class
         order with ``by_class``, see ``sort_by_class``).  The initial states are computed on `workers` host
-        threads, at most `workers` of them held at once."""
+        threads, at most `workers` of them held at once.  ``arith="fast"``: the hardware reciprocal and square
+        root in every member's collision (lbm_params.fast)."""
         import concurrent.futures as cf
         import numpy as np
         members = list(members)
@@ -191,7 +192,7 @@ class Ensemble:
                 batch = members[i:i + workers]
                 for m, f0 in zip(batch, ex.map(lambda mm: mm.f0(np.float32), batch)):
                     sims.append(Simulation(m.W, m.H, m.tau_inv, cs=m.cs, model="mrt", bc="periodic", f0=f0,
-                                           device=device, storage=storage))
+                                           device=device, storage=storage, arith=arith))
         return cls(sims, blocks=blocks, two_step=two_step)
 
     def pair_blocks(self, chunk, k=2):

@@ -157,3 +157,37 @@ def test_fast_rejected_where_not_built(gpu):
         Simulation(64, 32, 1.5, arith="fast", model="bgk").step(1)
     with pytest.raises(_lib.LbmError, match="fast"):
         Simulation(64, 32, 1.5, arith="fast", sums="neumaier").step(1)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_fast_ensemble_members_equal_solo(gpu, storage):
+    """An ensemble of uneven periodic lattices with the fast arithmetic: single-step, two-step and (FP32) four-step
+    launches leave every member bit-identical to its solo run with fast single steps; members must agree."""
+    from lbm.ensemble import Ensemble
+    from lbm.sim import Simulation
+    sizes = [(300, 72), (97, 41), (256, 130), (140, 33)]
+    sims, solo = [], []
+    for i, (W, H) in enumerate(sizes):
+        ux, uy = G.random_phase_velocity(W, H, 4, 0.04, seed=10 + i)
+        f0 = G.feq(np.ones((H, W)), ux, uy)
+        kw = dict(cs=0.1, bc="periodic", f0=f0, storage=storage, arith="fast")
+        sims.append(Simulation(W, H, 1.6 + 0.05 * i, **kw))
+        solo.append(Simulation(W, H, 1.6 + 0.05 * i, **kw))
+    ens = Ensemble(sims, two_step=True)
+    ens.chunk = 12
+    ens.step(4, k=2)
+    ens.step(3, k=1)
+    ens.step(1, k=1)
+    if storage == "f32":
+        ens.step(8, k=4)
+    n = 16 if storage == "f32" else 8
+    for s in solo:
+        s.step(n)
+    for m, s in zip(ens.members, solo):
+        sel = torch.from_numpy(O.observable(_op(s), s.mask_host, s.parity, ghost=True)).to(s.device)
+        assert m.parity == s.parity
+        assert int(((m.f != s.f) & sel).sum()) == 0
+    ens.close()
+    mixed = [Simulation(64, 32, 1.6, bc="periodic", arith="fast"), Simulation(64, 32, 1.6, bc="periodic")]
+    with pytest.raises(_lib.LbmError, match="fast"):
+        Ensemble(mixed)
