@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import threading
@@ -539,20 +540,44 @@ class Fp64Reference(threading.Thread):
 
 def parse_spec(spec):
     """A form's storage spec: f32, f16 or bf16; an "r" suffix on a 16-bit format stores deviations from the
-    equilibrium of the inlet profile on each row (lbm_row_base) instead of from the rest state; "+fast" computes the
+    equilibrium of the inlet profile on each row (lbm_row_base) instead of from the rest state, "rw" adds a base
+    window over the cylinder's near wake (per-cell bases, re-derived every 200 steps); "+fast" computes the
     collision with the hardware's approximate reciprocal and square root (lbm_params.fast), the reference's
-    -use_fast_math arithmetic.  Returns (storage, row_base, arith)."""
+    -use_fast_math arithmetic.  Returns (storage, row_base, arith, window)."""
     arith = "exact"
     if spec.endswith("+fast"):
         spec, arith = spec[:-5], "fast"
+    win = spec.endswith("rw") and spec != "f32"
+    if win:
+        spec = spec[:-1]
     rb = spec.endswith("r") and spec != "f32"
-    return (spec[:-1] if rb else spec), rb, arith
+    return (spec[:-1] if rb else spec), rb, arith, win
 
 
-def storage_kw(spec):
-    """Simulation keywords of a storage spec (parse_spec)."""
-    storage, rb, arith = parse_spec(spec)
-    return {"storage": storage, "row_base": "inlet" if rb else None, "arith": arith}
+def storage_kw(spec, case=None):
+    """Simulation keywords of a storage spec (parse_spec); a window needs the cylinder case it covers."""
+    from lbm import configs
+    storage, rb, arith, win = parse_spec(spec)
+    kw = {"storage": storage, "row_base": "inlet" if rb else None, "arith": arith}
+    if win:
+        kw["window"] = configs.wake_window(case)
+    return kw
+
+
+def _window_state(lat):
+    """A lattice's base-window bookkeeping (buffer order by identity, steps since the last re-derivation)."""
+    if getattr(lat, "window", None) is None:
+        return None
+    return tuple(id(t) for t in lat._wvel), lat._since_rebase
+
+
+def _set_window_state(lat, st):
+    if st is None:
+        return
+    by_id = {id(t): t for t in lat._wvel}
+    lat._wvel = [by_id[i] for i in st[0]]
+    lat._win.vel = lat._wvel[0].data_ptr()
+    lat._since_rebase = st[1]
 
 
 def form_name(kind, spec):
@@ -821,15 +846,20 @@ def main():
         nothing; it comes first, then `pre_steps` untimed steps (rounded up to a whole even number of launches
         of `unit` steps, so that the state is back in the buffer and at the parity the graph starts from) keep
         the GPU busy up to the timed region.  Returns (us per step by events, us per step by the host clock)."""
-        pre = -(-max(pre_steps, 1) // (2 * unit)) * 2 * unit
-        saved = (lat.f, lat.f_alt, lat.parity, lat.steps)
+        period = 2 * unit
+        if getattr(lat, "window", None) is not None and lat.rebase_every:
+            # whole pairs of re-derivations too: the window's base is back in the buffer the graph reads first
+            period = math.lcm(period, 2 * lat.rebase_every)
+        pre = -(-max(pre_steps, 1) // period) * period
+        saved = (lat.f, lat.f_alt, lat.parity, lat.steps, _window_state(lat))
         g = Graph()
         with g.capture():
             fn(nsteps)
-        after = (lat.f, lat.f_alt, lat.parity, lat.steps)
-        lat.f, lat.f_alt, lat.parity, lat.steps = saved
+        after = (lat.f, lat.f_alt, lat.parity, lat.steps, _window_state(lat))
+        lat.f, lat.f_alt, lat.parity, lat.steps = saved[:4]
+        _set_window_state(lat, saved[4])
         fn(pre)
-        if lat.f is not saved[0] or lat.parity != saved[2]:
+        if lat.f is not saved[0] or lat.parity != saved[2] or _window_state(lat) != saved[4]:
             raise RuntimeError("untimed steps did not return the state to the recorded buffer and parity")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -841,6 +871,7 @@ def main():
         g.close()
         lat.f, lat.f_alt, lat.parity = after[:3]
         lat.steps = after[3] + pre
+        _set_window_state(lat, after[4])
         return e0.elapsed_time(e1) * 1e3 / nsteps, wall_us
 
     # ---------------- plain FP32 single AA steps: exactly K steps, one graph replay
@@ -866,14 +897,14 @@ def main():
 
     # ---------------- the other forms: 16-bit deviation storage, and the fused kernels (autotuned)
     def make(storage):
-        return Simulation(W, H, case.tau_inv, **case.sim_kwargs(), device=dev, **storage_kw(storage))
+        return Simulation(W, H, case.tau_inv, **case.sim_kwargs(), device=dev, **storage_kw(storage, case))
 
     def make_dev(storage):
         """A lattice in `storage` from the developed state, run a further spin-up/3 steps in that storage (its
         own rounding developed too); timing only -- the errors start from the seeded state."""
         kw = dict(case.sim_kwargs())
         kw["f0"] = f_dev
-        s = Simulation(W, H, case.tau_inv, **kw, device=dev, **storage_kw(storage))
+        s = Simulation(W, H, case.tau_inv, **kw, device=dev, **storage_kw(storage, case))
         n = (spin // 3) & ~1
         if n:
             s.step(n)
@@ -917,7 +948,7 @@ def main():
         from lbm import autotune
         ks = tuple(k for k in autotune.KS if k <= args.k)
         for spec in [v.strip() for v in args.fused_storages.split(",") if v.strip()]:
-            storage, rb, arith = parse_spec(spec)
+            storage, rb, arith, win = parse_spec(spec)
             sf = sim if spec == "f32" else make_dev(spec)
             if sf.parity:  # the tuner offers the two-step tile kernel from even states only
                 sf.step(1)
@@ -935,6 +966,7 @@ def main():
             kernel = ("k_pair" if k == 2 else "k_quad") if strip == 0 else "k_fused"
             forms[name] = {"glups": cells / us / 1e3, "us_per_step": us, "k": k, "strip": strip, "chunk": chunk,
                            "kernel": kernel, "storage": storage, "spec": spec, "row_base": rb, "arith": arith,
+                           "window": list(sf.window) if win else None,
                            "tuned": {f"k{r[0]}": {"strip": r[1], "chunk": r[2], "us_per_step": round(r[3], 2)}
                                      for r in table}}
             runners[name] = (spec, k, cfg)

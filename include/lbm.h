@@ -88,7 +88,26 @@ typedef struct lbm_params {
                               floats, 16-byte aligned (lbm_row_base).  Used by lbm_step_aa_h16(_n), lbm_step_fused_h16(_n)
                               (k = 2, two-step tile kernel), lbm_macro_h16 and lbm_convert_state; FP32 entry points ignore
                               it; lbm_convert and lbm_ensemble_create reject it */
+    const struct lbm_base_window *window; /* NULL, or a window of per-cell base values (needs row_base; see
+                                             lbm_base_window) */
 } lbm_params;
+
+/*
+ * A window of per-cell base states for row-based 16-bit storage (no reference counterpart).  Cells in
+ * [x0, x0 + w) x [y0, y0 + h) take the base values of their slots from a velocity of their own instead of from
+ * their row: `vel` is a device array of two IEEE halves (ux, uy) per window cell, row-major over the window,
+ * 4-byte aligned (cell (x, y) at vel[2 ((y - y0) w + (x - x0))]), and the base value of direction q is the
+ * equilibrium deviation at rho = 1, d_q = w_q ((3 cu + 4.5 cu^2) - 1.5 (ux^2 + uy^2)) with cu = c_q.u, in FP32 from
+ * the half values (the table of lbm_row_base, computed per cell).  Where the flow departs from every per-row
+ * profile -- behind a body -- the stored deviations, and their rounding, stay small only with a local base;
+ * lbm_rebase_window re-derives it from the current flow.  The window must lie in
+ * 1 <= x <= W - 3, 1 <= y <= H - 2: the edges, and the channel's last fluid column with its outlet ghost (a copy
+ * stored against the ghost's own base), stay on their rows' base.
+ */
+typedef struct lbm_base_window {
+    int32_t x0, y0, w, h;
+    const uint16_t *vel;
+} lbm_base_window;
 
 /*
  * Per-row base state of the 16-bit formats (no reference counterpart: the reference stores FP32 only).  A 16-bit
@@ -218,6 +237,16 @@ int lbm_convert(const lbm_params *p, const void *src, int src_fmt, void *dst, in
  */
 int lbm_convert_state(const lbm_params *p, const void *src, int src_fmt, void *dst, int dst_fmt, int parity,
                       lbm_stream_t stream);
+
+/*
+ * Re-derive the base window (p->window) from the current flow: for every window cell, the new base velocity is the
+ * cell's u = j / rho (FP32, from its stored populations decoded with the current base; 0 where rho is outside
+ * (0.5, 2) or a component is not below 1 in magnitude) rounded to IEEE half, written to new_vel (the layout of
+ * lbm_base_window.vel; it must not overlap it), and the cell's 16-bit words are re-encoded against the base values
+ * it gives (decoded with the current ones, as lbm_convert_state does).  The caller then points p->window->vel at
+ * new_vel.  parity: the state's AA parity.  No reference counterpart.
+ */
+int lbm_rebase_window(const lbm_params *p, uint16_t *h, int fmt, int parity, uint16_t *new_vel, lbm_stream_t stream);
 
 /*
  * rho, ux, uy of a stored state (macro_kernel of lbm_core_smemaa_endfused.cu:377-379, extended to both

@@ -12,8 +12,11 @@ namespace lbm {
 // index wraps), which removes the per-cell boundary tests; the arithmetic is the same.  Row-based 16-bit formats
 // (row_based<FMT>) add the base value of each slot's row and direction on load and subtract it on store: the even step
 // reads and writes row y, the odd step gathers from rows y - c_qy (bounce-back: this cell's row) and scatters to rows
-// y + c_qy; the three rows' values sit in scalar registers (a wave covers part of one row).
-template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS, bool EXT = false, bool INTERIOR = false>
+// y + c_qy; the three rows' values sit in scalar registers (a wave covers part of one row).  NEARW: some cell whose
+// slots this cell reads or writes may lie in the base window (KPar::ww): each access then takes that cell's own base
+// values where it does.
+template <int MODEL, int FMT, int PARITY, bool PERIODIC, bool MACROS, bool EXT = false, bool INTERIOR = false,
+          bool NEARW = false>
 __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict__ mask, void *__restrict__ f,
                                         const float *__restrict__ inlet_u, float *__restrict__ oux,
                                         float *__restrict__ ouy, float *__restrict__ orho, int x, int y,
@@ -57,11 +60,18 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
     }
     // base value of direction q on the row y + dy (dy = -1, 0, 1)
     auto drow = [&](int dy, int q) { return dy < 0 ? dm[q] : (dy > 0 ? dp[q] : d0[q]); };
+    // base value of direction q for a slot of cell (x + dx, y + dy)
+    auto bv = [&](int q, int dx, int dy) -> float {
+        if constexpr (NEARW)
+            return window_or(P, x + dx, y + dy, q, drow(dy, q));
+        else
+            return drow(dy, q);
+    };
     if constexpr (PARITY == 0) {
 #pragma unroll
         for (int q = 0; q < 9; ++q)
             if constexpr (RB)
-                g[q] = S::ldb(rf, idx * B, (unsigned)q * plane * B, d0[q], one);
+                g[q] = S::ldb(rf, idx * B, (unsigned)q * plane * B, bv(q, 0, 0), one);
             else
                 g[q] = S::ld(rf, idx * B, (unsigned)q * plane * B, q);
     } else {
@@ -102,7 +112,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
             nb2[q] = opp(q) < q ? nbB[q] + 2u * pB : 0u; // used for q = 3, 4, 7, 8 only
         }
         if constexpr (RB)
-            g[0] = S::ldb(rf, idxB, 0, d0[0], one);
+            g[0] = S::ldb(rf, idxB, 0, bv(0, 0, 0), one);
         else
             g[0] = S::ld(rf, idxB, 0, 0);
 #pragma unroll
@@ -112,7 +122,8 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
             // o > q: slot o at the neighbour = nb2[o] + q pB, slot q here = idxB + q pB
             const unsigned off = o < q ? (solid[o] ? idx2 : nbB[o]) : (solid[o] ? idxB : nb2[o]);
             if constexpr (RB) // bounce-back: this row's slot holding direction o; else the source row's, direction q
-                g[q] = S::ldb(rf, off, (unsigned)(o < q ? o : q) * pB, solid[o] ? d0[o] : drow(-cy(q), q), one);
+                g[q] = S::ldb(rf, off, (unsigned)(o < q ? o : q) * pB, solid[o] ? bv(o, 0, 0) : bv(q, -cx(q), -cy(q)),
+                              one);
             else
                 g[q] = S::ld(rf, off, (unsigned)(o < q ? o : q) * pB, q);
         }
@@ -127,10 +138,10 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
     };
     if constexpr (PARITY == 0) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) put(idx * B, (unsigned)opp(q) * plane * B, q, g[q], d0[q]);
+        for (int q = 0; q < 9; ++q) put(idx * B, (unsigned)opp(q) * plane * B, q, g[q], bv(q, 0, 0));
         if (OPEN && x == W - 2) { // reversed copy into the outlet ghost (lbm_core4.cu:114-116)
 #pragma unroll
-            for (int q = 0; q < 9; ++q) put((idx + 1) * B, (unsigned)opp(q) * plane * B, q, g[q], d0[q]);
+            for (int q = 0; q < 9; ++q) put((idx + 1) * B, (unsigned)opp(q) * plane * B, q, g[q], bv(q, 1, 0));
         }
     } else {
         // Bounce-back writes this cell's slot opp(q), otherwise the target's slot q (again a select).  The
@@ -139,7 +150,7 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
         // the next step's Zou-He rebuilds without reading.
         const unsigned idxB = idx * B, pB = plane * B;
         const unsigned idx2 = idxB + 2u * pB;
-        put(idxB, 0, 0, g[0], d0[0]);
+        put(idxB, 0, 0, g[0], bv(0, 0, 0));
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
             const int o = opp(q);
@@ -148,13 +159,13 @@ __device__ __forceinline__ void aa_cell(const KPar &P, const uint8_t *__restrict
             // q > o: slot q at the target = nb2[q] + o pB, slot o here = idxB + o pB
             const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nb2[q]);
             // bounce-back: this row's slot that holds direction o next; else the target row's, direction q
-            put(off, (unsigned)(q < o ? q : o) * pB, q, g[q], RB ? (back ? d0[o] : drow(cy(q), q)) : 0.0f);
+            put(off, (unsigned)(q < o ? q : o) * pB, q, g[q], RB ? (back ? bv(o, 0, 0) : bv(q, cx(q), cy(q))) : 0.0f);
         }
         if (OPEN && x == W - 2) {
             // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-            put(idx * B, 3u * plane * B, 3, g[3], d0[3]);
-            if (!solid[2]) put(nb[2] * B, 6u * plane * B, 6, g[6], dp[6]);
-            if (!solid[4]) put(nb[4] * B, 7u * plane * B, 7, g[7], dm[7]);
+            put(idx * B, 3u * plane * B, 3, g[3], bv(3, 0, 0));
+            if (!solid[2]) put(nb[2] * B, 6u * plane * B, 6, g[6], bv(6, 0, 1));
+            if (!solid[4]) put(nb[4] * B, 7u * plane * B, 7, g[7], bv(7, 0, -1));
         }
     }
     if constexpr (MACROS) {
@@ -173,6 +184,13 @@ __device__ __forceinline__ void aa_tile_cell(const KPar &P, const uint8_t *__res
 {
     const int x0 = bx * TILE_X, y0 = by * TILE_Y;
     const int x = x0 + (int)threadIdx.x, y = y0 + (int)threadIdx.y;
+    if constexpr (row_based<FMT>()) {
+        // tiles whose cells or neighbours meet the base window take the per-cell base form (general tile form)
+        if (meets_window(P, x0 - 1, x0 + TILE_X, y0 - 1, y0 + TILE_Y)) {
+            aa_cell<MODEL, FMT, PARITY, PERIODIC, MACROS, false, false, true>(P, mask, f, inlet_u, oux, ouy, orho, x, y);
+            return;
+        }
+    }
     if (x0 >= 1 && x0 + TILE_X + 2 <= P.W && y0 >= 1 && y0 + TILE_Y + 1 <= P.H)
         aa_cell<MODEL, FMT, PARITY, PERIODIC, MACROS, false, true>(P, mask, f, inlet_u, oux, ouy, orho, x, y);
     else

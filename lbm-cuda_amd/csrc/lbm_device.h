@@ -42,6 +42,14 @@ struct KPar {
     float tau_inv, tau0, tau0sq, c18cs2, s_e, s_eps, s_q;
     int sums; // lbm_params.sums: compensated sums (MODEL_MRT_SUMS instantiations)
     const float *base; // lbm_params.row_base: per-row base of the row-based 16-bit formats (ROWB floats per row)
+    // lbm_params.window: cells [wx0, wx0 + ww) x [wy0, wy0 + wh) take their base values from the equilibrium of a
+    // velocity of their own, wvel (ux, uy as two IEEE halves per cell, ux in the low half, row-major over the
+    // window), instead of from their row; ww = 0: no window
+    int wx0, wy0, ww, wh;
+    const uint32_t *wvel;
+    // two-step kernel: the tiles meeting the window, [wtx0, wtx0 + wtw) x [wty0, wty0 + wth) in tile coordinates,
+    // are dispatched first (wtw = 0: none)
+    int wtx0, wty0, wtw, wth;
     int rowmap; // single step: the state (9 W H population bytes) exceeds the 256 MB MALL -> xcd_row_tile order
     int fast;   // lbm_params.fast: MODEL_FAST instantiations
 };
@@ -524,6 +532,50 @@ __device__ __forceinline__ void base_row_s(const float *base, int y, float d[9])
     const const_fptr t = (const_fptr)base + ROWB * yu;
 #pragma unroll
     for (int q = 0; q < 9; ++q) d[q] = t[q];
+}
+
+// The base window of the row-based formats: whether cell (x, y) lies in it, and its base values there
+__device__ __forceinline__ bool in_window(const KPar &P, int x, int y)
+{
+    return (unsigned)(x - P.wx0) < (unsigned)P.ww && (unsigned)(y - P.wy0) < (unsigned)P.wh;
+}
+// packed velocity of window cell (x, y), or of the nearest window cell when (x, y) lies outside it (a load from a
+// valid address either way, so that a wave's loads issue together; needs a non-empty window)
+__device__ __forceinline__ uint32_t window_vel(const KPar &P, int x, int y)
+{
+    const int ix = min(max(x - P.wx0, 0), P.ww - 1), iy = min(max(y - P.wy0, 0), P.wh - 1);
+    return P.wvel[(unsigned)(iy * P.ww + ix)];
+}
+// (ux, uy) of a packed window velocity
+__device__ __forceinline__ void unpack_vel(uint32_t v, float &ux, float &uy)
+{
+    ux = (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+    uy = (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
+}
+// The window's base value of direction q: the equilibrium deviation at rho = 1 of velocity (ux, uy),
+// w_q ((3 cu + 4.5 cu^2) - 1.5 |u|^2) with cu = c_q.u (one addition for the diagonals, exact for the axes), in this
+// order of FP32 operations everywhere (opposite directions give -cu exactly).  usq15 = 1.5 (ux ux + uy uy).
+__device__ __forceinline__ float vel_usq15(float ux, float uy) { return 1.5f * (ux * ux + uy * uy); }
+__device__ __forceinline__ float vel_base(int q, float ux, float uy, float usq15)
+{
+    const float cu = q == 0 ? 0.0f
+                            : (cx(q) == 0 ? (cy(q) > 0 ? uy : -uy)
+                                          : (cy(q) == 0 ? (cx(q) > 0 ? ux : -ux)
+                                                        : (cx(q) > 0 ? ux : -ux) + (cy(q) > 0 ? uy : -uy)));
+    return wq(q) * ((3.0f * cu + 4.5f * (cu * cu)) - usq15);
+}
+// base value q of cell (x, y) where the cell lies in the window, else `other` (branch-free; global table)
+__device__ __forceinline__ float window_or(const KPar &P, int x, int y, int q, float other)
+{
+    float ux, uy;
+    unpack_vel(window_vel(P, x, y), ux, uy);
+    const float v = vel_base(q, ux, uy, vel_usq15(ux, uy));
+    return in_window(P, x, y) ? v : other;
+}
+// whether the rectangle [x0, x1] x [y0, y1] meets the window
+__device__ __forceinline__ bool meets_window(const KPar &P, int x0, int x1, int y0, int y1)
+{
+    return P.ww > 0 && x1 >= P.wx0 && x0 < P.wx0 + P.ww && y1 >= P.wy0 && y0 < P.wy0 + P.wh;
 }
 
 // 1.0 in a VGPR the compiler cannot see through (the multiplier of the FP16 ldb form)

@@ -23,36 +23,41 @@ namespace lbm {
 namespace pairk {
 
 template <int MODEL, bool PERIODIC, int FMT, int CH>
-__global__ __launch_bounds__(NT) void k_pair(KPar P, const uint8_t *__restrict__ mask, const void *__restrict__ fin,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_pair(KPar P, const uint8_t *__restrict__ mask, const void *__restrict__ fin,
                                              void *__restrict__ fout, const float *__restrict__ inlet_u,
                                              float *__restrict__ oux, float *__restrict__ ouy,
                                              float *__restrict__ orho)
 {
     __shared__ TileLds<FMT, CH> L;
+    extern __shared__ uint32_t pair_wl[]; // with a base window: (CH + 2) x RW packed velocities (pair_tile WIN)
     int bx, by;
-    xcd_tile(bx, by);
-    pair_tile_any<MODEL, PERIODIC, FMT, CH>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L);
+    if constexpr (row_based<FMT>())
+        pair_tile_order(P, bx, by);
+    else
+        xcd_tile(bx, by);
+    pair_tile_any<MODEL, PERIODIC, FMT, CH>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L, pair_wl);
 }
 
 template <int MODEL, bool PERIODIC, int FMT>
-static void launch_ch(const KPar &P, int CH, dim3 grid, hipStream_t st, const uint8_t *mask, const void *fin,
+static void launch_ch(const KPar &P, int CH, dim3 grid, unsigned dyn, hipStream_t st, const uint8_t *mask,
+                      const void *fin,
                       void *fout, const float *inlet_u, float *ux, float *uy, float *rho)
 {
     switch (CH) {
     case 24:
-        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 24>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
+        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 24>), grid, dim3(NT), dyn, st, P, mask, fin, fout, inlet_u, ux,
                            uy, rho);
         break;
     case 16:
-        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 16>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
+        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 16>), grid, dim3(NT), dyn, st, P, mask, fin, fout, inlet_u, ux,
                            uy, rho);
         break;
     case 12:
-        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 12>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
+        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 12>), grid, dim3(NT), dyn, st, P, mask, fin, fout, inlet_u, ux,
                            uy, rho);
         break;
     default:
-        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 8>), grid, dim3(NT), 0, st, P, mask, fin, fout, inlet_u, ux,
+        hipLaunchKernelGGL((k_pair<MODEL, PERIODIC, FMT, 8>), grid, dim3(NT), dyn, st, P, mask, fin, fout, inlet_u, ux,
                            uy, rho);
     }
 }
@@ -67,38 +72,53 @@ int launch_pair_fmt(const KPar &P, int model, int bc, int CH, hipStream_t st, co
     using namespace pairk;
     const dim3 grid((P.W + CW - 1) / CW, (P.H + CH - 1) / CH);
     const bool per = bc == LBM_BC_PERIODIC;
+    // a base window: the rectangle of tiles whose region [x0 - 1, x0 + CW] x [y0 - 1, y0 + CH] meets it (the tiles
+    // that take the window form, dispatched first) and the dynamic LDS for their staged velocities
+    KPar Q = P;
+    Q.wtx0 = Q.wty0 = Q.wtw = Q.wth = 0;
+    unsigned dyn = 0;
+    if (row_based<FMT>() && P.ww > 0) {
+        const int bx0 = P.wx0 <= CW ? 0 : (P.wx0 - 1) / CW, by0 = P.wy0 <= CH ? 0 : (P.wy0 - 1) / CH;
+        const int bx1 = std::min((int)grid.x - 1, (P.wx0 + P.ww) / CW);
+        const int by1 = std::min((int)grid.y - 1, (P.wy0 + P.wh) / CH);
+        Q.wtx0 = bx0;
+        Q.wty0 = by0;
+        Q.wtw = bx1 - bx0 + 1;
+        Q.wth = by1 - by0 + 1;
+        dyn = (unsigned)((CH + 2) * RW * sizeof(uint32_t));
+    }
     if (model == LBM_BGK) {
         if (per)
-            launch_ch<MODEL_BGK, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            launch_ch<MODEL_BGK, true, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
         else
-            launch_ch<MODEL_BGK, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            launch_ch<MODEL_BGK, false, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
     } else if (P.sums) { // compensated sums read the clamp setting at run time
         if (per)
-            launch_ch<MODEL_MRT_SUMS, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            launch_ch<MODEL_MRT_SUMS, true, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
         else
-            launch_ch<MODEL_MRT_SUMS, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            launch_ch<MODEL_MRT_SUMS, false, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
     } else if (P.fast) { // hardware reciprocal and square root (lbm_params.fast)
         if (P.clamp) {
             if (per)
-                launch_ch<MODEL_MRT_CLAMP1 | MODEL_FAST, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+                launch_ch<MODEL_MRT_CLAMP1 | MODEL_FAST, true, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
             else
-                launch_ch<MODEL_MRT_CLAMP1 | MODEL_FAST, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+                launch_ch<MODEL_MRT_CLAMP1 | MODEL_FAST, false, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
         } else {
             if (per)
-                launch_ch<MODEL_MRT_CLAMP0 | MODEL_FAST, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+                launch_ch<MODEL_MRT_CLAMP0 | MODEL_FAST, true, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
             else
-                launch_ch<MODEL_MRT_CLAMP0 | MODEL_FAST, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+                launch_ch<MODEL_MRT_CLAMP0 | MODEL_FAST, false, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
         }
     } else if (P.clamp) { // the clamp setting as a template argument: no run-time select per cell
         if (per)
-            launch_ch<MODEL_MRT_CLAMP1, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            launch_ch<MODEL_MRT_CLAMP1, true, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
         else
-            launch_ch<MODEL_MRT_CLAMP1, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            launch_ch<MODEL_MRT_CLAMP1, false, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
     } else {
         if (per)
-            launch_ch<MODEL_MRT_CLAMP0, true, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            launch_ch<MODEL_MRT_CLAMP0, true, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
         else
-            launch_ch<MODEL_MRT_CLAMP0, false, FMT>(P, CH, grid, st, mask, fin, fout, inlet_u, ux, uy, rho);
+            launch_ch<MODEL_MRT_CLAMP0, false, FMT>(Q, CH, grid, dyn, st, mask, fin, fout, inlet_u, ux, uy, rho);
     }
     return LBM_OK;
 }

@@ -56,12 +56,15 @@ template <int FMT, int CH> struct TileLds : TileBaseRows<row_based<FMT>(), CH + 
 // Row-based 16-bit formats (row_based<FMT>): the region rows' base values are staged in LDS for phase A (whose
 // cells' rows vary within a wave) and read into scalar registers for phase B (a wave covers half a core row); the
 // LDS copy of the even step's result is stored relative to the base as in HBM, so both phases round as the single
-// step does.
-template <int MODEL, bool PERIODIC, int FMT, int CH, bool INTERIOR>
+// step does.  WIN (row-based formats, general tile form): the tile's region meets the base window (KPar::ww), whose
+// cells take their base values from their own velocity instead of their row: the region's packed window velocities
+// are staged in `wl` (RH x RW words of dynamic LDS, allocated by the launch when there is a window) with the row
+// bases, and every base value a window cell needs is computed from them (vel_base).
+template <int MODEL, bool PERIODIC, int FMT, int CH, bool INTERIOR, bool WIN = false>
 __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restrict__ mask, const void *__restrict__ fin,
                                           void *__restrict__ fout, const float *__restrict__ inlet_u,
                                           float *__restrict__ oux, float *__restrict__ ouy, float *__restrict__ orho,
-                                          int bx, int by, TileLds<FMT, CH> &L)
+                                          int bx, int by, TileLds<FMT, CH> &L, uint32_t *wl = nullptr)
 {
     using S = Store<FMT>;
     using C = Conv<FMT>;
@@ -98,6 +101,18 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             else if constexpr (OPEN)
                 ok = ok && (unsigned)gy < (unsigned)H;
             bstage = ok ? P.base[ROWB * gy + q] : 0.0f;
+        }
+    }
+    // WIN: the packed velocities of the region cells (region coordinates as they are: the window lies inside the
+    // lattice, so wrapped or outside cells are never window cells), fetched with the row bases
+    constexpr int NWS = WIN ? (RH * RW + NT - 1) / NT : 1;
+    uint32_t wstage[NWS];
+    if constexpr (WIN) {
+#pragma unroll
+        for (int i = 0; i < NWS; ++i) {
+            const int e = t + i * NT;
+            const int r = e / RW, c = e - r * RW;
+            wstage[i] = window_vel(P, x0 - 1 + c, y0 - 1 + r);
         }
     }
     // the staged group g (0, 1, 2) of region row rr, and the base value of direction q on region row rr
@@ -169,6 +184,14 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             const f32x4_t b[3] = {bgroup(r, 0), bgroup(r, 1), bgroup(r, 2)};
 #pragma unroll
             for (int q = 0; q < 9; ++q) db[q] = b[bgrp(q)][bidx(q)];
+            if constexpr (WIN) { // region coordinates as they are: the window lies inside the lattice
+                const bool inw = in_window(P, x0 - 1 + c, y0 - 1 + r);
+                float ux, uy;
+                unpack_vel(wl[r * RW + c], ux, uy);
+                const float us = vel_usq15(ux, uy);
+#pragma unroll
+                for (int q = 0; q < 9; ++q) db[q] = inw ? vel_base(q, ux, uy, us) : db[q];
+            }
         };
         if constexpr (RB) {
             read_base();
@@ -212,6 +235,11 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
     for (int a = 0; a < PD; ++a) load_cell(a);
     if constexpr (RB) {
         if (t < ROWB * RH) L.base[t / ROWB][t % ROWB] = bstage;
+        if constexpr (WIN) {
+#pragma unroll
+            for (int i = 0; i < NWS; ++i)
+                if (t + i * NT < RH * RW) wl[t + i * NT] = wstage[i];
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -260,10 +288,20 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                 solid[q] = !FLUID && smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(OPEN && inlet && cx(q) < 0);
             // row-based formats: base value of direction q on region row r + dy (the staged rows; read where used,
             // with one address for the whole wave)
-            auto drow = [&](int dy, int q) -> float { return bgroup(r + dy, bgrp(q))[bidx(q)]; };
+            auto drow0 = [&](int dy, int q) -> float { return bgroup(r + dy, bgrp(q))[bidx(q)]; };
+            // base value of direction q for a slot of cell (gx + dx, gy + dy) (WIN: that cell's own in the window)
+            auto bv = [&](int q, int dx, int dy) -> float {
+                if constexpr (WIN) {
+                    float ux, uy;
+                    unpack_vel(wl[(r + dy) * RW + (c + dx)], ux, uy);
+                    return in_window(P, gx + dx, gy + dy) ? vel_base(q, ux, uy, vel_usq15(ux, uy)) : drow0(dy, q);
+                } else {
+                    return drow0(dy, q);
+                }
+            };
             float g[9], mac[3];
             if constexpr (RB)
-                g[0] = C::ldb(at(0, 0, 0), drow(0, 0), one);
+                g[0] = C::ldb(at(0, 0, 0), bv(0, 0, 0), one);
             else
                 g[0] = C::ld(at(0, 0, 0), 0);
 #pragma unroll
@@ -273,7 +311,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                 // At the inlet the q with c_qx > 0 come from x = -1 and are rebuilt by Zou-He, which never reads them.
                 const T own = at(q, 0, 0), src = at(o, -cy(q), -cx(q));
                 if constexpr (RB) // own: direction o on this row; source: direction q on row gy - c_qy
-                    g[q] = C::ldb(solid[o] ? own : src, solid[o] ? drow(0, o) : drow(-cy(q), q), one);
+                    g[q] = C::ldb(solid[o] ? own : src, solid[o] ? bv(o, 0, 0) : bv(q, -cx(q), -cy(q)), one);
                 else
                     g[q] = C::ld(solid[o] ? own : src, q);
             }
@@ -298,14 +336,15 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                     for (int q = 0; q < 8; q += 2) {
                         unsigned pk;
                         if constexpr (RB) // direction q lands on row gy + c_qy
-                            pk = C::st2b(g[q], q, drow(cy(q), q), g[q + 1], q + 1, drow(cy(q + 1), q + 1), K);
+                            pk = C::st2b(g[q], q, bv(q, cx(q), cy(q)), g[q + 1], q + 1,
+                                         bv(q + 1, cx(q + 1), cy(q + 1)), K);
                         else
                             pk = C::st2(g[q], q, g[q + 1], q + 1, K);
                         put(q, (uint16_t)pk);
                         put(q + 1, (uint16_t)(pk >> 16));
                     }
                     if constexpr (RB)
-                        put(8, C::stb(g[8], 8, drow(cy(8), 8), K));
+                        put(8, C::stb(g[8], 8, bv(8, cx(8), cy(8)), K));
                     else
                         put(8, C::st(g[8], 8, K));
                 } else {
@@ -333,7 +372,7 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                     else
                         S::st(rout, voff, soff, q, h);
                 };
-                put(idxB, 0, 0, g[0], drow(0, 0));
+                put(idxB, 0, 0, g[0], bv(0, 0, 0));
 #pragma unroll
                 for (int q = 1; q < 9; ++q) {
                     // bounce-back writes this cell's slot opp(q); populations leaving through the open inlet are parked
@@ -343,13 +382,13 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
                     const int o = opp(q);
                     const bool back = solid[q] || (OPEN && inlet && cx(q) < 0);
                     const unsigned off = q < o ? (back ? idx2 : nbB[q]) : (back ? idxB : nbB[q] + 2u * pB);
-                    put(off, (unsigned)(q < o ? q : o) * pB, q, g[q], back ? drow(0, o) : drow(cy(q), q));
+                    put(off, (unsigned)(q < o ? q : o) * pB, q, g[q], back ? bv(o, 0, 0) : bv(q, cx(q), cy(q)));
                 }
                 if (OPEN && gx == W - 2) {
                     // the ghost (W-1, y) is a copy of this cell: stream its q = 3, 6, 7 into column W-2
-                    put(idxB, 3u * pB, 3, g[3], drow(0, 3));
-                    if (!solid[2]) put(nbB[2], 6u * pB, 6, g[6], drow(1, 6));
-                    if (!solid[4]) put(nbB[4], 7u * pB, 7, g[7], drow(-1, 7));
+                    put(idxB, 3u * pB, 3, g[3], bv(3, 0, 0));
+                    if (!solid[2]) put(nbB[2], 6u * pB, 6, g[6], bv(6, 0, 1));
+                    if (!solid[4]) put(nbB[4], 7u * pB, 7, g[7], bv(7, 0, -1));
                 }
             }
             if (orho) {
@@ -380,14 +419,54 @@ template <int CH> __device__ __forceinline__ bool pair_interior(int W, int H, in
     return x0 >= 2 && x0 + CW + 3 <= W && y0 >= 2 && y0 + CH + 2 <= H;
 }
 
+// Tile of this workgroup: the tiles meeting the base window (KPar::wt*, a rectangle of tiles) first, one per
+// workgroup in dispatch order, so that their longer runs overlap the rest of the launch instead of trailing it;
+// then the others in xcd_tile's banded order (eight contiguous bands, one per XCD), counted over the tiles outside
+// the rectangle.  Without a window it is xcd_tile.
+__device__ __forceinline__ void pair_tile_order(const KPar &P, int &bx, int &by)
+{
+    const int nbx = (int)gridDim.x, nb = (int)(gridDim.x * gridDim.y);
+    const int b = (int)blockIdx.y * nbx + (int)blockIdx.x;
+    const int nw = P.wtw * P.wth;
+    if (b < nw) {
+        bx = P.wtx0 + b % P.wtw;
+        by = P.wty0 + b / P.wtw;
+        return;
+    }
+    const int j = b - nw, per = (nb - nw) / 8;
+    int t = j < per * 8 ? (j % 8) * per + j / 8 : j; // index among the tiles outside the rectangle, row-major
+    const int above = P.wty0 * nbx, rest = nbx - P.wtw;
+    if (t >= above) {
+        const int k = t - above;
+        if (k < P.wth * rest) { // a rectangle row (rest > 0 here)
+            by = P.wty0 + k / rest;
+            const int i = k % rest;
+            bx = i < P.wtx0 ? i : i + P.wtw;
+            return;
+        }
+        t += nw; // below the rectangle
+    }
+    bx = t % nbx;
+    by = t / nbx;
+}
+
 // One tile, with the interior form where it applies (a workgroup-uniform branch).
 template <int MODEL, bool PERIODIC, int FMT, int CH>
 __device__ __forceinline__ void pair_tile_any(const KPar &P, const uint8_t *__restrict__ mask,
                                               const void *__restrict__ fin, void *__restrict__ fout,
                                               const float *__restrict__ inlet_u, float *__restrict__ oux,
                                               float *__restrict__ ouy, float *__restrict__ orho, int bx, int by,
-                                              TileLds<FMT, CH> &L)
+                                              TileLds<FMT, CH> &L, uint32_t *wl = nullptr)
 {
+    if constexpr (row_based<FMT>()) {
+        // tiles whose region meets the base window: the general form with per-cell base values
+        const int x0 = bx * CW, y0 = by * CH;
+        if (meets_window(P, x0 - 1, x0 + CW, y0 - 1, y0 + CH)) {
+            pair_tile<MODEL, PERIODIC, FMT, CH, false, true>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L,
+                                                             wl);
+            return;
+        }
+    }
     if (pair_interior<CH>(P.W, P.H, bx, by))
         pair_tile<MODEL, PERIODIC, FMT, CH, true>(P, mask, fin, fout, inlet_u, oux, ouy, orho, bx, by, L);
     else

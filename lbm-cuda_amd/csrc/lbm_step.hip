@@ -81,6 +81,26 @@ int make_kpar(const lbm_params *p, KPar *k, unsigned pop_bytes)
     if (p->row_base && (reinterpret_cast<uintptr_t>(p->row_base) & 15u))
         return fail(LBM_ERR_ARG, "row_base must be 16-byte aligned");
     k->base = p->row_base;
+    k->wx0 = k->wy0 = k->ww = k->wh = 0;
+    k->wvel = nullptr;
+    k->wtx0 = k->wty0 = k->wtw = k->wth = 0;
+    if (p->window) {
+        const lbm_base_window &w = *p->window;
+        if (!p->row_base) return fail(LBM_ERR_ARG, "a base window needs row_base (the base outside the window)");
+        if (!w.vel || (reinterpret_cast<uintptr_t>(w.vel) & 3u))
+            return fail(LBM_ERR_ARG, "the window's velocity table must be non-NULL and 4-byte aligned");
+        // not x = W - 2: the two-step kernel's outlet ghost is a copy of that column's words, valid only when
+        // both columns share their rows' base
+        if (w.w < 1 || w.h < 1 || w.x0 < 1 || w.y0 < 1 || (long long)w.x0 + w.w > p->W - 2 ||
+            (long long)w.y0 + w.h > p->H - 1)
+            return fail(LBM_ERR_ARG, "base window [%d, %d) x [%d, %d) must lie in 1 <= x <= W - 3, 1 <= y <= H - 2",
+                        w.x0, w.x0 + w.w, w.y0, w.y0 + w.h);
+        k->wx0 = w.x0;
+        k->wy0 = w.y0;
+        k->ww = w.w;
+        k->wh = w.h;
+        k->wvel = reinterpret_cast<const uint32_t *>(w.vel);
+    }
     // measured on the cylinder case: the banded order is 1-2 % faster while the state fits the MALL (226 MB),
     // the row order 5-10 % faster beyond it (302 MB and up)
     k->rowmap = (long long)p->W * p->H * 9 * pop_bytes > (256LL << 20) ? 1 : 0;
@@ -237,9 +257,17 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_macro(KPar P, const uint8_t 
     }
     float g[9];
     if constexpr (row_based<FMT>()) {
-        // the slot holding direction q lies on this row: base value d[q] (a wave covers part of one row)
+        // the slot holding direction q lies on this row: base value d[q] (a wave covers part of one row), or the
+        // cell's own in the base window
         float d[9];
         base_row_s(P.base, y, d);
+        if (in_window(P, x, y)) {
+            float wx, wy;
+            unpack_vel(window_vel(P, x, y), wx, wy);
+            const float us = vel_usq15(wx, wy);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) d[q] = vel_base(q, wx, wy, us);
+        }
 #pragma unroll
         for (int q = 0; q < 9; ++q) g[q] = Conv<FMT>::popb(f[(parity ? opp(q) : q) * P.plane + idx], q, d[q]);
     } else {
@@ -311,13 +339,23 @@ __global__ __launch_bounds__(256) void k_convert(unsigned plane, const typename 
 // Conversion with a per-row base on either side (row-based 16-bit formats): slot p of a cell on row y holds direction
 // p (parity 0) or opp(p) (parity 1), whose base value on that row is applied.
 template <int SRC, int DST>
-__global__ __launch_bounds__(256) void k_convert_rb(unsigned plane, unsigned W, const float *__restrict__ base,
-                                                     int parity, const typename Conv<SRC>::T *__restrict__ src,
+__global__ __launch_bounds__(256) void k_convert_rb(KPar P, int parity, const typename Conv<SRC>::T *__restrict__ src,
                                                      typename Conv<DST>::T *__restrict__ dst)
 {
+    const unsigned plane = (unsigned)P.plane;
     const unsigned i = blockIdx.x * 256u + threadIdx.x;
     if (i >= plane) return;
-    const float *bt = base + ROWB * (i / W);
+    const int y = (int)(i / (unsigned)P.W), x = (int)(i - (unsigned)y * (unsigned)P.W);
+    float bt[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) bt[q] = P.base[ROWB * y + q];
+    if (in_window(P, x, y)) {
+        float wx, wy;
+        unpack_vel(window_vel(P, x, y), wx, wy);
+        const float us = vel_usq15(wx, wy);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) bt[q] = vel_base(q, wx, wy, us);
+    }
 #pragma unroll
     for (int p = 0; p < 9; ++p) {
         const int q = parity ? opp(p) : p;
@@ -332,6 +370,41 @@ __global__ __launch_bounds__(256) void k_convert_rb(unsigned plane, unsigned W, 
         else
             dst[(unsigned)p * plane + i] = Conv<DST>::from_pop(f, q);
     }
+}
+
+// Base window re-derivation: per window cell, decode its nine words with the current base, take the cell's velocity
+// (0 where rho is outside (0.5, 2) or a component is not below 1 in magnitude), rounded to IEEE half, as its new
+// base velocity, store it to nv and re-encode the words against the base values it gives (vel_base).  Each cell
+// touches only its own slots (in either parity slot q of a cell holds one of that cell's populations), so cells are
+// independent.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_rebase(KPar P, int parity, uint16_t *__restrict__ h, uint32_t *__restrict__ nv)
+{
+    using C = Conv<FMT>;
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= (unsigned)(P.ww * P.wh)) return;
+    const int x = P.wx0 + (int)(i % (unsigned)P.ww), y = P.wy0 + (int)(i / (unsigned)P.ww);
+    const unsigned idx = (unsigned)(y * P.W + x), plane = (unsigned)P.plane;
+    float ox, oy;
+    unpack_vel(P.wvel[i], ox, oy);
+    const float ous = vel_usq15(ox, oy);
+    float f[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        f[q] = C::popb(h[(unsigned)(parity ? opp(q) : q) * plane + idx], q, vel_base(q, ox, oy, ous));
+    const float rho = (((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7]))) + f[8];
+    const float jx = ((f[1] - f[3]) + (f[5] - f[6])) + (f[8] - f[7]);
+    const float jy = ((f[2] - f[4]) + (f[5] + f[6])) - (f[7] + f[8]);
+    const float ux = __fdiv_rn(jx, rho), uy = __fdiv_rn(jy, rho);
+    const bool ok = rho > 0.5f && rho < 2.0f && __builtin_fabsf(ux) < 1.0f && __builtin_fabsf(uy) < 1.0f;
+    const _Float16 hx = ok ? (_Float16)ux : (_Float16)0.0f, hy = ok ? (_Float16)uy : (_Float16)0.0f;
+    const uint32_t packed = (uint32_t)__builtin_bit_cast(uint16_t, hx) | ((uint32_t)__builtin_bit_cast(uint16_t, hy) << 16);
+    nv[i] = packed;
+    float nx, ny;
+    unpack_vel(packed, nx, ny);
+    const float nus = vel_usq15(nx, ny);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) h[(unsigned)(parity ? opp(q) : q) * plane + idx] = C::from_popb(f[q], q, vel_base(q, nx, ny, nus));
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -473,7 +546,7 @@ using namespace lbm;
 
 extern "C" {
 
-int lbm_abi_version(void) { return 4; }
+int lbm_abi_version(void) { return 5; }
 
 const char *lbm_last_error(void) { return g_err; }
 
@@ -710,11 +783,10 @@ int lbm_convert_state(const lbm_params *p, const void *src, int src_fmt, void *d
     }
     const dim3 grid((P.plane + 255) / 256), block(256);
     hipStream_t s = to_hip(stream);
-    const unsigned plane = (unsigned)P.plane, W = (unsigned)P.W;
     // the 16-bit sides are row-based
 #define LBM_CONV(S, D, KS, KD)                                                                                          \
     if (src_fmt == S && dst_fmt == D) {                                                                                 \
-        hipLaunchKernelGGL((k_convert_rb<KS, KD>), grid, block, 0, s, plane, W, P.base, parity,                        \
+        hipLaunchKernelGGL((k_convert_rb<KS, KD>), grid, block, 0, s, P, parity,                                       \
                            static_cast<const typename Conv<KS>::T *>(src), static_cast<typename Conv<KD>::T *>(dst));   \
         return check_launch("lbm_convert_state");                                                                       \
     }
@@ -729,6 +801,32 @@ int lbm_convert_state(const lbm_params *p, const void *src, int src_fmt, void *d
     const hipError_t e = hipMemcpyAsync(dst, src, 9ull * P.plane * bytes(src_fmt), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return fail((int)e, "lbm_convert_state: copy failed: %s", hipGetErrorString(e));
     return LBM_OK;
+}
+
+int lbm_rebase_window(const lbm_params *p, uint16_t *h, int fmt, int parity, uint16_t *new_vel, lbm_stream_t stream)
+{
+    KPar P;
+    int rc = make_kpar(p, &P, 2);
+    if (rc) return rc;
+    if (!p->window) return fail(LBM_ERR_ARG, "lbm_rebase_window: p->window is NULL");
+    if (!h || !new_vel || (reinterpret_cast<uintptr_t>(new_vel) & 3u))
+        return fail(LBM_ERR_ARG, "lbm_rebase_window: h and a 4-byte aligned new_vel are required");
+    if (fmt != LBM_FMT_F16 && fmt != LBM_FMT_BF16) return fail(LBM_ERR_ARG, "lbm_rebase_window: bad format %d", fmt);
+    if (parity != 0 && parity != 1) return fail(LBM_ERR_ARG, "lbm_rebase_window: parity must be 0 or 1");
+    {
+        const char *a = reinterpret_cast<const char *>(P.wvel), *b = reinterpret_cast<const char *>(new_vel);
+        const size_t n = sizeof(uint32_t) * (size_t)P.ww * (size_t)P.wh;
+        if (a < b + n && b < a + n) return fail(LBM_ERR_ARG, "lbm_rebase_window: new_vel overlaps the current table");
+    }
+    const unsigned cells = (unsigned)P.ww * (unsigned)P.wh;
+    const dim3 grid((cells + 255) / 256), block(256);
+    if (fmt == LBM_FMT_F16)
+        hipLaunchKernelGGL((k_rebase<FMT_F16R>), grid, block, 0, to_hip(stream), P, parity, h,
+                           reinterpret_cast<uint32_t *>(new_vel));
+    else
+        hipLaunchKernelGGL((k_rebase<FMT_BF16R>), grid, block, 0, to_hip(stream), P, parity, h,
+                           reinterpret_cast<uint32_t *>(new_vel));
+    return check_launch("lbm_rebase_window");
 }
 
 int lbm_row_base(int H, const float *ux, const float *uy, float *table)

@@ -15,8 +15,15 @@ MRT_LES, BGK = 0, 1
 CHANNEL, PERIODIC = 0, 1
 FMT_F32, FMT_F16, FMT_BF16 = 0, 1, 2
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 ROW_BASE_STRIDE = 16  # LBM_ROW_BASE_STRIDE
+
+
+class LbmBaseWindow(ctypes.Structure):
+    """Mirror of ``lbm_base_window`` (include/lbm.h)."""
+
+    _fields_ = [("x0", ctypes.c_int32), ("y0", ctypes.c_int32), ("w", ctypes.c_int32), ("h", ctypes.c_int32),
+                ("vel", ctypes.c_void_p)]
 
 
 class LbmParams(ctypes.Structure):
@@ -36,6 +43,7 @@ class LbmParams(ctypes.Structure):
         ("sums", ctypes.c_int32),
         ("fast", ctypes.c_int32),
         ("row_base", ctypes.c_void_p),
+        ("window", ctypes.POINTER(LbmBaseWindow)),
     ]
 
 
@@ -87,6 +95,7 @@ SIGNATURES = {
     "lbm_convert": (_i, [_P, _vp, _i, _vp, _i, _vp]),
     "lbm_convert_state": (_i, [_P, _vp, _i, _vp, _i, _i, _vp]),
     "lbm_row_base": (_i, [_i, _vp, _vp, _vp]),
+    "lbm_rebase_window": (_i, [_P, _vp, _i, _i, _vp, _vp]),
     "lbm_visualize": (_i, [_P, _vp, _vp, _vp, _vp, _f, _vp]),
     "lbm_diagnostics_work_size": (ctypes.c_size_t, [_P]),
     "lbm_diagnostics": (_i, [_P, _vp, _vp, _vp, _i, _vp, _vp, _vp]),

@@ -32,7 +32,9 @@ def _key(sim, k):
     p = sim.params
     dev = torch.cuda.get_device_name(sim.device)
     rb = int(bool(p.row_base)) and sim.fmt != _lib.FMT_F32
-    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|fmt{sim.fmt}|rb{rb}|fast{p.fast}|k{k}"
+    win = "" if not rb or not p.window else "|win{0}x{1}+{2}+{3}".format(*(getattr(p.window.contents, a)
+                                                                          for a in ("w", "h", "x0", "y0")))
+    return f"{dev}|{p.W}x{p.H}|bc{p.bc}|model{p.model}|les{int(p.cs != 0)}|fmt{sim.fmt}|rb{rb}{win}|fast{p.fast}|k{k}"
 
 
 def load_cache():

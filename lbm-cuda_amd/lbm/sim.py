@@ -22,6 +22,7 @@ from . import geometry as geo
 _MODELS = {"mrt": _lib.MRT_LES, "mrt_les": _lib.MRT_LES, "bgk": _lib.BGK}
 _BCS = {"channel": _lib.CHANNEL, "periodic": _lib.PERIODIC}
 _FMTS = {"f32": _lib.FMT_F32, "f16": _lib.FMT_F16, "bf16": _lib.FMT_BF16}
+_CX, _CY, _OPP = geo.CX.astype(np.float64), geo.CY.astype(np.float64), geo.OPP
 
 
 class LbmHealthError(RuntimeError):
@@ -52,6 +53,11 @@ class Simulation:
     a ready table [H, 16] (lbm_row_base's layout).  The blocked form is then the two-step kernel (k = 2, even
     parity); single steps take either parity.
 
+    ``window`` = (x0, y0, w, h) (with ``row_base``): inside that rectangle the base is a per-cell one, the
+    equilibrium of each cell's own velocity, re-derived from the state every ``rebase_every`` steps (0: only by
+    ``rebase()``) -- for the region where the flow departs most from the rows' base (a near wake).  The window lies
+    in 1 <= x <= W - 3, 1 <= y <= H - 2; ``rebase_every`` must be a multiple of the k of the launches used.
+
     ``arith="fast"`` (MRT only): the collision uses the hardware's approximate reciprocal and square root (within 1
     ulp), as the reference's -use_fast_math build does; single steps and the tile kernels (k = 2, 4 from even
     parity) take it, each blocked form bit-identical to its single steps.
@@ -59,7 +65,7 @@ class Simulation:
 
     def __init__(self, W, H, tau_inv, *, cs=0.0, model="mrt", bc="channel", mask=None, inlet_u=None, f0=None,
                  clamp=True, s_e=1.1, s_eps=1.1, s_q=1.1, parity=0, device="cuda", storage="f32", sums="plain",
-                 row_base=None, arith="exact"):
+                 row_base=None, arith="exact", window=None, rebase_every=200):
         self.lib = _lib.load()
         self.W, self.H = int(W), int(H)
         self.model = _MODELS[model] if isinstance(model, str) else int(model)
@@ -99,6 +105,29 @@ class Simulation:
             self.row_base_host = self._base_table(row_base)
             self._row_base = torch.from_numpy(self.row_base_host).to(self.device)  # 256-byte aligned
             self.params.row_base = self._row_base.data_ptr()
+        self.window = None
+        self.rebase_every = int(rebase_every)
+        self._since_rebase = 0
+        if window is not None:
+            if self.row_base_host is None:
+                raise ValueError("a base window needs 16-bit storage with row_base")
+            x0, y0, w, h = (int(v) for v in window)
+            if not (1 <= x0 and x0 + w <= self.W - 2 and 1 <= y0 and y0 + h <= self.H - 1 and w > 0 and h > 0):
+                raise ValueError(f"window {window} must lie in 1 <= x <= W - 3, 1 <= y <= H - 2")
+            self.window = (x0, y0, w, h)
+            # two velocity tables [h, w, 2] of IEEE halves (lbm_rebase_window reads one and writes the other); the
+            # first holds each window cell's velocity in f0
+            fw = np.asarray(f0, np.float64)[:, y0:y0 + h, x0:x0 + w]
+            fw = fw[_OPP] if self.parity else fw
+            rho = fw.sum(0)
+            vx = (fw * _CX[:, None, None]).sum(0) / rho
+            vy = (fw * _CY[:, None, None]).sum(0) / rho
+            ok = (rho > 0.5) & (rho < 2.0) & (np.abs(vx) < 1.0) & (np.abs(vy) < 1.0)
+            vel = np.stack([np.where(ok, vx, 0.0), np.where(ok, vy, 0.0)], -1).astype(np.float16)
+            self._wvel = [torch.from_numpy(vel).to(self.device), torch.empty((h, w, 2), dtype=torch.float16,
+                                                                              device=self.device)]
+            self._win = _lib.LbmBaseWindow(x0, y0, w, h, self._wvel[0].data_ptr())
+            self.params.window = ctypes.pointer(self._win)
         f32 = torch.from_numpy(np.ascontiguousarray(f0, dtype=np.float32)).to(self.device)
         if self.fmt == _lib.FMT_F32:
             self.f = f32
@@ -132,14 +161,47 @@ class Simulation:
             return _lib.row_base_table(a)
         raise ValueError(f"row_base shape {a.shape}: expected ({self.H},) or ({self.H}, {_lib.ROW_BASE_STRIDE})")
 
+    def window_vel_host(self):
+        """The base window's current per-cell base velocities [h, w, 2] (float16; None without a window)."""
+        return None if self.window is None else self._wvel[0].cpu().numpy()
+
+    def rebase(self):
+        """Re-derive the base window's per-cell base velocities from the stored state (lbm_rebase_window): the
+        stored words are re-encoded against the equilibrium of each window cell's current velocity."""
+        if self.window is None:
+            return self
+        rc = self.lib.lbm_rebase_window(ctypes.byref(self.params), _ptr(self.f), self.fmt, self.parity,
+                                        _ptr(self._wvel[1]), _stream())
+        _lib.check(rc, "lbm_rebase_window")
+        self._wvel.reverse()
+        self._win.vel = self._wvel[0].data_ptr()
+        self._since_rebase = 0
+        return self
+
     # ------------------------------------------------------------------ stepping
     def step(self, n=1, k=1, macros=False, cfg=None):
         """Advance n collide-stream steps; k > 1 uses the temporally blocked kernel (k steps per launch).
         ``cfg`` = (strip, chunk) selects its launch configuration (default: the library's choice).  Runs of
         several launches are issued by one C call (lbm_step_aa_n / lbm_step_fused_n); with ``macros`` only
-        the last launch writes them."""
+        the last launch writes them.  With a base window, the run is cut at every ``rebase_every`` steps for a
+        re-derivation of the window's base."""
         n = int(n)
         k = int(k)
+        if self.window is not None and self.rebase_every > 0:
+            if self.rebase_every % max(k, 1) or self._since_rebase % max(k, 1):
+                raise ValueError(f"rebase_every={self.rebase_every} (at {self._since_rebase} since the last) is "
+                                 f"not a multiple of k={k}")
+            while n > 0:
+                m = min(n, self.rebase_every - self._since_rebase)
+                self._step(m, k, macros and m == n, cfg)
+                self._since_rebase += m
+                n -= m
+                if self._since_rebase >= self.rebase_every:
+                    self.rebase()
+            return self
+        return self._step(n, k, macros, cfg)
+
+    def _step(self, n, k, macros, cfg):
         if k <= 1:
             if n > 1:
                 self._aa_n(n - 1 if macros else n)
@@ -303,6 +365,9 @@ class Simulation:
         """Write f, parity, step count and parameters to an .npz file."""
         p = self.params
         extra = {} if self.row_base_host is None else {"row_base": self.row_base_host}
+        if self.window is not None:
+            extra.update(window=np.array(self.window, np.int64), window_vel=self.window_vel_host(),
+                         rebase=np.array([self.rebase_every, self._since_rebase], np.int64))
         np.savez(path, f=self.populations().cpu().numpy(), parity=self.parity, steps=self.steps, mask=self.mask_host,
                  inlet_u=self.inlet_u_host, raw=self.f.cpu().numpy(),
                  params=np.array([p.W, p.H, p.model, p.bc, p.clamp, self.fmt], np.int64),
@@ -316,7 +381,12 @@ class Simulation:
         sim = cls(W, H, tau_inv, cs=cs, model=model, bc=bc, mask=z["mask"], inlet_u=z["inlet_u"], f0=z["f"],
                   clamp=bool(clamp), s_e=s_e, s_eps=s_eps, s_q=s_q, parity=int(z["parity"]), device=device,
                   storage=fmt, row_base=z["row_base"] if "row_base" in z.files else None,
-                  arith="fast" if ("fast" in z.files and int(z["fast"])) else "exact")
+                  arith="fast" if ("fast" in z.files and int(z["fast"])) else "exact",
+                  window=z["window"] if "window" in z.files else None,
+                  rebase_every=int(z["rebase"][0]) if "rebase" in z.files else 200)
         sim.f.copy_(torch.from_numpy(z["raw"]).to(sim.device))  # exact stored bits
+        if sim.window is not None:
+            sim._wvel[0].copy_(torch.from_numpy(z["window_vel"]).to(sim.device))
+            sim._since_rebase = int(z["rebase"][1])
         sim.steps = int(z["steps"])
         return sim

@@ -98,7 +98,8 @@ def lib():
         L.oracle_mrt_forward_f64.argtypes = [vp, vp]
         L.oracle_mrt_inverse_f64.argtypes = [vp, vp]
         L.oracle_diagnostics_f32.argtypes = [P, vp, vp, vp, ctypes.c_int, vp]
-        L.oracle_aa_step_h16.argtypes = [P, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp]
+        L.oracle_aa_step_h16.argtypes = [P, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.oracle_rebase_window.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.oracle_convert_f32_h16.argtypes = [vp, vp, ctypes.c_long, ctypes.c_int]
         L.oracle_convert_h16_f32.argtypes = [vp, vp, ctypes.c_long, ctypes.c_int]
         _lib = L
@@ -232,37 +233,90 @@ def from_h16(bits, fmt):
     return y
 
 
-def _slot_base(base, parity, shape):
+def _slot_base(base, parity, shape, window=None):
     """FP32 base value of every slot [9, H, W] of a state at AA parity `parity` (slot p holds direction p, or opp(p)
-    when reversed), from a base table [H, 16]; zeros without a table."""
+    when reversed), from a base table [H, 16] and, inside a window (x0, y0, w, h, vel[h, w, 2]), the base values of
+    the window's per-cell velocities; zeros without a table."""
     if base is None:
         return np.zeros(shape, np.float32)
     b = np.asarray(base, np.float32)
     dirs = OPP if parity else np.arange(9)
-    return np.ascontiguousarray(np.broadcast_to(b[:, dirs].T[:, :, None], shape))
+    out = np.ascontiguousarray(np.broadcast_to(b[:, dirs].T[:, :, None], shape))
+    if window is not None:
+        x0, y0, w, h, vel = window
+        out[:, y0:y0 + h, x0:x0 + w] = np.moveaxis(vel_base(vel)[:, :, dirs], -1, 0)
+    return out
 
 
-def deviations(f64, fmt, base=None, parity=0):
+def vel_base(vel):
+    """Base values [..., 9] (float32) of window velocities vel[..., 2] (float16): w_q ((3 cu + 4.5 cu^2) - usq15),
+    usq15 = 1.5 (ux^2 + uy^2), cu = c_q.u (one addition for the diagonals), in FP32 operations in the C oracle's
+    order (vel_base_f32), so the values are the same bits."""
+    v = np.asarray(vel, np.float16).astype(np.float32)
+    ux, uy = v[..., 0], v[..., 1]
+    f = np.float32
+    usq15 = f(1.5) * (ux * ux + uy * uy)
+    out = np.empty(v.shape[:-1] + (9,), np.float32)
+    for q in range(9):
+        if q == 0:
+            cu = np.zeros_like(ux)
+        elif CX[q] == 0:
+            cu = uy if CY[q] > 0 else -uy
+        elif CY[q] == 0:
+            cu = ux if CX[q] > 0 else -ux
+        else:
+            cu = (ux if CX[q] > 0 else -ux) + (uy if CY[q] > 0 else -uy)
+        out[..., q] = W_F32[q] * ((f(3.0) * cu + f(4.5) * (cu * cu)) - usq15)
+    return out
+
+
+def deviations(f64, fmt, base=None, parity=0, window=None):
     """16-bit state from an FP64 population array: h = f - w rounded to FP32 (w: the FP32 weight literals), minus
-    the slot's base value (FP32) when a per-row base table [H, 16] is given, rounded to 16 bits."""
+    the slot's base value (FP32) when a per-row base table [H, 16] (and a window) is given, rounded to 16 bits."""
     f64 = np.asarray(f64, np.float64)
     h = (f64 - W_F32.astype(np.float64)[:, None, None]).astype(np.float32)
     if base is not None:
-        h = (h - _slot_base(base, parity, h.shape)).astype(np.float32)
+        h = (h - _slot_base(base, parity, h.shape, window)).astype(np.float32)
     return to_h16(h, fmt)
 
 
-def populations(h16, fmt, base=None, parity=0):
+def populations(h16, fmt, base=None, parity=0, window=None):
     """FP64 populations w + h of a 16-bit state (h = stored + the slot's base value, in FP32, with a base table)."""
     h = from_h16(h16, fmt)
     if base is not None:
-        h = (h + _slot_base(base, parity, h.shape)).astype(np.float32)
+        h = (h + _slot_base(base, parity, h.shape, window)).astype(np.float32)
     return h.astype(np.float64) + W_F32.astype(np.float64)[:, None, None]
 
 
-def aa_step_h16(p, mask, h16, fmt, inlet_u32, parity, macros=False, base=None):
+class OracleWindow(ctypes.Structure):
+    """Mirror of the C oracle's oracle_window: a window of per-cell base velocities (IEEE halves)."""
+
+    _fields_ = [("x0", ctypes.c_int), ("y0", ctypes.c_int), ("w", ctypes.c_int), ("h", ctypes.c_int),
+                ("vel", ctypes.c_void_p)]
+
+
+def _window(window):
+    """(x0, y0, w, h, vel[h, w, 2] float16) -> (OracleWindow, the velocity array kept alive), or (None, None)."""
+    if window is None:
+        return None, None
+    x0, y0, w, h, vel = window
+    vel = np.ascontiguousarray(vel, np.float16)
+    assert vel.shape == (h, w, 2)
+    return OracleWindow(int(x0), int(y0), int(w), int(h), vel.ctypes.data), vel
+
+
+def rebase_window(W, H, h16, fmt, parity, window):
+    """The kernels' window re-derivation on a 16-bit state in place; returns the new window velocities [h, w, 2]."""
+    ow, keep = _window(window)
+    nv = np.zeros(keep.shape, np.float16)
+    lib().oracle_rebase_window(int(W), int(H), _ptr(h16), int(fmt), int(parity), ctypes.byref(ow), _ptr(nv))
+    return nv
+
+
+def aa_step_h16(p, mask, h16, fmt, inlet_u32, parity, macros=False, base=None, window=None):
     """One AA step in place on a 16-bit deviation state (FP32 collision, RNE storage); base: None or a per-row
-    base table [H, 16] (float32) applied to every load and store."""
+    base table [H, 16] (float32) applied to every load and store; window: None or (x0, y0, w, h, vel[h, w, 2]),
+    per-cell base velocities (float16) whose base values take precedence inside the window."""
     assert h16.dtype == np.uint16 and h16.shape == (9, p.H, p.W) and h16.flags.c_contiguous
     prof = np.ascontiguousarray(inlet_u32, dtype=np.float32)
     mac = [np.zeros((p.H, p.W), np.float32) for _ in range(3)] if macros else [None] * 3
@@ -270,8 +324,10 @@ def aa_step_h16(p, mask, h16, fmt, inlet_u32, parity, macros=False, base=None):
     if base is not None:
         b = np.ascontiguousarray(base, np.float32)
         assert b.shape == (p.H, 16)
+    ow, keep = _window(window)
     lib().oracle_aa_step_h16(ctypes.byref(p), _ptr(mask), _ptr(h16), int(fmt), _ptr(prof), int(parity),
-                             _ptr(mac[1]), _ptr(mac[2]), _ptr(mac[0]), None if b is None else _ptr(b))
+                             _ptr(mac[1]), _ptr(mac[2]), _ptr(mac[0]), None if b is None else _ptr(b),
+                             None if ow is None else ctypes.byref(ow))
     return tuple(mac) if macros else None
 
 

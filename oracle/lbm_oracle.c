@@ -245,16 +245,52 @@ static inline uint16_t h16_store(int fmt, int q, float h)
    the clamped deviation minus d = base[16 y + q] (one FP32 subtraction, then the 16-bit rounding) and loads the
    stored value plus d (one FP32 addition).  base == NULL: d = 0, the plain formats. */
 #define ROWB 16
-static inline float h16_load_b(int fmt, uint16_t b, const float *base, int q, int y)
+/* The base of a slot: a window of per-cell values (cells [x0, x0 + w) x [y0, y0 + h), ROWB floats per cell,
+   row-major; base == NULL: none) takes precedence over the per-row table. */
+typedef struct oracle_window {
+    int x0, y0, w, h;
+    const uint16_t *vel; /* (ux, uy) as IEEE halves per window cell, row-major */
+} oracle_window;
+static const oracle_window *g_win = 0; /* set for the duration of one oracle_aa_step_h16 call */
+
+/* A window cell's base value of direction q: w_q ((3 cu + 4.5 cu^2) - 1.5 (ux^2 + uy^2)) in FP32 from its half
+ * velocity, cu = c_q.u as one addition for the diagonals (the library's vel_base, lbm_device.h). */
+static float vel_base_f32(int q, float ux, float uy)
 {
-    return base ? h16_load(fmt, b) + base[ROWB * y + q] : h16_load(fmt, b);
+    const float usq15 = 1.5f * (ux * ux + uy * uy);
+    float cu;
+    if (q == 0)
+        cu = 0.0f;
+    else if (CX[q] == 0)
+        cu = CY[q] > 0 ? uy : -uy;
+    else if (CY[q] == 0)
+        cu = CX[q] > 0 ? ux : -ux;
+    else
+        cu = (CX[q] > 0 ? ux : -ux) + (CY[q] > 0 ? uy : -uy);
+    return wq_f32(q) * ((3.0f * cu + 4.5f * (cu * cu)) - usq15);
 }
 
-static inline uint16_t h16_store_b(int fmt, int q, float h, const float *base, int y)
+/* base value of direction q for a slot of cell (x, y): the window cell's own, else its row's */
+static inline float base_of(const float *rows, int q, int x, int y)
+{
+    const oracle_window *W = g_win;
+    if (W && W->vel && x >= W->x0 && x < W->x0 + W->w && y >= W->y0 && y < W->y0 + W->h) {
+        const uint16_t *v = W->vel + 2 * ((long)(y - W->y0) * W->w + (x - W->x0));
+        return vel_base_f32(q, half_to_float(v[0]), half_to_float(v[1]));
+    }
+    return rows[ROWB * y + q];
+}
+
+static inline float h16_load_b(int fmt, uint16_t b, const float *base, int q, int x, int y)
+{
+    return base ? h16_load(fmt, b) + base_of(base, q, x, y) : h16_load(fmt, b);
+}
+
+static inline uint16_t h16_store_b(int fmt, int q, float h, const float *base, int x, int y)
 {
     if (!base) return h16_store(fmt, q, h);
     float w = wq_f32(q);
-    float c = clampr_f32(h, -w, 10.0f - w) - base[ROWB * y + q];
+    float c = clampr_f32(h, -w, 10.0f - w) - base_of(base, q, x, y);
     return fmt == 2 ? float_to_bf16(c) : float_to_half(c);
 }
 
@@ -271,8 +307,9 @@ void oracle_convert_h16_f32(const uint16_t *x, float *y, long n, int fmt)
 /* One AA step in place on 16-bit deviations; same streaming as oracle_aa_step_f32.  base: NULL, or the per-row
    base table (16 floats per row) that the row-based formats apply to every load and store. */
 void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *f, int fmt, const float *inlet_u,
-                        int parity, float *oux, float *ouy, float *orho, const float *base)
+                        int parity, float *oux, float *ouy, float *orho, const float *base, const oracle_window *win)
 {
+    g_win = win;
     kpar_t_f32 P;
     make_kpar_f32(pp, &P);
     const long plane = (long)P.W * P.H;
@@ -286,7 +323,7 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
             const long idx = (long)y * P.W + x;
             float g[9], mac[3];
             if (parity == 0) {
-                for (int q = 0; q < 9; ++q) g[q] = h16_load_b(fmt, f[q * plane + idx], base, q, y);
+                for (int q = 0; q < 9; ++q) g[q] = h16_load_b(fmt, f[q * plane + idx], base, q, x, y);
             } else {
                 for (int q = 0; q < 9; ++q) {
                     if (inlet && CX[q] == 1) {
@@ -296,8 +333,8 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
                     long s = nbr_f32(&P, x, y, -CX[q], -CY[q]);
                     /* bounce-back: this cell's slot q holds direction opp(q); else the source's slot opp(q), on the
                        source's row, holds direction q */
-                    g[q] = mask[s] ? h16_load_b(fmt, f[q * plane + idx], base, OPP[q], y)
-                                   : h16_load_b(fmt, f[OPP[q] * plane + s], base, q, (int)(s / P.W));
+                    g[q] = mask[s] ? h16_load_b(fmt, f[q * plane + idx], base, OPP[q], x, y)
+                                   : h16_load_b(fmt, f[OPP[q] * plane + s], base, q, (int)(s % P.W), (int)(s / P.W));
                 }
             }
             if (P.model == ORACLE_BGK)
@@ -305,9 +342,9 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
             else
                 collide_mrt_f32(&P, g, inlet, inlet ? inlet_u[y] : 0.0f, mac, -1.0f);
             if (parity == 0) {
-                for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx] = h16_store_b(fmt, q, g[q], base, y);
+                for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx] = h16_store_b(fmt, q, g[q], base, x, y);
                 if (channel && x == P.W - 2)
-                    for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx + 1] = h16_store_b(fmt, q, g[q], base, y);
+                    for (int q = 0; q < 9; ++q) f[OPP[q] * plane + idx + 1] = h16_store_b(fmt, q, g[q], base, x + 1, y);
             } else {
                 for (int q = 0; q < 9; ++q) {
                     if (inlet && CX[q] == -1) continue;
@@ -315,15 +352,15 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
                     /* bounce-back: this cell's slot opp(q) then holds direction opp(q); else the target's slot q, on
                        the target's row (the clamp interval is the same for q and opp(q)) */
                     if (mask[d])
-                        f[OPP[q] * plane + idx] = h16_store_b(fmt, OPP[q], g[q], base, y);
+                        f[OPP[q] * plane + idx] = h16_store_b(fmt, OPP[q], g[q], base, x, y);
                     else
-                        f[q * plane + d] = h16_store_b(fmt, q, g[q], base, (int)(d / P.W));
+                        f[q * plane + d] = h16_store_b(fmt, q, g[q], base, (int)(d % P.W), (int)(d / P.W));
                 }
                 if (channel && x == P.W - 2) {
                     for (int q = 0; q < 9; ++q) {
                         if (CX[q] != -1) continue;
                         long d = (long)(y + CY[q]) * P.W + x;
-                        if (!mask[d]) f[q * plane + d] = h16_store_b(fmt, q, g[q], base, y + CY[q]);
+                        if (!mask[d]) f[q * plane + d] = h16_store_b(fmt, q, g[q], base, x, y + CY[q]);
                     }
                 }
             }
@@ -334,6 +371,44 @@ void oracle_aa_step_h16(const oracle_params *pp, const uint8_t *mask, uint16_t *
             }
         }
     }
+}
+
+/* The kernels' base-window re-derivation (lbm_rebase_window), the same FP32 operations in the same order: per
+   window cell decode with the current base, velocity u = j / rho (0 where rho is outside (0.5, 2) or u is not
+   finite), new base d_q = w_q ((3 c.u + 4.5 (c.u)^2) - 1.5 u.u), re-encode as the conversion does:
+   round16((f - w_q) - d). */
+/* lbm_rebase_window's re-derivation on a 16-bit state in place: per window cell, decode with the current window
+ * velocity, u = j / rho in FP32 (0 where rho is outside (0.5, 2) or |u_x| or |u_y| is not below 1), rounded to
+ * half into new_vel, and the words re-encoded against the base values of the rounded velocity. */
+void oracle_rebase_window(int W, int H, uint16_t *f, int fmt, int parity, const oracle_window *win, uint16_t *new_vel)
+{
+    const long plane = (long)W * H;
+    (void)H;
+    for (int j = 0; j < win->h; ++j)
+        for (int i = 0; i < win->w; ++i) {
+            const int x = win->x0 + i, y = win->y0 + j;
+            const long idx = (long)y * W + x, cell = (long)j * win->w + i;
+            const float ox = half_to_float(win->vel[2 * cell]), oy = half_to_float(win->vel[2 * cell + 1]);
+            float fq[9];
+            for (int q = 0; q < 9; ++q) {
+                const int slot = parity ? OPP[q] : q;
+                fq[q] = wq_f32(q) + (h16_load(fmt, f[slot * plane + idx]) + vel_base_f32(q, ox, oy));
+            }
+            const float rho = (((fq[0] + fq[1]) + (fq[2] + fq[3])) + ((fq[4] + fq[5]) + (fq[6] + fq[7]))) + fq[8];
+            const float jx = ((fq[1] - fq[3]) + (fq[5] - fq[6])) + (fq[8] - fq[7]);
+            const float jy = ((fq[2] - fq[4]) + (fq[5] + fq[6])) - (fq[7] + fq[8]);
+            const float ux = jx / rho, uy = jy / rho;
+            const int ok = rho > 0.5f && rho < 2.0f && fabsf(ux) < 1.0f && fabsf(uy) < 1.0f;
+            const uint16_t hx = ok ? float_to_half(ux) : 0, hy = ok ? float_to_half(uy) : 0;
+            new_vel[2 * cell] = hx;
+            new_vel[2 * cell + 1] = hy;
+            const float nx = half_to_float(hx), ny = half_to_float(hy);
+            for (int q = 0; q < 9; ++q) {
+                const float v = (fq[q] - wq_f32(q)) - vel_base_f32(q, nx, ny);
+                const int slot = parity ? OPP[q] : q;
+                f[slot * plane + idx] = fmt == 2 ? float_to_bf16(v) : float_to_half(v);
+            }
+        }
 }
 
 int oracle_abi_version(void) { return 1; }

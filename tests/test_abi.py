@@ -35,8 +35,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_params_struct_layout():
     # int32 W, H, model, bc, clamp; float tau_inv, cs, s_e, s_eps, s_q; int32 sums, fast; const float *row_base
-    assert ctypes.sizeof(_lib.LbmParams) == 56
+    assert ctypes.sizeof(_lib.LbmParams) == 64
     assert _lib.LbmParams.fast.offset == 44 and _lib.LbmParams.row_base.offset == 48
+    assert _lib.LbmParams.window.offset == 56 and ctypes.sizeof(_lib.LbmBaseWindow) == 24
     assert _lib.LbmParams.tau_inv.offset == 20
     assert _lib.LbmParams.sums.offset == 40
     import oracle as O

@@ -201,3 +201,117 @@ def test_rowbase_accuracy_gain_scaled_cylinder(gpu, oracle_lib):
         errs[rb] = np.sqrt((((a - A) ** 2 + (b - B) ** 2)[upd]).sum() / ((A ** 2 + B ** 2)[upd]).sum())
     assert errs["inlet"] < 2.5e-4, errs
     assert errs["inlet"] < errs[None] / 5, errs
+
+
+# ------------------------------------------------------------------ per-cell base window
+def _wsim(storage, case="channel", W=300, H=72, window=(60, 20, 90, 30), rebase_every=0, arith="exact", seed=6):
+    """_sim's lattices with a base window (lbm_params.window); channel: from the cylinder's trailing side over
+    the block; the window crosses tile boundaries of every chunk height."""
+    from lbm.sim import Simulation
+    if case == "periodic":
+        mask = G.periodic_mask(W, H)
+        mask[20:25, 60:66] = 1
+        ux, uy = G.random_phase_velocity(W, H, 4, 0.02, seed=seed)
+        bx, by = _shear(W, H)
+        f0 = G.feq(np.ones((H, W)), ux + bx[:, None], uy + by[:, None])
+        return Simulation(W, H, 1.7, cs=0.12, bc="periodic", mask=mask, f0=f0, storage=storage, row_base=(bx, by),
+                          window=window, rebase_every=rebase_every, arith=arith)
+    mask = G.channel_mask(W, H, cylinder=(W // 4, H // 2, 6))
+    mask[20:26, 200 % W:206 % W] = 1
+    prof = G.parabolic_profile(H, 0.05)
+    return Simulation(W, H, 1.75, cs=0.12, mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, seed),
+                      storage=storage, row_base="inlet", window=window, rebase_every=rebase_every, arith=arith)
+
+
+WINDOWS = [("channel", 300, 72, (60, 20, 90, 30)), ("channel", 131, 37, (1, 1, 128, 35)),
+           ("periodic", 300, 72, (37, 13, 200, 41)), ("periodic", 97, 41, (5, 30, 64, 10))]
+
+
+@pytest.mark.parametrize("storage,fmt", FMTS)
+@pytest.mark.parametrize("case,W,H,window", WINDOWS)
+def test_window_single_step_and_rebase_bitwise(gpu, storage, fmt, case, W, H, window):
+    """Single steps with a base window, and lbm_rebase_window at both parities, bit for bit against the oracle's
+    16-bit mirror with the same row table and window (state words and the new per-cell base)."""
+    sim = _wsim(storage, case, W, H, window)
+    op = _op(sim)
+    prev, par = sim.state()
+    wb = sim.window_vel_host()
+    for n in range(10):
+        if n in (3, 6):
+            sim.rebase()
+            got, _ = sim.state()
+            nb = O.rebase_window(W, H, prev, fmt, par, window + (wb,))
+            wb = sim.window_vel_host()
+            assert np.array_equal(nb, wb), f"rebase at parity {par}: base differs"
+            assert np.array_equal(got, prev), f"rebase at parity {par}: {np.count_nonzero(got != prev)} words differ"
+        sim.step(1)
+        got, gpar = sim.state()
+        ref = prev.copy()
+        O.aa_step_h16(op, sim.mask_host, ref, fmt, sim.inlet_u_host, par, base=sim.row_base_host, window=window + (wb,))
+        sel = O.observable(op, sim.mask_host, par ^ 1, ghost=True)
+        assert np.array_equal(got[sel], ref[sel]), f"step {n}: {np.count_nonzero(got[sel] != ref[sel])} differ"
+        prev, par = got, gpar
+
+
+@pytest.mark.parametrize("storage,fmt", FMTS)
+@pytest.mark.parametrize("case,W,H,window", WINDOWS + [("channel", 301, 90, (2, 40, 297, 49))])
+@pytest.mark.parametrize("arith", ["exact", "fast"])
+def test_window_pair_equals_single(gpu, storage, fmt, case, W, H, window, arith):
+    """The two-step kernel with a base window (tiles that meet it take the per-cell form) bit-identical to single
+    steps, every tile height, with the window re-derived between launches."""
+    for chunk in (8, 12, 16, 24):
+        a = _wsim(storage, case, W, H, window, rebase_every=4, arith=arith)
+        b = _wsim(storage, case, W, H, window, rebase_every=4, arith=arith)
+        sel = torch.from_numpy(O.observable(_op(a), a.mask_host, 0, ghost=True)).to(a.device)
+        for launch in range(3):
+            a.step(4)
+            b.step(4, k=2, cfg=(0, chunk))
+            assert int(((a.f != b.f) & sel).sum()) == 0, f"chunk {chunk} launch {launch}"
+            assert torch.equal(a._wvel[0], b._wvel[0])
+
+
+def test_window_conversion_macros_and_checkpoint(gpu, tmp_path):
+    from lbm.sim import Simulation
+    sim = _wsim("f16", rebase_every=4)
+    op = _op(sim)
+    sim.step(7)  # one rebase behind it, odd parity
+    h, par = sim.state()
+    win = sim.window + (sim.window_vel_host(),)
+    f = O.populations(h, O.F16, base=sim.row_base_host, parity=par, window=win)
+    assert np.allclose(sim.populations().cpu().numpy(), f, rtol=0, atol=1e-7)
+    rho, ux, uy = (t.cpu().numpy() for t in sim.macro())
+    r, a, b = O.macro(op, sim.mask_host, sim.populations().cpu().numpy(), sim.inlet_u_host, par)
+    upd = O.observable(op, sim.mask_host, 1)[0]
+    assert np.array_equal(a[upd], ux[upd]) and np.array_equal(b[upd], uy[upd]) and np.array_equal(r[upd], rho[upd])
+    sim.save(tmp_path / "w.npz")
+    back = Simulation.load(tmp_path / "w.npz")
+    assert back.window == sim.window and back.rebase_every == 4 and back._since_rebase == sim._since_rebase
+    assert torch.equal(back.f, sim.f) and torch.equal(back._wvel[0], sim._wvel[0])
+    sim.step(1)
+    back.step(1)
+    sim.step(8, k=2, cfg=(0, 12))
+    back.step(8, k=2, cfg=(0, 12))
+    assert torch.equal(back.f, sim.f) and torch.equal(back._wvel[0], sim._wvel[0])
+
+
+def test_window_accuracy_scaled_cylinder(gpu):
+    """The cylinder flow at 1024 x 256 (the 4096 x 1024 geometry scaled), 1000 steps: FP16 storage relative to the
+    inlet profile with a base window over the near wake (re-derived every 200 steps) against FP64 is no worse than
+    the row base alone -- printed for both."""
+    from lbm.sim import Simulation
+    case = configs.cylinder_scaled(1024, 256, steps=1000)
+    steps = 1000
+    p = _op(Simulation(case.W, case.H, case.tau_inv, **case.sim_kwargs()))
+    f64 = case.f0.astype(np.float64).copy()
+    par = O.aa_steps(p, case.mask, f64, np.asarray(case.inlet_u, np.float64), 0, steps)
+    R, A, B = O.macro(p, case.mask, f64, np.asarray(case.inlet_u, np.float64), par)
+    upd = O.observable(p, case.mask, 1)[0]
+    errs = {}
+    for win in (None, configs.wake_window(case)):
+        sim = Simulation(case.W, case.H, case.tau_inv, storage="f16", row_base="inlet", window=win,
+                         **case.sim_kwargs())
+        sim.step(steps, k=2)
+        r, a, b = (t.cpu().numpy().astype(np.float64) for t in sim.macro())
+        errs[win] = np.sqrt((((a - A) ** 2 + (b - B) ** 2)[upd]).sum() / ((A ** 2 + B ** 2)[upd]).sum())
+    print(f"\nL2(u) vs FP64 at {steps} steps: {errs}")
+    assert errs[configs.wake_window(case)] <= errs[None] * 1.05, errs
