@@ -1,7 +1,7 @@
 """D2Q9 throughput on one MI355X: GLUPS at 4096 x 1024 (plain FP32 vs chosen form), % HBM roofline, L2 error.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--k 8] [--cpu-seconds 10] [--l2 full|short|off]
-                    [--fused-storages f32,f32+fast,f16,f16r,f16r+fast,bf16] [--ensemble-steps 200] [--only ...]
+                    [--fused-storages f32,f32+fast,f16,f16r,f16r+fast,f16rw+fast,bf16] [--ensemble-steps 200] [--only ...]
 
 Workload: channel flow past a cylinder, 4096 x 1024, R = 64 at (1024, 512), Re = 1000, Cs = 0.15,
 parabolic inlet u_max = 0.05 (synthetic initial state: feq at rho = 1, u = inlet profile, seeded noise).
@@ -43,11 +43,13 @@ TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-
 # instructions per cell and launch (rocprofv3 --pmc SQ_INSTS_VALU x 64 / cells); the summary with the command
 # lines is profiles/k_pair_cylinder_4096x1024.md.  Configurations not listed there report null.
 # Keys are (storage spec, tile rows): the spec of parse_spec (an r suffix: row-based 16-bit storage, +fast: the
-# hardware reciprocal and square root); the row-based and fast rows are in profiles/k_pair_rowbase_fast_4096x1024.md.
+# hardware reciprocal and square root); the row-based and fast rows are in profiles/k_pair_rowbase_fast_4096x1024.md,
+# the base-window row (rw) in profiles/k_pair_window_fast_4096x1024.md.
 PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.93e6, ("bf16", 12): 156.97e6, ("f32", 12): 311.19e6,
-                           ("f16+fast", 12): 156.9e6, ("f16r", 12): 157.2e6, ("f16r+fast", 12): 157.1e6}
+                           ("f16+fast", 12): 156.9e6, ("f16r", 12): 157.2e6, ("f16r+fast", 12): 157.1e6,
+                           ("f16rw+fast", 12): 160.75e6}
 PAIR_VALU_PER_CELL = {("f16", 12): 363.8, ("bf16", 12): 363.4, ("f32", 12): 376.9, ("f16+fast", 12): 311.0,
-                      ("f16r", 12): 397.9, ("f16r+fast", 12): 345.0}
+                      ("f16r", 12): 397.9, ("f16r+fast", 12): 345.0, ("f16rw+fast", 12): 351.5}
 # The same for the four-step tile kernel (FP32, per tile shape); profiles/k_quad_cylinder_4096x1024.md (fast
 # arithmetic: profiles/k_quad_fast_cylinder_4096x1024.md).
 QUAD_TRAFFIC_PER_LAUNCH = {("f32", 12): 327.2e6, ("f32", 14): 336.1e6, ("f32", 24): 345.5e6,
@@ -694,10 +696,11 @@ def main():
     ap.add_argument("--k", type=int, default=8, help="largest steps per launch the tuner tries")
     ap.add_argument("--fused-cfg", default="", metavar="K,STRIP,CHUNK",
                     help="time the fused form at this configuration instead of autotuning it")
-    ap.add_argument("--fused-storages", default="f32,f32+fast,f16,f16r,f16r+fast,bf16",
+    ap.add_argument("--fused-storages", default="f32,f32+fast,f16,f16r,f16r+fast,f16rw+fast,bf16",
                     help="storage specs whose fused form is tuned and timed (comma-separated): f32, f16, bf16, an r "
-                         "suffix for 16-bit deviations from the inlet profile's per-row equilibrium, +fast for the "
-                         "hardware reciprocal and square root")
+                         "suffix for 16-bit deviations from the inlet profile's per-row equilibrium, rw for that plus "
+                         "a per-cell base window over the near wake, +fast for the hardware reciprocal and square "
+                         "root")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--l2", choices=("full", "short", "off"), default="full",
                     help="L2 errors against FP64: 'full' = the cylinder at 1000/2000/5000/10000 steps and the "
