@@ -2,7 +2,10 @@
 the oracle's 16-bit mirror with the same table, the two-step kernel bit for bit against single steps (every tile
 height, channel, periodic, narrow and compensated-sum cases, and the full 10 000-step 4096 x 1024 cylinder run),
 conversions and macro fields, the errors the library reports for forms that do not take a base, and the accuracy
-gain over the rest-state base on the scaled cylinder flow against the FP64 oracle."""
+gain over the rest-state base on the scaled cylinder flow against the FP64 oracle.  The base window
+(lbm_params.window: per-cell base velocities, re-derived by lbm_rebase_window) is held to the same bars: single
+steps and re-derivations bit for bit against the mirror, the two-step kernel bit for bit against single steps
+(also over the 4096 x 1024 cylinder's 10 000 steps with the wake window)."""
 import ctypes
 
 import numpy as np
@@ -315,3 +318,23 @@ def test_window_accuracy_scaled_cylinder(gpu):
         errs[win] = np.sqrt((((a - A) ** 2 + (b - B) ** 2)[upd]).sum() / ((A ** 2 + B ** 2)[upd]).sum())
     print(f"\nL2(u) vs FP64 at {steps} steps: {errs}")
     assert errs[configs.wake_window(case)] <= errs[None] * 1.05, errs
+
+
+def test_window_pair_cylinder_large(gpu):
+    """The 4096 x 1024 cylinder's 10 000 steps with FP16 storage relative to the inlet profile, the wake's base window
+    re-derived every 200 steps and the fast arithmetic (the bench's headline form): the 12-row two-step kernel
+    bit-identical to single steps, window velocities included."""
+    from lbm.sim import Simulation
+    case = configs.cylinder_large()
+    kw = dict(storage="f16", row_base="inlet", arith="fast", window=configs.wake_window(case), rebase_every=200,
+              **case.sim_kwargs())
+    a = Simulation(case.W, case.H, case.tau_inv, **kw)
+    b = Simulation(case.W, case.H, case.tau_inv, **kw)
+    sel = torch.from_numpy(O.observable(_op(a), a.mask_host, 0, ghost=True)).to(a.device)
+    for chunk in range(5):
+        a.step(2000)
+        b.step(2000, k=2, cfg=(0, 12))
+        n = int(((a.f != b.f) & sel).sum())
+        assert n == 0, f"after {2000 * (chunk + 1)} steps: {n} differ"
+        assert torch.equal(a._wvel[0], b._wvel[0])
+    assert np.isfinite(b.diagnostics()).all()
