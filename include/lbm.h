@@ -357,8 +357,9 @@ int lbm_ensemble_destroy(lbm_ensemble *e);
 /*
  * Stream-capture graphs: the C-ABI form of the reference driver's CuPy capture of two fused launches
  * (lbm_smemaa_main.py:136-151, replayed at :173-174).  lbm_graph_begin starts capturing `stream` (not the default
- * stream); every liblbm call issued on it until lbm_graph_end is recorded, not run.  lbm_graph_launch
- * replays the recorded sequence on any stream.  No liblbm entry point records a memset, so the graph
+ * stream); every liblbm call issued on it until lbm_graph_end is recorded, not run; lbm_graph_end instantiates
+ * the graph and uploads it to the device (so that its first launch carries no setup) before it returns.
+ * lbm_graph_launch replays the recorded sequence on any stream.  No liblbm entry point records a memset, so the graph
  * may be replayed any number of times.
  */
 typedef struct lbm_graph lbm_graph;

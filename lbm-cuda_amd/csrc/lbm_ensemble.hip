@@ -415,7 +415,11 @@ int lbm_graph_end(lbm_stream_t stream, lbm_graph **out)
     auto *g = new lbm_graph();
     hipError_t he = hipStreamEndCapture(to_hip(stream), &g->graph);
     if (he == hipSuccess) he = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    // the instantiated graph's device-side setup now, not in its first launch
+    if (he == hipSuccess) he = hipGraphUpload(g->exec, to_hip(stream));
+    if (he == hipSuccess) he = hipStreamSynchronize(to_hip(stream));
     if (he != hipSuccess) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
         delete g;
         return fail((int)he, "lbm_graph_end: %s", hipGetErrorString(he));

@@ -287,7 +287,8 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
             for (int q = 1; q < 9; ++q)
                 solid[q] = !FLUID && smb[(1 + cy(q)) * LS + 1 + cx(q)] && !(OPEN && inlet && cx(q) < 0);
             // row-based formats: base value of direction q on region row r + dy (the staged rows; read where used,
-            // with one address for the whole wave)
+            // with one address for the whole wave; scalar loads of the three rows' values from the table instead
+            // ran the kernel out of SGPRs: 35 spilled to VGPR lanes)
             auto drow0 = [&](int dy, int q) -> float { return bgroup(r + dy, bgrp(q))[bidx(q)]; };
             // base value of direction q for a slot of cell (gx + dx, gy + dy) (WIN: that cell's own in the window)
             auto bv = [&](int q, int dx, int dy) -> float {
