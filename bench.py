@@ -848,7 +848,10 @@ def main():
         in the reference driver's replayed capture (lbm_smemaa_main.py:136-151,173-174).  The recording runs
         nothing; it comes first, then `pre_steps` untimed steps (rounded up to a whole even number of launches
         of `unit` steps, so that the state is back in the buffer and at the parity the graph starts from) keep
-        the GPU busy up to the timed region.  Returns (us per step by events, us per step by the host clock)."""
+        the GPU busy up to the timed region.  When the recorded steps themselves end where they start (buffer,
+        parity, base window), one untimed replay of the graph precedes the timed one, so that the timed replay is
+        not the graph's first (measured: a first replay of a 20-step graph runs 0.5-3 % slower than the next).
+        Returns (us per step by events, us per step by the host clock)."""
         period = 2 * unit
         if getattr(lat, "window", None) is not None and lat.rebase_every:
             # whole pairs of re-derivations too: the window's base is back in the buffer the graph reads first
@@ -864,6 +867,11 @@ def main():
         fn(pre)
         if lat.f is not saved[0] or lat.parity != saved[2] or _window_state(lat) != saved[4]:
             raise RuntimeError("untimed steps did not return the state to the recorded buffer and parity")
+        # a base window: no re-derivation recorded (same table buffer), and none due within the two replays
+        w_ok = saved[4] is None or (after[4][0] == saved[4][0] and after[4][1] + nsteps <= lat.rebase_every)
+        warm = after[0] is saved[0] and after[2] == saved[2] and w_ok
+        if warm:
+            g.replay()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         e0.record()
@@ -873,8 +881,10 @@ def main():
         wall_us = (time.perf_counter() - t0) * 1e6 / nsteps
         g.close()
         lat.f, lat.f_alt, lat.parity = after[:3]
-        lat.steps = after[3] + pre
+        lat.steps = after[3] + pre + (nsteps if warm else 0)
         _set_window_state(lat, after[4])
+        if warm and saved[4] is not None:
+            lat._since_rebase += nsteps
         return e0.elapsed_time(e1) * 1e3 / nsteps, wall_us
 
     # ---------------- plain FP32 single AA steps: exactly K steps, one graph replay
