@@ -7,9 +7,10 @@ Workload: channel flow past a cylinder, 4096 x 1024, R = 64 at (1024, 512), Re =
 parabolic inlet u_max = 0.05 (synthetic initial state: feq at rho = 1, u = inlet profile, seeded noise).
 A "step" is one collide-stream update of all W*H cells with the state resident in HBM.  The flow is first
 developed by --spinup untimed FP32 steps (default 3000) and every form is timed from that state.  Timed
-region: exactly K steps, recorded beforehand as a graph and replayed once, bracketed by torch.cuda.synchronize();
-HIP events on the replay stream give the per-step kernel time used for the roofline (72 B/cell/step algorithmic
-traffic for FP32 single steps), the host clock around the same replay the wall time.
+region: exactly K steps, recorded beforehand as a graph and replayed once right behind the untimed warm-up steps
+(no idle GPU between them), the two bracketed by torch.cuda.synchronize(); HIP events on the replay stream around
+the K steps give the per-step kernel time used for the value and the roofline (72 B/cell/step algorithmic traffic
+for FP32 single steps), the host clock between the synchronizations, per step run, the wall time.
 The CPU baseline is the oracle's FP32 mirror (C, OpenMP over rows) on a bounded number of steps of the
 same lattice, timed on this host.  The "ensemble" object reports the batched workload of 64 periodic
 lattices of uneven sizes (one batched launch per step, or per two or four steps, vs 64 separate launches per step).
@@ -843,15 +844,18 @@ def main():
 
     def timed_replay(lat, fn, nsteps, pre_steps, unit):
         """Exactly `nsteps` steps of lattice `lat` (advanced by fn(n)) timed as one replay of a graph that
-        recorded fn(nsteps) beforehand, HIP events on the replay stream around the replay, bracketed by
-        synchronizations: the Python and ctypes work of issuing the launches stays out of the timed region, as
-        in the reference driver's replayed capture (lbm_smemaa_main.py:136-151,173-174).  The recording runs
-        nothing; it comes first, then `pre_steps` untimed steps (rounded up to a whole even number of launches
-        of `unit` steps, so that the state is back in the buffer and at the parity the graph starts from) keep
-        the GPU busy up to the timed region.  When the recorded steps themselves end where they start (buffer,
-        parity, base window), one untimed replay of the graph precedes the timed one, so that the timed replay is
-        not the graph's first (measured: a first replay of a 20-step graph runs 0.5-3 % slower than the next).
-        Returns (us per step by events, us per step by the host clock)."""
+        recorded fn(nsteps) beforehand, HIP events on the replay stream around the replay: the Python and ctypes
+        work of issuing the launches stays out of the timed region, as in the reference driver's replayed capture
+        (lbm_smemaa_main.py:136-151,173-174).  The recording runs nothing; it comes first.  Then, between two
+        synchronizations: `pre_steps` untimed steps (rounded up to a whole even number of launches of `unit`
+        steps, so that the state is back in the buffer and at the parity the graph starts from), issued
+        asynchronously and followed by the timed replay without a wait between them, so that the GPU runs the
+        region at the clocks of a busy chip -- after even a short idle (a synchronization) a 20-step region of the
+        compute-bound tile kernels ran 7-10 % slower than the same steps following other work, the memory-bound
+        single step 0.4 %.  When the recorded steps end where they start (buffer, parity, base window), one
+        untimed replay of the graph comes last before the timed one, so that the timed replay is not the graph's
+        first (a first replay ran 0.5-3 % slower than the next).  Returns (us per step by events, us per step by
+        the host clock over everything between the synchronizations)."""
         period = 2 * unit
         if getattr(lat, "window", None) is not None and lat.rebase_every:
             # whole pairs of re-derivations too: the window's base is back in the buffer the graph reads first
@@ -864,21 +868,21 @@ def main():
         after = (lat.f, lat.f_alt, lat.parity, lat.steps, _window_state(lat))
         lat.f, lat.f_alt, lat.parity, lat.steps = saved[:4]
         _set_window_state(lat, saved[4])
-        fn(pre)
-        if lat.f is not saved[0] or lat.parity != saved[2] or _window_state(lat) != saved[4]:
-            raise RuntimeError("untimed steps did not return the state to the recorded buffer and parity")
         # a base window: no re-derivation recorded (same table buffer), and none due within the two replays
         w_ok = saved[4] is None or (after[4][0] == saved[4][0] and after[4][1] + nsteps <= lat.rebase_every)
         warm = after[0] is saved[0] and after[2] == saved[2] and w_ok
-        if warm:
-            g.replay()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        fn(pre)
+        if lat.f is not saved[0] or lat.parity != saved[2] or _window_state(lat) != saved[4]:
+            raise RuntimeError("untimed steps did not return the state to the recorded buffer and parity")
+        if warm:
+            g.replay()
         e0.record()
         g.replay()
         e1.record()
         torch.cuda.synchronize()
-        wall_us = (time.perf_counter() - t0) * 1e6 / nsteps
+        wall_us = (time.perf_counter() - t0) * 1e6 / (pre + nsteps * (2 if warm else 1))
         g.close()
         lat.f, lat.f_alt, lat.parity = after[:3]
         lat.steps = after[3] + pre + (nsteps if warm else 0)
