@@ -48,9 +48,9 @@ TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), copy-
 # the base-window row (rw) in profiles/k_pair_window_fast_4096x1024.md.
 PAIR_TRAFFIC_PER_LAUNCH = {("f16", 12): 156.93e6, ("bf16", 12): 156.97e6, ("f32", 12): 311.19e6,
                            ("f16+fast", 12): 156.9e6, ("f16r", 12): 157.2e6, ("f16r+fast", 12): 157.1e6,
-                           ("f16rw+fast", 12): 160.75e6}
+                           ("f16rw+fast", 12): 160.4e6}
 PAIR_VALU_PER_CELL = {("f16", 12): 363.8, ("bf16", 12): 363.4, ("f32", 12): 376.9, ("f16+fast", 12): 311.0,
-                      ("f16r", 12): 397.9, ("f16r+fast", 12): 345.0, ("f16rw+fast", 12): 351.5}
+                      ("f16r", 12): 397.9, ("f16r+fast", 12): 347.0, ("f16rw+fast", 12): 352.9}
 # The same for the four-step tile kernel (FP32, per tile shape); profiles/k_quad_cylinder_4096x1024.md (fast
 # arithmetic: profiles/k_quad_fast_cylinder_4096x1024.md).
 QUAD_TRAFFIC_PER_LAUNCH = {("f32", 12): 327.2e6, ("f32", 14): 336.1e6, ("f32", 24): 345.5e6,
