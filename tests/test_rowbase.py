@@ -328,3 +328,57 @@ def test_window_argument_errors():
     p = _lib.params(16, 8, 1.5, row_base=base.ctypes.data)
     assert L.lbm_rebase_window(ctypes.byref(p), fake, O.F16, 0, fake, None) == -1
     assert b"window" in L.lbm_last_error()
+
+
+# ------------------------------------------------------------------ the two-step kernel's window-first tile order
+def _window_tile_rect(W, H, CH, window, CW=128):
+    """launch_pair_fmt's rectangle of tiles whose region [x0 - 1, x0 + CW] x [y0 - 1, y0 + CH] meets the window."""
+    nbx, nby = -(-W // CW), -(-H // CH)
+    wx0, wy0, ww, wh = window
+    bx0 = 0 if wx0 <= CW else (wx0 - 1) // CW
+    by0 = 0 if wy0 <= CH else (wy0 - 1) // CH
+    bx1, by1 = min(nbx - 1, (wx0 + ww) // CW), min(nby - 1, (wy0 + wh) // CH)
+    return bx0, by0, bx1 - bx0 + 1, by1 - by0 + 1
+
+
+def _pair_tile_order(b, nbx, nby, rect):
+    """pair_tile_order (lbm_pair_tile.h): block b -> tile (bx, by), the rectangle's tiles first."""
+    tx0, ty0, tw, th = rect
+    nb, nw = nbx * nby, tw * th
+    if b < nw:
+        return tx0 + b % tw, ty0 + b // tw
+    j, per = b - nw, (nb - nw) // 8
+    t = (j % 8) * per + j // 8 if j < per * 8 else j
+    above, rest = ty0 * nbx, nbx - tw
+    if t >= above:
+        k = t - above
+        if k < th * rest:
+            i = k % rest
+            return (i if i < tx0 else i + tw), ty0 + k // rest
+        t += nw
+    return t % nbx, t // nbx
+
+
+@pytest.mark.parametrize("W,H,CH", [(4096, 1024, 12), (300, 72, 8), (301, 90, 24), (131, 37, 16), (1024, 256, 12)])
+def test_window_first_tile_order_is_a_permutation(W, H, CH):
+    """The window's tiles (exactly those whose region meets it) come first, and the order is a permutation of all
+    tiles, for windows anywhere in the lattice (edges, corners, the whole interior) and without one."""
+    rng = np.random.default_rng(W + H + CH)
+    nbx, nby = -(-W // 128), -(-H // CH)
+    wins = [(1, 1, W - 3, H - 2), (1, 1, 1, 1), (W - 3, H - 2, 1, 1), (1, H // 2, W - 3, 1), (W // 2, 1, 1, H - 2)]
+    for _ in range(20):
+        x0, y0 = int(rng.integers(1, W - 2)), int(rng.integers(1, H - 1))
+        wins.append((x0, y0, int(rng.integers(1, W - 1 - x0)), int(rng.integers(1, H - y0))))
+    for win in wins:
+        rect = _window_tile_rect(W, H, CH, win)
+        meets = {(bx, by) for bx in range(nbx) for by in range(nby)
+                 if bx * 128 + 128 >= win[0] and bx * 128 - 1 < win[0] + win[2]
+                 and by * CH + CH >= win[1] and by * CH - 1 < win[1] + win[3]}
+        order = [_pair_tile_order(b, nbx, nby, rect) for b in range(nbx * nby)]
+        assert sorted(order) == [(bx, by) for bx in range(nbx) for by in range(nby)], win
+        assert set(order[:rect[2] * rect[3]]) == meets, win
+    # no window: the banded XCD order of xcd_tile
+    per = nbx * nby // 8
+    for b in range(nbx * nby):
+        t = (b % 8) * per + b // 8 if b < per * 8 else b
+        assert _pair_tile_order(b, nbx, nby, (0, 0, 0, 0)) == (t % nbx, t // nbx)
