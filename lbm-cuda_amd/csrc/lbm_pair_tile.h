@@ -53,8 +53,8 @@ template <int FMT, int CH> struct TileLds : TileBaseRows<row_based<FMT>(), CH + 
 // and the outlet cells x >= W - 2 (pair_interior), so every region cell is in the lattice, is updated by the
 // even step, is no inlet cell and feeds no outlet ghost, and no index wraps.  That tile-uniform fact removes
 // the per-cell boundary tests; the arithmetic is the same, so both forms give the same bits.
-// Row-based 16-bit formats (row_based<FMT>): the region rows' base values are staged in LDS for phase A (whose
-// cells' rows vary within a wave) and read into scalar registers for phase B (a wave covers half a core row); the
+// Row-based 16-bit formats (row_based<FMT>): the region rows' base values are staged in LDS, read per lane by phase A
+// (whose cells' rows vary within a wave) and with one address per wave by phase B (a wave covers half a core row); the
 // LDS copy of the even step's result is stored relative to the base as in HBM, so both phases round as the single
 // step does.  WIN (row-based formats, general tile form): the tile's region meets the base window (KPar::ww), whose
 // cells take their base values from their own velocity instead of their row: the region's packed window velocities
@@ -125,7 +125,8 @@ __device__ __forceinline__ void pair_tile(const KPar &P, const uint8_t *__restri
 
     // ---- phase A: even step of every region cell, reversed result into LDS.  A thread's HBM loads (mask byte
     //      + 9 populations per cell) are issued PD cells ahead of their use, and unconditionally, so their
-    //      latencies overlap and no load waits behind a mask test; PD = 4 bounds the registers they hold.
+    //      latencies overlap and no load waits behind a mask test; PD (4, or 2 for row-based formats) bounds the
+    //      registers they hold.
     constexpr int NA = (RH * RW + NT - 1) / NT;
     // row-based: two cells in flight, for the registers of the base values held across the collision (measured
     // on the 4096 x 1024 cylinder: 1.8-2.2 % faster than three cells with the values read again after it)
