@@ -50,25 +50,11 @@ def cylinder(W, H, R, cx, cy, re, cs, steps, seed, u_max=0.05, name="cylinder"):
 
 
 def wake_window(case, tile=(128, 12)):
-    """Base window (x0, y0, w, h) over a cylinder case's near wake, the region [cx, cx + 4 R) x [cy - 2 R, cy + 2 R)
-    where FP16 storage relative to the inlet profile keeps most of its error (at 4096 x 1024 and 10 000 steps, 9/10
-    of the squared velocity error lies in [960, 1472) x [384, 640), over half of it in the first 128 columns behind
-    the body).  The window is the inside of the whole tiles of the two-step kernel's tiling (`tile` = columns x
-    rows of a tile) that this region holds, less one cell at each side, so that only those tiles take the kernel's
-    per-cell form (a tile whose halo reaches a window cell takes it too); a region holding no whole tile (small
-    lattices) is taken as it is, clipped to the interior."""
+    """Base window (x0, y0, w, h) over a cylinder case's near wake (geometry.wake_window of its cylinder): at
+    4096 x 1024 and 10 000 steps, 9/10 of the FP16 row-based storage's squared velocity error lies in
+    [960, 1472) x [384, 640), over half of it in the first 128 columns behind the body."""
     cx, cy, R = (int(case.meta[k]) for k in ("cx", "cy", "R"))
-    tw, th = tile
-    rx0, rx1, ry0, ry1 = cx, cx + 4 * R, cy - 2 * R, cy + 2 * R
-    tx0, tx1 = -(-rx0 // tw), rx1 // tw
-    ty0, ty1 = -(-ry0 // th), ry1 // th
-    if tx1 > tx0 and ty1 > ty0:
-        x0, x1, y0, y1 = tx0 * tw + 1, tx1 * tw - 1, ty0 * th + 1, ty1 * th - 1
-    else:
-        x0, x1, y0, y1 = rx0, rx1, ry0, ry1
-    x0, y0 = max(1, x0), max(1, y0)
-    x1, y1 = min(case.W - 2, x1), min(case.H - 1, y1)
-    return (x0, y0, x1 - x0, y1 - y0)
+    return geo.wake_window(case.W, case.H, cx, cy, R, tile)
 
 
 def cylinder_small():

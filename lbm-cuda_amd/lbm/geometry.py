@@ -50,6 +50,42 @@ def channel_mask(W, H, cylinder=None):
     return finalize_channel_mask(m)
 
 
+def wake_window(W, H, cx, cy, R, tile=(128, 12)):
+    """Base window (x0, y0, w, h) over the near wake of a body of half-height R centred at (cx, cy) in a channel
+    flowing in +x: the region [cx, cx + 4 R) x [cy - 2 R, cy + 2 R), where FP16 storage relative to the inlet
+    profile keeps most of its error.  The window is the inside of the whole tiles of the two-step kernel's tiling
+    (`tile` = columns x rows of a tile) that this region holds, less one cell at each side, so that only those tiles
+    take the kernel's per-cell form (a tile whose halo reaches a window cell takes it too); a region holding no
+    whole tile (small lattices) is taken as it is.  Clipped to 1 <= x <= W - 3, 1 <= y <= H - 2."""
+    tw, th = tile
+    rx0, rx1, ry0, ry1 = cx, cx + 4 * R, cy - 2 * R, cy + 2 * R
+    tx0, tx1 = -(-rx0 // tw), rx1 // tw
+    ty0, ty1 = -(-ry0 // th), ry1 // th
+    if tx1 > tx0 and ty1 > ty0:
+        x0, x1, y0, y1 = tx0 * tw + 1, tx1 * tw - 1, ty0 * th + 1, ty1 * th - 1
+    else:
+        x0, x1, y0, y1 = rx0, rx1, ry0, ry1
+    x0, y0 = max(1, x0), max(1, y0)
+    x1, y1 = min(W - 2, x1), min(H - 1, y1)
+    if x1 <= x0 or y1 <= y0:
+        return None
+    return (x0, y0, x1 - x0, y1 - y0)
+
+
+def obstacle_wake_window(mask, tile=(128, 12)):
+    """wake_window of the bodies in a channel mask [H, W] (solid cells off the wall rows): centred on their
+    bounding box, R its half-height; None without a body."""
+    m = np.asarray(mask)
+    H, W = m.shape
+    ys, xs = np.nonzero(m[1:-1])
+    if ys.size == 0:
+        return None
+    ys = ys + 1
+    cx, cy = (int(xs.min()) + int(xs.max())) // 2, (int(ys.min()) + int(ys.max())) // 2
+    R = max(1, (int(ys.max()) - int(ys.min())) // 2)
+    return wake_window(W, H, cx, cy, R, tile)
+
+
 def periodic_mask(W, H):
     return np.zeros((H, W), np.uint8)
 

@@ -57,6 +57,7 @@ class Simulation:
     equilibrium of each cell's own velocity, re-derived from the state every ``rebase_every`` steps (0: only by
     ``rebase()``) -- for the region where the flow departs most from the rows' base (a near wake).  The window lies
     in 1 <= x <= W - 3, 1 <= y <= H - 2; ``rebase_every`` must be a multiple of the k of the launches used.
+    ``window="wake"``: the near wake of the channel's bodies (geometry.obstacle_wake_window of the mask).
 
     ``arith="fast"`` (MRT only): the collision uses the hardware's approximate reciprocal and square root (within 1
     ulp), as the reference's -use_fast_math build does; single steps and the tile kernels (k = 2, 4 from even
@@ -108,6 +109,12 @@ class Simulation:
         self.window = None
         self.rebase_every = int(rebase_every)
         self._since_rebase = 0
+        if isinstance(window, str):
+            if window != "wake":
+                raise ValueError(f"window {window!r}: expected 'wake' or (x0, y0, w, h)")
+            window = geo.obstacle_wake_window(mask)
+            if window is None:
+                raise ValueError("window='wake': the mask holds no body off the wall rows")
         if window is not None:
             if self.row_base_host is None:
                 raise ValueError("a base window needs 16-bit storage with row_base")

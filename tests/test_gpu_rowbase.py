@@ -338,3 +338,23 @@ def test_window_pair_cylinder_large(gpu):
         assert n == 0, f"after {2000 * (chunk + 1)} steps: {n} differ"
         assert torch.equal(a._wvel[0], b._wvel[0])
     assert np.isfinite(b.diagnostics()).all()
+
+
+def test_window_wake_keyword(gpu):
+    """Simulation(window="wake") takes the window of the mask's bodies (geometry.obstacle_wake_window); the
+    two-step kernel then equals single steps."""
+    from lbm.sim import Simulation
+    W, H = 300, 72
+    mask = G.channel_mask(W, H, cylinder=(W // 4, H // 2, 6))
+    prof = G.parabolic_profile(H, 0.05)
+    kw = dict(cs=0.12, mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, 6), storage="f16",
+              row_base="inlet", window="wake", rebase_every=4, arith="fast")
+    a = Simulation(W, H, 1.75, **kw)
+    b = Simulation(W, H, 1.75, **kw)
+    assert a.window == G.obstacle_wake_window(a.mask_host) == (75, 24, 24, 24)
+    sel = torch.from_numpy(O.observable(_op(a), a.mask_host, 0, ghost=True)).to(a.device)
+    a.step(8)
+    b.step(8, k=2, cfg=(0, 12))
+    assert int(((a.f != b.f) & sel).sum()) == 0
+    with pytest.raises(ValueError, match="no body"):
+        Simulation(W, H, 1.75, storage="f16", row_base="inlet", window="wake")

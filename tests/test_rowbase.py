@@ -382,3 +382,16 @@ def test_window_first_tile_order_is_a_permutation(W, H, CH):
     for b in range(nbx * nby):
         t = (b % 8) * per + b // 8 if b < per * 8 else b
         assert _pair_tile_order(b, nbx, nby, (0, 0, 0, 0)) == (t % nbx, t // nbx)
+
+
+def test_obstacle_wake_window():
+    """The wake window found from a channel mask's bodies equals the cylinder cases' own (4096 x 1024: the tile-
+    aligned [1025, 1279) x [385, 635)); a channel without a body has none; a block's window starts at its centre."""
+    from lbm import configs
+    for case in (configs.cylinder_large(), configs.cylinder_scaled(1024, 256), configs.cylinder_small()):
+        assert G.obstacle_wake_window(case.mask) == configs.wake_window(case), case.name
+    assert configs.wake_window(configs.cylinder_large()) == (1025, 385, 254, 250)
+    assert G.obstacle_wake_window(G.channel_mask(300, 64)) is None
+    m = G.channel_mask(300, 64)
+    m[20:31, 50:61] = 1  # an 11 x 11 block: centre (55, 25), half-height 5
+    assert G.obstacle_wake_window(m) == (55, 15, 20, 20)
