@@ -88,8 +88,8 @@ typedef struct lbm_params {
                               floats, 16-byte aligned (lbm_row_base).  Used by lbm_step_aa_h16(_n), lbm_step_fused_h16(_n)
                               (k = 2, two-step tile kernel), lbm_macro_h16 and lbm_convert_state; FP32 entry points ignore
                               it; lbm_convert and lbm_ensemble_create reject it */
-    const struct lbm_base_window *window; /* NULL, or a window of per-cell base values (needs row_base; see
-                                             lbm_base_window) */
+    const struct lbm_base_window *window; /* NULL, or a window of per-cell base velocities (needs row_base; see
+                                             lbm_base_window); taken by the entry points that take row_base */
 } lbm_params;
 
 /*
@@ -97,8 +97,8 @@ typedef struct lbm_params {
  * [x0, x0 + w) x [y0, y0 + h) take the base values of their slots from a velocity of their own instead of from
  * their row: `vel` is a device array of two IEEE halves (ux, uy) per window cell, row-major over the window,
  * 4-byte aligned (cell (x, y) at vel[2 ((y - y0) w + (x - x0))]), and the base value of direction q is the
- * equilibrium deviation at rho = 1, d_q = w_q ((3 cu + 4.5 cu^2) - 1.5 (ux^2 + uy^2)) with cu = c_q.u, in FP32 from
- * the half values (the table of lbm_row_base, computed per cell).  Where the flow departs from every per-row
+ * equilibrium deviation at rho = 1, d_q = w_q ((3 cu + 4.5 cu^2) - 1.5 (ux^2 + uy^2)) with cu = c_q.u, evaluated in
+ * FP32 from the half values (the equilibrium lbm_row_base tabulates per row in FP64, here per cell).  Where the flow departs from every per-row
  * profile -- behind a body -- the stored deviations, and their rounding, stay small only with a local base;
  * lbm_rebase_window re-derives it from the current flow.  The window must lie in
  * 1 <= x <= W - 3, 1 <= y <= H - 2: the edges, and the channel's last fluid column with its outlet ghost (a copy
