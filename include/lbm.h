@@ -188,7 +188,9 @@ typedef struct lbm_fused_cfg {
     int32_t chunk;
 } lbm_fused_cfg;
 
-/* The configuration lbm_step_fused uses when none is given. */
+/* The strip configuration lbm_step_fused uses when none is given (128-column strips, the tallest chunk that leaves
+ * at least 512 workgroups); from an even state with k = 2 (any storage, and row-based storage always) or k = 4
+ * (FP32 populations) it uses the tile kernels instead, {strip 0, chunk 12}. */
 int lbm_fused_default_cfg(const lbm_params *p, int k, lbm_fused_cfg *cfg);
 
 /* lbm_step_fused with an explicit configuration (NULL = default); used by the autotuner. */

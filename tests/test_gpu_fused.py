@@ -54,7 +54,8 @@ def test_fused_equals_single_small(gpu, k, periodic, model):
             _assert_equal(single, fusedsim, sel, f"k={k} launch {launch} start parity {start_parity}")
 
 
-@pytest.mark.parametrize("k,cfg", [(3, (64, 24)), (8, (64, 7)), (8, (128, 33)), (2, (64, 256)), (5, (128, 16))])
+@pytest.mark.parametrize("k,cfg", [(3, (64, 24)), (8, (64, 7)), (8, (128, 33)), (2, (64, 256)), (5, (128, 16)),
+                                   (2, (128, 32)), (4, (128, 32)), (4, (64, 40))])
 def test_fused_explicit_configs(gpu, k, cfg):
     single, fusedsim = _pair(301, 90, False, "mrt")
     sel = _sel(single)
