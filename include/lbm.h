@@ -166,8 +166,9 @@ int lbm_step_pull(const lbm_params *p, const uint8_t *mask, const float *f_now, 
                   float *uy, float *rho, const float *inlet_u, lbm_stream_t stream);
 
 /*
- * k collide-stream steps in one launch, temporally blocked in LDS (the fused_lbmkernel of
- * lbm_core_smemaa.cu:12-15 / lbm_core_smemaa_endfused.cu:12-14, with T = k chosen at run time).
+ * k collide-stream steps, temporally blocked in LDS (the fused_lbmkernel of lbm_core_smemaa.cu:12-15 /
+ * lbm_core_smemaa_endfused.cu:12-14, with T = k chosen at run time): one tile launch for k = 2 and 4 from an even
+ * state, otherwise a tile launch or a copy followed by single steps on f_out (lbm_fused_default_cfg).
  * f_now (parity `parity`) is read, f_out receives the state after k steps (parity (parity + k) & 1);
  * f_now and f_out must not overlap.  The result is bit-identical to k calls of lbm_step_aa.
  * 1 <= k <= lbm_fused_max_k().  ux/uy/rho: NULL or the macros of the last step's collision.
@@ -188,9 +189,12 @@ typedef struct lbm_fused_cfg {
     int32_t chunk;
 } lbm_fused_cfg;
 
-/* The strip configuration lbm_step_fused uses when none is given (128-column strips, the tallest chunk that leaves
- * at least 512 workgroups); from an even state with k = 2 (any storage, and row-based storage always) or k = 4
- * (FP32 populations) it uses the tile kernels instead, {strip 0, chunk 12}. */
+/* A strip configuration for k (128-column strips, the tallest chunk that leaves at least 512 workgroups), for callers
+ * that select the strip sweep.  lbm_step_fused without a configuration does not run the strip sweep: from an even
+ * state with k = 2 (any storage) or k = 4 (FP32 populations) it runs the tile kernels, {strip 0, chunk 12}; for
+ * other k and states one tile launch (even states, k >= 2) or a device copy of f_now to f_out (odd states, k = 1),
+ * then the remaining steps as single steps in place on f_out, bit-identical like every form and faster than the
+ * strip sweep at 4096 x 1024.  Such a call issues up to k launches (and a copy) on `stream`. */
 int lbm_fused_default_cfg(const lbm_params *p, int k, lbm_fused_cfg *cfg);
 
 /* lbm_step_fused with an explicit configuration (NULL = default); used by the autotuner. */

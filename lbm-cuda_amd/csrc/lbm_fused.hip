@@ -38,8 +38,8 @@ static int launch_any(const KPar &P, const lbm_params *p, int k, const lbm_fused
                                  uy, rho);
 }
 
-// Default launch configuration: 128-column strips, and the tallest chunk that still gives at least 512
-// workgroups (two per CU on 256 CUs) so the k-row warm-up at each chunk end stays small.
+// Strip configuration offered by lbm_fused_default_cfg: 128-column strips, and the tallest chunk that still gives
+// at least 512 workgroups (two per CU on 256 CUs) so the k-row warm-up at each chunk end stays small.
 static void default_cfg(int W, int H, int k, lbm_fused_cfg *c)
 {
     c->strip = 128;
@@ -51,10 +51,14 @@ static void default_cfg(int W, int H, int k, lbm_fused_cfg *c)
 }
 
 
+// Checks the arguments and picks the launch configuration c.  Without a configuration (cfg = NULL): the tile kernel
+// when one launch of it covers the k steps, otherwise `composed` (run_composed below); the strip sweep runs for an
+// explicit strip configuration only.
 static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_now, void *f_out, unsigned bytes,
                         const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
-                        float *rho, KPar &P, lbm_fused_cfg &c)
+                        float *rho, KPar &P, lbm_fused_cfg &c, bool &composed)
 {
+    composed = false;
     int rc = make_kpar(p, &P, bytes);
     if (rc) return rc;
     if (!mask || !f_now || !f_out) return fail(LBM_ERR_ARG, "lbm_step_fused: mask, f_now, f_out must be non-NULL");
@@ -68,19 +72,20 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
     if ((ux == nullptr) != (rho == nullptr) || (uy == nullptr) != (rho == nullptr))
         return fail(LBM_ERR_ARG, "lbm_step_fused: ux, uy, rho must be all NULL or all non-NULL");
     if (k < 1 || k > FUSED_MAXK) return fail(LBM_ERR_ARG, "lbm_step_fused: k=%d outside [1, %d]", k, FUSED_MAXK);
-    if (cfg)
+    if (cfg) {
         c = *cfg;
-    else if (P.base && bytes == 2 && k == 2)
-        c = lbm_fused_cfg{0, 12}; // row-based 16-bit storage: the two-step tile kernel is its blocked form
-    else if (parity == 0 && (k == 2 || (k == 4 && bytes == 4)))
+    } else if (parity == 0 && (k == 2 || (k == 4 && bytes == 4))) {
         c = lbm_fused_cfg{0, 12}; // from an even state the tile kernels: 2-3x the strip sweep's rate at 4096 x 1024
-    else
-        default_cfg(P.W, P.H, k, &c);
+    } else {
+        composed = true; // every form the single-step kernel takes
+        return LBM_OK;
+    }
     if (P.fast && c.strip != 0)
-        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: fast arithmetic runs the tile kernels (strip 0) only");
+        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused: fast arithmetic takes the tile kernels (strip 0) or no "
+                                         "configuration, not the strip sweep");
     if (P.base && bytes == 2 && !(c.strip == 0 && k == 2))
-        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: row-based storage (p->row_base) runs k = 2 with the "
-                                         "two-step tile kernel (strip 0) only");
+        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: row-based storage (p->row_base) takes the two-step "
+                                         "tile kernel (strip 0, k = 2) or no configuration, not the strip sweep");
     if (c.strip == 0) {
         if ((k != 2 && k != 4) || parity != 0)
             return fail(LBM_ERR_UNSUPPORTED,
@@ -99,6 +104,56 @@ static int fused_common(const lbm_params *p, const uint8_t *mask, const void *f_
     if (c.strip - 2 * (k + 1) < 16) return fail(LBM_ERR_ARG, "lbm_step_fused: strip %d too narrow for k=%d", c.strip, k);
     if (c.chunk > P.H) c.chunk = P.H;
     return LBM_OK;
+}
+
+// The first launch of lbm_step_fused_h16 for a configuration: the row-based formats run the two-step tile kernel
+template <int FMT, int FMT_R>
+static int launch_h16(const KPar &P, const lbm_params *p, int k, const lbm_fused_cfg &c, int parity, hipStream_t st,
+                      const uint8_t *mask, const void *fin, void *fout, const float *inlet_u, float *ux, float *uy,
+                      float *rho)
+{
+    if (P.base) return launch_pair_fmt<FMT_R>(P, p->model, p->bc, c.chunk, st, mask, fin, fout, inlet_u, ux, uy, rho);
+    return launch_any<FMT>(P, p, k, c, parity, st, mask, fin, fout, inlet_u, ux, uy, rho);
+}
+
+// k steps that no single tile launch covers, without a configuration: from an even state one tile launch f_now ->
+// f_out (the four-step kernel for FP32 populations with k >= 4, else the two-step one), from an odd state a device
+// copy of f_now to f_out; then the remaining steps in place on f_out with the single-step kernel, the last one with
+// the macros.  Each piece is bit-identical to its single steps, so the whole is.  At 4096 x 1024 (FP32) single
+// steps take 45 us and the copy 67 us, against 64-100 us per step for the strip sweep at k = 3..8 and 228 us at
+// k = 1; from an even state k = 6 runs at 31.5-33.7 us per step, k = 8 at 35.8.
+static int run_composed(const KPar &P, const lbm_params *p, const uint8_t *mask, const void *f_now, void *f_out,
+                        int fmt, const float *inlet_u, int parity, int k, float *ux, float *uy, float *rho,
+                        lbm_stream_t stream)
+{
+    const hipStream_t st = to_hip(stream);
+    const bool f32 = fmt == LBM_FMT_F32;
+    int done = 0, rc = LBM_OK;
+    if (parity == 0 && k >= 2) {
+        done = f32 && k >= 4 ? 4 : 2;
+        const lbm_fused_cfg t{0, 12};
+        float *mx = done == k ? ux : nullptr, *my = done == k ? uy : nullptr, *mr = done == k ? rho : nullptr;
+        if (f32)
+            rc = launch_any<FMT_F32>(P, p, done, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
+        else if (fmt == LBM_FMT_F16)
+            rc = launch_h16<FMT_F16, FMT_F16R>(P, p, done, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
+        else
+            rc = launch_h16<FMT_BF16, FMT_BF16R>(P, p, done, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
+        if (rc) return rc;
+        rc = check_launch("lbm_step_fused");
+    } else {
+        const size_t n = (size_t)(f32 ? 4 : 2) * 9 * (size_t)P.plane;
+        const hipError_t e = hipMemcpyAsync(f_out, f_now, n, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return fail((int)e, "lbm_step_fused: state copy: %s", hipGetErrorString(e));
+    }
+    for (; done < k && rc == LBM_OK; ++done) {
+        const bool last = done == k - 1;
+        float *mx = last ? ux : nullptr, *my = last ? uy : nullptr, *mr = last ? rho : nullptr;
+        const int par = (parity + done) & 1;
+        rc = f32 ? lbm_step_aa(p, mask, static_cast<float *>(f_out), inlet_u, par, mx, my, mr, stream)
+                 : lbm_step_aa_h16(p, mask, static_cast<uint16_t *>(f_out), fmt, inlet_u, par, mx, my, mr, stream);
+    }
+    return rc;
 }
 
 } // namespace lbm
@@ -124,8 +179,10 @@ int lbm_step_fused_cfg(const lbm_params *p, const uint8_t *mask, const float *f_
 {
     KPar P;
     lbm_fused_cfg c;
-    int rc = fused_common(p, mask, f_now, f_out, 4, inlet_u, parity, k, cfg, ux, uy, rho, P, c);
+    bool composed;
+    int rc = fused_common(p, mask, f_now, f_out, 4, inlet_u, parity, k, cfg, ux, uy, rho, P, c, composed);
     if (rc) return rc;
+    if (composed) return run_composed(P, p, mask, f_now, f_out, LBM_FMT_F32, inlet_u, parity, k, ux, uy, rho, stream);
     rc = launch_any<FMT_F32>(P, p, k, c, parity, to_hip(stream), mask, f_now, f_out, inlet_u, ux, uy, rho);
     if (rc) return rc;
     return check_launch("lbm_step_fused");
@@ -144,19 +201,18 @@ int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t 
     if (fmt != LBM_FMT_F16 && fmt != LBM_FMT_BF16) return fail(LBM_ERR_ARG, "lbm_step_fused_h16: bad format %d", fmt);
     KPar P;
     lbm_fused_cfg c;
-    int rc = fused_common(p, mask, h_now, h_out, 2, inlet_u, parity, k, cfg, ux, uy, rho, P, c);
+    bool composed;
+    int rc = fused_common(p, mask, h_now, h_out, 2, inlet_u, parity, k, cfg, ux, uy, rho, P, c, composed);
     if (rc) return rc;
+    if (composed) return run_composed(P, p, mask, h_now, h_out, fmt, inlet_u, parity, k, ux, uy, rho, stream);
     if (c.strip != 0 && c.strip != 128) // the 16-bit strip-sweep kernels are built for 128-column strips only
         return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: 16-bit storage takes strip 0 or 128, not %d", c.strip);
-    if (P.base) // row-based: the two-step tile kernel (checked in fused_common)
-        rc = fmt == LBM_FMT_F16 ? launch_pair_fmt<FMT_F16R>(P, p->model, p->bc, c.chunk, to_hip(stream), mask, h_now,
-                                                            h_out, inlet_u, ux, uy, rho)
-                                : launch_pair_fmt<FMT_BF16R>(P, p->model, p->bc, c.chunk, to_hip(stream), mask, h_now,
-                                                             h_out, inlet_u, ux, uy, rho);
-    else if (fmt == LBM_FMT_F16)
-        rc = launch_any<FMT_F16>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);
+    // row-based storage runs the two-step tile kernel (checked in fused_common)
+    if (fmt == LBM_FMT_F16)
+        rc = launch_h16<FMT_F16, FMT_F16R>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);
     else
-        rc = launch_any<FMT_BF16>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);
+        rc = launch_h16<FMT_BF16, FMT_BF16R>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy,
+                                             rho);
     if (rc) return rc;
     return check_launch("lbm_step_fused_h16");
 }

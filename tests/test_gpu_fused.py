@@ -54,6 +54,31 @@ def test_fused_equals_single_small(gpu, k, periodic, model):
             _assert_equal(single, fusedsim, sel, f"k={k} launch {launch} start parity {start_parity}")
 
 
+@pytest.mark.parametrize("k", [1, 3, 6, 8])
+def test_fused_default_fast_composed(gpu, k):
+    """FP32 populations with the fast arithmetic and no configuration, k steps that no single tile launch covers:
+    a four- or two-step launch (even states) or a copy (odd states, k = 1), then single steps in place, equal to
+    single fast steps bit for bit, from both parities and over two launches of one C call."""
+    from lbm.sim import Simulation
+    W, H = 301, 90
+    mask = G.channel_mask(W, H, cylinder=(70, 45, 9))
+    prof = G.parabolic_profile(H, 0.05)
+    kw = dict(cs=0.12, bc="channel", mask=mask, inlet_u=prof, f0=G.channel_initial_state(W, H, prof, 1e-3, 4),
+              arith="fast")
+    single, fusedsim = Simulation(W, H, 1.75, **kw), Simulation(W, H, 1.75, **kw)
+    sel = _sel(single)
+    for start in (0, 1):
+        if single.parity != start:
+            single.step(1)
+            fusedsim.step(1)
+        single.step(k)
+        fusedsim._fused(k)
+        _assert_equal(single, fusedsim, sel, f"k={k} from parity {start}")
+        single.step(2 * k)
+        fusedsim._fused_n(k, 2)
+        _assert_equal(single, fusedsim, sel, f"two launches of k={k}")
+
+
 @pytest.mark.parametrize("k,cfg", [(3, (64, 24)), (8, (64, 7)), (8, (128, 33)), (2, (64, 256)), (5, (128, 16)),
                                    (2, (128, 32)), (4, (128, 32)), (4, (64, 40))])
 def test_fused_explicit_configs(gpu, k, cfg):

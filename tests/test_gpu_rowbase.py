@@ -156,7 +156,7 @@ def test_rowbase_rejected_forms(gpu):
     lbm_convert (the base depends on the parity), ensembles."""
     sim = _sim("f16")
     with pytest.raises(_lib.LbmError, match="row-based"):
-        sim.step(4, k=4)
+        sim.step(4, k=4, cfg=(128, 32))
     with pytest.raises(_lib.LbmError, match="row-based"):
         sim.step(2, k=2, cfg=(128, 32))
     sim.step(1)
@@ -271,6 +271,44 @@ def test_window_pair_equals_single(gpu, storage, fmt, case, W, H, window, arith)
             b.step(4, k=2, cfg=(0, chunk))
             assert int(((a.f != b.f) & sel).sum()) == 0, f"chunk {chunk} launch {launch}"
             assert torch.equal(a._wvel[0], b._wvel[0])
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("arith", ["exact", "fast"])
+@pytest.mark.parametrize("window", [None, (60, 20, 90, 30)])
+def test_rowbase_fused_default_composed(gpu, storage, arith, window):
+    """lbm_step_fused_h16(_n) without a configuration where one tile launch does not cover the k steps (k = 3 and 4
+    from an even state: a two-step launch, then single steps in place; odd states and k = 1: a copy, then single
+    steps) equal single steps bit for bit, the last step's macros included."""
+    a = _wsim(storage, window=window, arith=arith)
+    b = _wsim(storage, window=window, arith=arith)
+    op = _op(a)
+    upd = torch.from_numpy(O.observable(op, a.mask_host, 0)[0]).to(a.device)
+
+    def same(what):
+        sel = torch.from_numpy(O.observable(op, a.mask_host, a.parity, ghost=True)).to(a.device)
+        assert a.parity == b.parity and a.steps == b.steps
+        assert int(((a.f != b.f) & sel).sum()) == 0, what
+
+    for k, macros in ((4, False), (3, True), (3, False), (4, True)):
+        a.step(k, macros=macros)
+        b.step(k, k=k, macros=macros)
+        same(f"k={k}")
+        if macros:
+            for x, y in ((a.rho, b.rho), (a.ux, b.ux), (a.uy, b.uy)):
+                assert torch.equal(x[upd], y[upd]), f"k={k} macros"
+    for start in (0, 1):  # k = 1 (a copy and one step), then two launches of three steps from one C call
+        if a.parity != start:
+            a.step(1)
+            b.step(1)
+        a.step(1, macros=True)
+        b._fused(1, True)
+        same(f"k=1 from parity {start}")
+        for x, y in ((a.rho, b.rho), (a.ux, b.ux), (a.uy, b.uy)):
+            assert torch.equal(x[upd], y[upd]), "k=1 macros"
+        a.step(6)
+        b.step(6, k=3)
+        same(f"two launches of k=3 from parity {start ^ 1}")
 
 
 def test_window_conversion_macros_and_checkpoint(gpu, tmp_path):

@@ -60,14 +60,12 @@ def test_row_base_argument_errors():
     assert rc == -1 and b"lbm_convert_state" in L.lbm_last_error()
     rc = L.lbm_convert_state(ctypes.byref(p), fake, _lib.FMT_F16, ctypes.c_void_p(0x200000), _lib.FMT_F32, 2, None)
     assert rc == -1 and b"parity" in L.lbm_last_error()
-    # the strip kernel and k != 2 have no base path
+    # the strip kernel has no base path (any k; without a configuration other k run as single steps)
     cfg = _lib.LbmFusedCfg(128, 8)
-    rc = L.lbm_step_fused_h16(ctypes.byref(p), fake, fake, ctypes.c_void_p(0x200000), O.F16, fake, 0, 2,
-                              ctypes.byref(cfg), None, None, None, None)
-    assert rc == -3 and b"row-based" in L.lbm_last_error()
-    rc = L.lbm_step_fused_h16(ctypes.byref(p), fake, fake, ctypes.c_void_p(0x200000), O.F16, fake, 0, 4, None, None,
-                              None, None, None)
-    assert rc == -3 and b"row-based" in L.lbm_last_error()
+    for k in (2, 4):
+        rc = L.lbm_step_fused_h16(ctypes.byref(p), fake, fake, ctypes.c_void_p(0x200000), O.F16, fake, 0, k,
+                                  ctypes.byref(cfg), None, None, None, None)
+        assert rc == -3 and b"row-based" in L.lbm_last_error()
     # ensembles: rejected while the member table is checked, before anything is allocated
     from lbm.ensemble import LbmInstance
     inst = (LbmInstance * 1)()
