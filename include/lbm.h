@@ -205,7 +205,9 @@ int lbm_step_fused_cfg(const lbm_params *p, const uint8_t *mask, const float *f_
 /*
  * nlaunch consecutive lbm_step_fused_cfg launches on `stream`, alternating between the two buffers: launch i
  * reads f_a (i even) or f_b (i odd), writes the other one, and starts at parity (parity + i k) & 1.  The state
- * ends in f_b when nlaunch is odd, in f_a otherwise.  Issued from C so that a long run costs no
+ * ends in f_b when nlaunch is odd, in f_a otherwise; the other buffer's contents are then unspecified (without a
+ * configuration a launch from an odd state with k >= 3 takes its first step in place on the buffer it reads, then
+ * a tile launch, instead of copying that buffer).  Issued from C so that a long run costs no
  * host-language overhead per launch; replaces the replay loop of the reference's captured graph of two fused
  * launches (lbm_smemaa_main.py:136-151,173-174).
  */

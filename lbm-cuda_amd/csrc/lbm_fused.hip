@@ -116,31 +116,49 @@ static int launch_h16(const KPar &P, const lbm_params *p, int k, const lbm_fused
     return launch_any<FMT>(P, p, k, c, parity, st, mask, fin, fout, inlet_u, ux, uy, rho);
 }
 
+// One single step in place (the single-step kernel of lbm_step_aa / lbm_step_aa_h16)
+static int single_step(const lbm_params *p, const uint8_t *mask, void *f, int fmt, const float *inlet_u, int parity,
+                       float *ux, float *uy, float *rho, lbm_stream_t stream)
+{
+    return fmt == LBM_FMT_F32 ? lbm_step_aa(p, mask, static_cast<float *>(f), inlet_u, parity, ux, uy, rho, stream)
+                              : lbm_step_aa_h16(p, mask, static_cast<uint16_t *>(f), fmt, inlet_u, parity, ux, uy, rho,
+                                                stream);
+}
+
 // k steps that no single tile launch covers, without a configuration: from an even state one tile launch f_now ->
 // f_out (the four-step kernel for FP32 populations with k >= 4, else the two-step one), from an odd state a device
 // copy of f_now to f_out; then the remaining steps in place on f_out with the single-step kernel, the last one with
-// the macros.  Each piece is bit-identical to its single steps, so the whole is.  At 4096 x 1024 (FP32) single
-// steps take 45 us and the copy 67 us, against 64-100 us per step for the strip sweep at k = 3..8 and 228 us at
-// k = 1; from an even state k = 6 runs at 31.5-33.7 us per step, k = 8 at 35.8.
-static int run_composed(const KPar &P, const lbm_params *p, const uint8_t *mask, const void *f_now, void *f_out,
-                        int fmt, const float *inlet_u, int parity, int k, float *ux, float *uy, float *rho,
-                        lbm_stream_t stream)
+// the macros.  When the caller lets f_now be overwritten (f_scratch = f_now, the launches of lbm_step_fused_n), an
+// odd state with k >= 3 takes its first step in place on f_now and the tile launch after it instead of the copy.
+// Each piece is bit-identical to its single steps, so the whole is.  At 4096 x 1024 (FP32) single steps take 45 us
+// and the copy 67 us, against 64-100 us per step for the strip sweep at k = 3..8 and 228 us at k = 1; from an even
+// state k = 6 runs at 31.5-33.7 us per step, k = 8 at 35.8.
+static int run_composed(const KPar &P, const lbm_params *p, const uint8_t *mask, const void *f_now, void *f_scratch,
+                        void *f_out, int fmt, const float *inlet_u, int parity, int k, float *ux, float *uy,
+                        float *rho, lbm_stream_t stream)
 {
     const hipStream_t st = to_hip(stream);
     const bool f32 = fmt == LBM_FMT_F32;
     int done = 0, rc = LBM_OK;
-    if (parity == 0 && k >= 2) {
-        done = f32 && k >= 4 ? 4 : 2;
+    if (parity == 1 && f_scratch == f_now && f_scratch && k >= 3) {
+        rc = single_step(p, mask, f_scratch, fmt, inlet_u, 1, nullptr, nullptr, nullptr, stream);
+        if (rc) return rc;
+        done = 1;
+    }
+    if (((parity + done) & 1) == 0 && k - done >= 2) {
+        const int kt = f32 && k - done >= 4 ? 4 : 2;
         const lbm_fused_cfg t{0, 12};
-        float *mx = done == k ? ux : nullptr, *my = done == k ? uy : nullptr, *mr = done == k ? rho : nullptr;
+        const bool last = done + kt == k;
+        float *mx = last ? ux : nullptr, *my = last ? uy : nullptr, *mr = last ? rho : nullptr;
         if (f32)
-            rc = launch_any<FMT_F32>(P, p, done, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
+            rc = launch_any<FMT_F32>(P, p, kt, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
         else if (fmt == LBM_FMT_F16)
-            rc = launch_h16<FMT_F16, FMT_F16R>(P, p, done, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
+            rc = launch_h16<FMT_F16, FMT_F16R>(P, p, kt, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
         else
-            rc = launch_h16<FMT_BF16, FMT_BF16R>(P, p, done, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
+            rc = launch_h16<FMT_BF16, FMT_BF16R>(P, p, kt, t, 0, st, mask, f_now, f_out, inlet_u, mx, my, mr);
         if (rc) return rc;
         rc = check_launch("lbm_step_fused");
+        done += kt;
     } else {
         const size_t n = (size_t)(f32 ? 4 : 2) * 9 * (size_t)P.plane;
         const hipError_t e = hipMemcpyAsync(f_out, f_now, n, hipMemcpyDeviceToDevice, st);
@@ -148,12 +166,40 @@ static int run_composed(const KPar &P, const lbm_params *p, const uint8_t *mask,
     }
     for (; done < k && rc == LBM_OK; ++done) {
         const bool last = done == k - 1;
-        float *mx = last ? ux : nullptr, *my = last ? uy : nullptr, *mr = last ? rho : nullptr;
-        const int par = (parity + done) & 1;
-        rc = f32 ? lbm_step_aa(p, mask, static_cast<float *>(f_out), inlet_u, par, mx, my, mr, stream)
-                 : lbm_step_aa_h16(p, mask, static_cast<uint16_t *>(f_out), fmt, inlet_u, par, mx, my, mr, stream);
+        rc = single_step(p, mask, f_out, fmt, inlet_u, (parity + done) & 1, last ? ux : nullptr, last ? uy : nullptr,
+                         last ? rho : nullptr, stream);
     }
     return rc;
+}
+
+// lbm_step_fused_cfg / lbm_step_fused_h16 (fmt = LBM_FMT_F32, F16 or BF16); f_scratch = f_now when f_now may be
+// overwritten (run_composed), else NULL
+static int fused_any(const lbm_params *p, const uint8_t *mask, const void *f_now, void *f_scratch, void *f_out,
+                     int fmt, const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
+                     float *rho, lbm_stream_t stream)
+{
+    KPar P;
+    lbm_fused_cfg c;
+    bool composed;
+    const bool f32 = fmt == LBM_FMT_F32;
+    int rc = fused_common(p, mask, f_now, f_out, f32 ? 4 : 2, inlet_u, parity, k, cfg, ux, uy, rho, P, c, composed);
+    if (rc) return rc;
+    if (composed) return run_composed(P, p, mask, f_now, f_scratch, f_out, fmt, inlet_u, parity, k, ux, uy, rho, stream);
+    const hipStream_t st = to_hip(stream);
+    if (f32) {
+        rc = launch_any<FMT_F32>(P, p, k, c, parity, st, mask, f_now, f_out, inlet_u, ux, uy, rho);
+        if (rc) return rc;
+        return check_launch("lbm_step_fused");
+    }
+    if (c.strip != 0 && c.strip != 128) // the 16-bit strip-sweep kernels are built for 128-column strips only
+        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: 16-bit storage takes strip 0 or 128, not %d", c.strip);
+    // row-based storage runs the two-step tile kernel (checked in fused_common)
+    if (fmt == LBM_FMT_F16)
+        rc = launch_h16<FMT_F16, FMT_F16R>(P, p, k, c, parity, st, mask, f_now, f_out, inlet_u, ux, uy, rho);
+    else
+        rc = launch_h16<FMT_BF16, FMT_BF16R>(P, p, k, c, parity, st, mask, f_now, f_out, inlet_u, ux, uy, rho);
+    if (rc) return rc;
+    return check_launch("lbm_step_fused_h16");
 }
 
 } // namespace lbm
@@ -177,21 +223,13 @@ int lbm_step_fused_cfg(const lbm_params *p, const uint8_t *mask, const float *f_
                        const float *inlet_u, int parity, int k, const lbm_fused_cfg *cfg, float *ux, float *uy,
                        float *rho, lbm_stream_t stream)
 {
-    KPar P;
-    lbm_fused_cfg c;
-    bool composed;
-    int rc = fused_common(p, mask, f_now, f_out, 4, inlet_u, parity, k, cfg, ux, uy, rho, P, c, composed);
-    if (rc) return rc;
-    if (composed) return run_composed(P, p, mask, f_now, f_out, LBM_FMT_F32, inlet_u, parity, k, ux, uy, rho, stream);
-    rc = launch_any<FMT_F32>(P, p, k, c, parity, to_hip(stream), mask, f_now, f_out, inlet_u, ux, uy, rho);
-    if (rc) return rc;
-    return check_launch("lbm_step_fused");
+    return fused_any(p, mask, f_now, nullptr, f_out, LBM_FMT_F32, inlet_u, parity, k, cfg, ux, uy, rho, stream);
 }
 
 int lbm_step_fused(const lbm_params *p, const uint8_t *mask, const float *f_now, float *f_out, const float *inlet_u,
                    int parity, int k, float *ux, float *uy, float *rho, lbm_stream_t stream)
 {
-    return lbm_step_fused_cfg(p, mask, f_now, f_out, inlet_u, parity, k, nullptr, ux, uy, rho, stream);
+    return fused_any(p, mask, f_now, nullptr, f_out, LBM_FMT_F32, inlet_u, parity, k, nullptr, ux, uy, rho, stream);
 }
 
 int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t *h_now, uint16_t *h_out, int fmt,
@@ -199,32 +237,18 @@ int lbm_step_fused_h16(const lbm_params *p, const uint8_t *mask, const uint16_t 
                        float *rho, lbm_stream_t stream)
 {
     if (fmt != LBM_FMT_F16 && fmt != LBM_FMT_BF16) return fail(LBM_ERR_ARG, "lbm_step_fused_h16: bad format %d", fmt);
-    KPar P;
-    lbm_fused_cfg c;
-    bool composed;
-    int rc = fused_common(p, mask, h_now, h_out, 2, inlet_u, parity, k, cfg, ux, uy, rho, P, c, composed);
-    if (rc) return rc;
-    if (composed) return run_composed(P, p, mask, h_now, h_out, fmt, inlet_u, parity, k, ux, uy, rho, stream);
-    if (c.strip != 0 && c.strip != 128) // the 16-bit strip-sweep kernels are built for 128-column strips only
-        return fail(LBM_ERR_UNSUPPORTED, "lbm_step_fused_h16: 16-bit storage takes strip 0 or 128, not %d", c.strip);
-    // row-based storage runs the two-step tile kernel (checked in fused_common)
-    if (fmt == LBM_FMT_F16)
-        rc = launch_h16<FMT_F16, FMT_F16R>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy, rho);
-    else
-        rc = launch_h16<FMT_BF16, FMT_BF16R>(P, p, k, c, parity, to_hip(stream), mask, h_now, h_out, inlet_u, ux, uy,
-                                             rho);
-    if (rc) return rc;
-    return check_launch("lbm_step_fused_h16");
+    return fused_any(p, mask, h_now, nullptr, h_out, fmt, inlet_u, parity, k, cfg, ux, uy, rho, stream);
 }
 
+// the buffer a launch reads holds the previous launch's output (or the caller's state): scratch for run_composed
 int lbm_step_fused_n(const lbm_params *p, const uint8_t *mask, float *f_a, float *f_b, const float *inlet_u,
                      int parity, int k, int nlaunch, const lbm_fused_cfg *cfg, lbm_stream_t stream)
 {
     if (nlaunch < 0) return fail(LBM_ERR_ARG, "lbm_step_fused_n: nlaunch must be >= 0");
     for (int i = 0; i < nlaunch; ++i) {
         float *src = i & 1 ? f_b : f_a, *dst = i & 1 ? f_a : f_b;
-        int rc = lbm_step_fused_cfg(p, mask, src, dst, inlet_u, (parity + i * k) & 1, k, cfg, nullptr, nullptr,
-                                    nullptr, stream);
+        int rc = fused_any(p, mask, src, src, dst, LBM_FMT_F32, inlet_u, (parity + i * k) & 1, k, cfg, nullptr,
+                           nullptr, nullptr, stream);
         if (rc) return rc;
     }
     return LBM_OK;
@@ -235,10 +259,11 @@ int lbm_step_fused_h16_n(const lbm_params *p, const uint8_t *mask, uint16_t *h_a
                          lbm_stream_t stream)
 {
     if (nlaunch < 0) return fail(LBM_ERR_ARG, "lbm_step_fused_h16_n: nlaunch must be >= 0");
+    if (fmt != LBM_FMT_F16 && fmt != LBM_FMT_BF16) return fail(LBM_ERR_ARG, "lbm_step_fused_h16_n: bad format %d", fmt);
     for (int i = 0; i < nlaunch; ++i) {
         uint16_t *src = i & 1 ? h_b : h_a, *dst = i & 1 ? h_a : h_b;
-        int rc = lbm_step_fused_h16(p, mask, src, dst, fmt, inlet_u, (parity + i * k) & 1, k, cfg, nullptr, nullptr,
-                                    nullptr, stream);
+        int rc = fused_any(p, mask, src, src, dst, fmt, inlet_u, (parity + i * k) & 1, k, cfg, nullptr, nullptr,
+                           nullptr, stream);
         if (rc) return rc;
     }
     return LBM_OK;

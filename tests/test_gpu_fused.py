@@ -52,6 +52,9 @@ def test_fused_equals_single_small(gpu, k, periodic, model):
             single.step(k)
             fusedsim.step(k, k=k)
             _assert_equal(single, fusedsim, sel, f"k={k} launch {launch} start parity {start_parity}")
+        single.step(4 * k)
+        fusedsim.step(4 * k, k=k)  # four launches from one C call (lbm_step_fused_n)
+        _assert_equal(single, fusedsim, sel, f"k={k} four launches of one call, start parity {start_parity}")
 
 
 @pytest.mark.parametrize("k", [1, 3, 6, 8])
