@@ -63,7 +63,8 @@ def time_config(sim, k, cfg, reps=10, bufs=None):
     if bufs is None:
         bufs = (sim.f.clone(), torch.empty_like(sim.f))
     a, b = bufs
-    c = _lib.LbmFusedCfg(int(cfg[0]), int(cfg[1]))
+    c = None if cfg is None or cfg[0] is None else _lib.LbmFusedCfg(int(cfg[0]), int(cfg[1]))
+    cp = None if c is None else ctypes.byref(c)  # (None, None): no configuration, the library's choice
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     par = sim.parity
 
@@ -71,10 +72,10 @@ def time_config(sim, k, cfg, reps=10, bufs=None):
         ptrs = (ctypes.c_void_p(sim.mask.data_ptr()), ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()))
         if sim.fmt == _lib.FMT_F32:
             rc = L.lbm_step_fused_cfg(ctypes.byref(sim.params), *ptrs, ctypes.c_void_p(sim.inlet_u.data_ptr()), parity,
-                                      k, ctypes.byref(c), None, None, None, st)
+                                      k, cp, None, None, None, st)
         else:
             rc = L.lbm_step_fused_h16(ctypes.byref(sim.params), *ptrs, sim.fmt, ctypes.c_void_p(sim.inlet_u.data_ptr()),
-                                      parity, k, ctypes.byref(c), None, None, None, st)
+                                      parity, k, cp, None, None, None, st)
         _lib.check(rc, "lbm_step_fused")
 
     reps += reps & 1  # an even number of launches returns the state to buffer a and to parity par
@@ -105,13 +106,18 @@ def candidates(sim, k):
     """(strip, chunk) pairs to time.  From an even state: the tile kernels only (strip 0, chunk = tile rows) --
     the two-step kernel for k = 2, the four-step kernel for k = 4 (FP32 populations) -- and nothing for other k:
     on 4096 x 1024 the strip sweep ran 61-85 us per step against 20-36 us for the tile kernels and 34-45 us for
-    single steps, so it cannot win there.  From an odd state (no tile kernel applies): strip-sweep shapes.
-    Row-based 16-bit storage runs the two-step kernel only."""
+    single steps, so it cannot win there.  From an odd state (no tile kernel applies): (None, None) -- no
+    configuration, where the library takes one in-place single step or a copy and then a tile launch and single
+    steps (64 us per step at k = 4 on 4096 x 1024 through lbm_step_fused_cfg, 42 us through lbm_step_fused_n) --
+    and the strip-sweep shapes (80 us per step at k = 4 there), which the fast arithmetic and row-based 16-bit
+    storage do not have."""
     H = sim.params.H
     if sim.fmt != _lib.FMT_F32 and sim.params.row_base:  # row-based 16-bit storage: the two-step kernel only
-        return [(0, ch) for ch in (8, 12, 16, 24)] if sim.parity == 0 and k == 2 else []
+        if sim.parity == 1:
+            return [(None, None)]
+        return [(0, ch) for ch in (8, 12, 16, 24)] if k == 2 else []
     if p_fast(sim) and sim.parity == 1:  # fast arithmetic: the tile kernels only
-        return []
+        return [(None, None)]
     if sim.parity == 0:
         if k == 2:
             return [(0, ch) for ch in (8, 12, 16, 24)]
@@ -126,7 +132,7 @@ def candidates(sim, k):
             if ch > 2 * H:
                 continue
             out.append((strip, min(ch, H)))
-    return sorted(set(out))
+    return [(None, None)] + sorted(set(out))
 
 
 def tune_fused(sim, ks=KS, use_cache=True, reps=10, verbose=False):

@@ -188,7 +188,8 @@ class Simulation:
     # ------------------------------------------------------------------ stepping
     def step(self, n=1, k=1, macros=False, cfg=None):
         """Advance n collide-stream steps; k > 1 uses the temporally blocked kernel (k steps per launch).
-        ``cfg`` = (strip, chunk) selects its launch configuration (default: the library's choice).  Runs of
+        ``cfg`` = (strip, chunk) selects its launch configuration (default, or (None, None) as the tuner
+        returns it: the library's choice).  Runs of
         several launches are issued by one C call (lbm_step_aa_n / lbm_step_fused_n); with ``macros`` only
         the last launch writes them.  With a base window, the run is cut at every ``rebase_every`` steps for a
         re-derivation of the window's base."""
@@ -242,7 +243,7 @@ class Simulation:
     def _fused_n(self, k, nl, cfg=None):
         if self.f_alt is None:
             self.f_alt = torch.empty_like(self.f)
-        c = None if cfg is None else ctypes.byref(_lib.LbmFusedCfg(int(cfg[0]), int(cfg[1])))
+        c = None if cfg is None or cfg[0] is None else ctypes.byref(_lib.LbmFusedCfg(int(cfg[0]), int(cfg[1])))
         if self.fmt == _lib.FMT_F32:
             rc = self.lib.lbm_step_fused_n(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f), _ptr(self.f_alt),
                                            _ptr(self.inlet_u), self.parity, k, nl, c, _stream())
@@ -273,7 +274,7 @@ class Simulation:
         if self.f_alt is None:
             self.f_alt = torch.empty_like(self.f)
         m = (self.ux, self.uy, self.rho) if macros else (None, None, None)
-        c = None if cfg is None else ctypes.byref(_lib.LbmFusedCfg(int(cfg[0]), int(cfg[1])))
+        c = None if cfg is None or cfg[0] is None else ctypes.byref(_lib.LbmFusedCfg(int(cfg[0]), int(cfg[1])))
         if self.fmt == _lib.FMT_F32:
             rc = self.lib.lbm_step_fused_cfg(ctypes.byref(self.params), _ptr(self.mask), _ptr(self.f),
                                              _ptr(self.f_alt), _ptr(self.inlet_u), self.parity, k, c, _ptr(m[0]),

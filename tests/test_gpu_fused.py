@@ -111,7 +111,8 @@ def test_autotuner_picks_a_valid_config(gpu, tmp_path, monkeypatch):
 
 
 def test_autotuner_candidates_by_parity(gpu):
-    """Even states: tile kernels only (k = 2; k = 4 for FP32), none for other k; odd states: strip shapes."""
+    """Even states: tile kernels only (k = 2; k = 4 for FP32), none for other k; odd states: the library's own
+    choice (None, None) and the strip shapes."""
     from lbm import autotune
     _, sim = _pair(512, 256, True, "mrt")
     assert sim.parity == 0
@@ -120,9 +121,15 @@ def test_autotuner_candidates_by_parity(gpu):
     assert autotune.candidates(sim, 6) == []
     sim.step(1)
     odd = autotune.candidates(sim, 4)
-    assert odd and all(s in (64, 128) for s, _ in odd)
+    assert odd[0] == (None, None) and len(odd) > 1 and all(s in (64, 128) for s, _ in odd[1:])
     best, table = autotune.tune_fused(sim, ks=(2, 4), use_cache=False, reps=2)
-    assert best[1] in (64, 128) and len(table) == 2
+    assert best[1] in (None, 64, 128) and len(table) == 2
+    single = _pair(512, 256, True, "mrt")[0]
+    single.step(1)
+    sel = _sel(single)
+    single.step(2 * best[0])
+    sim.step(2 * best[0], k=best[0], cfg=best[1:3])  # (None, None) runs without a configuration
+    _assert_equal(single, sim, sel, f"tuned {best}")
 
 
 def test_fused_macros_match_single(gpu):
